@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused per-pixel tracer on one MI355X.
+
+A step is one pass of the hot path over one camera: trace every pixel's
+Kerr null geodesic (FP64 RKF45, rtol 1e-8, <= 20000 steps), integrate the
+DIS optical depth and the 8-bin UHE transfer along it, and write the image.
+Workload: the 512x512 spin-0.99 / 80 deg / powerlaw_funnel_envelope /
+powerlaw_cutoff / funnel_wall camera (hadros_amd.workloads.c2_*).  Inputs are
+parameters (no HBM-resident input); outputs stay in HBM until the timed
+region ends.
+
+Prints ONE JSON line (rank 0).  `value` is rays/s of the plain FP64 tracer;
+the precision form (FP64 geodesics + FP32-Kahan tau/I) is reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rays/s on 1 MI355X, plain vs batched+surrogate, at ≤1e-3 rel err vs FP64"
+
+# Algorithmic work per unit (fixed constants; the operation count of the
+# analytic method, not of this implementation's instruction stream):
+#   per RKF45 attempt: 6 right-hand sides x F_RHS + F_COMB stage/err flops
+#   per emitted path point: F_PT (dl metric, redshift, rho, source) + n_bins x F_BIN
+ALGO = {"F_RHS": 69, "F_COMB": 390, "F_PT": 150, "F_BIN": 20}
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix), AMD datasheet
+PEAK_HBM_GBS = 8000.0
+
+
+def tracer_flops(attempts, points, nbins):
+    return attempts * (6 * ALGO["F_RHS"] + ALGO["F_COMB"]) + points * (ALGO["F_PT"] + nbins * ALGO["F_BIN"])
+
+
+def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0):
+    """Time the FP64 CPU oracle (same algorithm, OpenMP schedule(dynamic)) on a
+    band of image rows through the image centre, growing the band until it
+    takes >= min_seconds."""
+    from oracle import py_oracle
+    from hadros_amd import workloads
+
+    cam = cam_fn(nx=nx, ny=nx)
+    sc = scene_fn()
+    e, _, s = workloads.synthetic_sigma()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    rows = 4
+    while True:
+        i0 = nx // 2 - rows // 2
+        k0, k1 = i0 * nx, (i0 + rows) * nx
+        t0 = time.perf_counter()
+        py_oracle.trace_image(cam, sc, e, s, nthreads=threads, k0=k0, k1=k1)
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds or rows >= nx:
+            break
+        rows = min(nx, max(rows * 2, int(rows * min_seconds / max(dt, 1e-3) * 1.1)))
+    return {"value": (k1 - k0) / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{k1 - k0} rays = image rows [{i0},{i0 + rows}) of the {nx}x{nx} camera, "
+                      f"FP64 oracle (hadros_oracle, CFG mode), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the per-ray work has no "
+              "cross-GPU exchange, so N GPUs would each trace their own cameras.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from hadros_amd import abi, ffi, workloads
+
+    torch.cuda.set_device(0)
+    dev_name, cus, _ = ffi.device_info()
+    e, _, s = workloads.synthetic_sigma()
+    cam = workloads.c2_camera(nx=args.size, ny=args.size)
+    sc = workloads.c2_scene()
+    npix = cam.nx * cam.ny
+
+    def timed(precision):
+        t = ffi.Tracer(cam, sc, e, s, precision, args.block)
+        for _ in range(args.warmup):
+            t.run()
+            t.last_ms()
+        torch.cuda.synchronize()
+        kern = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            t.run()
+            kern.append(t.last_ms())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        img, st = t.fetch()
+        t.close()
+        return wall, kern, img, st
+
+    wall, kern, img, st = timed(abi.PREC_FP64)
+    ms_step = 1e3 * wall / args.steps
+    value = args.steps * npix / wall
+    kavg = float(np.mean(kern))
+    flops = tracer_flops(st.total_attempts, st.total_points, st.n_bins)
+    achieved = flops / (kavg * 1e-3) / 1e12
+
+    wall32, kern32, img32, st32 = timed(abi.PREC_FP32K)
+    m = (img.tau > 0) & (img.tau < 700)
+    err_tau = float(np.max(np.abs(img32.tau[m] - img.tau[m]) / img.tau[m])) if m.any() else 0.0
+    err_p = float(np.max(np.abs(img32.p - img.p)))
+    si = float(img.I.sum())
+    err_i = abs(float(img32.I.sum()) - si) / abs(si) if si != 0 else abs(float(img32.I.sum()))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain FP64 tracer, {cam.nx}x{cam.ny} camera, spin 0.99, incl 80 deg, "
+                               "powerlaw_funnel_envelope, powerlaw_cutoff 8 bins, funnel_wall, CFG mode rtol 1e-8",
+                   "rays_per_step": npix, "device": dev_name, "block": st.block_size},
+        "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix peak)", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": None, "kernel_ms": kavg, "algorithmic_flop_per_launch": flops,
+                     "points_per_launch": st.total_points, "attempts_per_launch": st.total_attempts},
+        "precision_form": {"value": args.steps * npix / wall32, "unit": "rays/s", "dtype": "f64 geodesic + f32 Kahan",
+                           "kernel_ms": float(np.mean(kern32)), "max_rel_tau_vs_fp64": err_tau,
+                           "max_abs_p_vs_fp64": err_p, "rel_sum_i_vs_fp64": err_i},
+        "image": {"captured": st.n_captured, "exhausted": st.n_exhausted, "mean_P": float(img.p.mean()),
+                  "sum_I": si, "points_per_ray": st.total_points / npix},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,105 @@
+// eval.hip -- point-wise device evaluations of each physics term, so that
+// every device function on the path can be checked against the oracle on its
+// own (rho, sigma, emissivity, rhs, metric).
+#include "kernels.h"
+
+namespace hd {
+
+__global__ void k_eval_rho(DevMedium m, const double* r, const double* th, double* out, long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = rho(m, r[k], th[k]);
+}
+
+__global__ void k_eval_sigma(const double* sig, int ns, double e_lo, double e_hi, const double* e, double* out,
+                             long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double x = e[k];
+    if (!(x > 0.0) || x < e_lo || x > e_hi) {
+        out[k] = __builtin_nan("");
+        return;
+    }
+    out[k] = exp(sigma_ln(sig, sig + ns, sig + 2 * ns, ns, log(x)));
+}
+
+// emissivity_uhe = (norm * spatial) * S(E) (src/radiative_transfer.cpp:369-395)
+__global__ void k_eval_emis(DevRt P, const double* r, const double* th, const double* e, double* out, long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double E = e[k];
+    double j = 0.0;
+    if (E > 0.0 && P.spec_ok) {
+        const double A = source_factor(P.src, P.med, r[k], th[k]);
+        j = A * exp(-P.gam * log(E) - E * P.invcut);
+    }
+    out[k] = j;
+}
+
+__global__ void k_eval_rhs(double a, int mode, const double* y, double* dy, long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double* s = y + 8 * k;
+    const Y5 yy{s[1], s[2], s[3], s[5], s[6]};
+    const Y5 d = (mode == HADROS_MODE_CFG) ? rhs_cfg(a, yy, s[4], s[7]) : rhs_ref(a, yy, s[4], s[7]);
+    double* o = dy + 8 * k;
+    // dt/dlambda = g^tt p_t + g^tphi p_phi
+    double sn, cs;
+    sincos(s[2], &sn, &cs);
+    const Metric5 g = inv_metric(a, s[1], sn, cs);
+    o[0] = g.tt * s[4] + g.tph * s[7];
+    o[1] = d.r;
+    o[2] = d.th;
+    o[3] = d.ph;
+    o[4] = 0.0;
+    o[5] = d.pr;
+    o[6] = d.pth;
+    o[7] = 0.0;
+}
+
+__global__ void k_eval_metric(double a, const double* r, const double* th, double* out, long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Metric5 g = metric(a, r[k], th[k]);
+    double sn, cs;
+    sincos(th[k], &sn, &cs);
+    const Metric5 gi = inv_metric(a, r[k], sn, cs);
+    double* o = out + 10 * k;
+    o[0] = g.tt; o[1] = g.tph; o[2] = g.rr; o[3] = g.thth; o[4] = g.phph;
+    o[5] = gi.tt; o[6] = gi.tph; o[7] = gi.rr; o[8] = gi.thth; o[9] = gi.phph;
+}
+
+static inline int blocks(long n) { return int((n + 255) / 256); }
+
+hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th, double* out, long n)
+{
+    hipLaunchKernelGGL(k_eval_rho, dim3(blocks(n)), dim3(256), 0, 0, m, r, th, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_eval_sigma(const double* sig, int ns, double e_lo, double e_hi, const double* e, double* out,
+                             long n)
+{
+    hipLaunchKernelGGL(k_eval_sigma, dim3(blocks(n)), dim3(256), 0, 0, sig, ns, e_lo, e_hi, e, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_eval_emis(const DevRt& P, const double* r, const double* th, const double* e, double* out, long n)
+{
+    hipLaunchKernelGGL(k_eval_emis, dim3(blocks(n)), dim3(256), 0, 0, P, r, th, e, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_eval_rhs(double a, int mode, const double* y, double* dy, long n)
+{
+    hipLaunchKernelGGL(k_eval_rhs, dim3(blocks(n)), dim3(256), 0, 0, a, mode, y, dy, n);
+    return hipGetLastError();
+}
+hipError_t launch_eval_metric(double a, const double* r, const double* th, double* out, long n)
+{
+    hipLaunchKernelGGL(k_eval_metric, dim3(blocks(n)), dim3(256), 0, 0, a, r, th, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace hd

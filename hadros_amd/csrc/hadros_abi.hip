@@ -1,0 +1,974 @@
+// hadros_abi.hip -- host side of the C-ABI declared in include/hadros.h.
+//
+// Parameter normalisation (the reference's constructor clamps and per-scene
+// constants) happens here on the host, with the host libm, so that the values
+// every lane starts from are the ones the reference computes; the per-ray
+// work runs in the kernels of tracer.hip / replay.hip.
+#include "kernels.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+using namespace hd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPC(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(HADROS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+constexpr double C_CGS = 2.99792458e10;   // include/constants.hpp:10-12
+constexpr double G_CGS = 6.67430e-8;
+constexpr double MSUN_G = 1.98847e33;
+
+int g_device_ok = -1;   // -1 unknown, 0 missing, 1 ok
+std::string g_device_msg;
+
+int ensure_device()
+{
+    if (g_device_ok == 1) return HADROS_OK;
+    if (g_device_ok == 0) return fail(HADROS_ERR_DEVICE, g_device_msg);
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        g_device_ok = 0;
+        g_device_msg = "no HIP device available (hadros requires an MI355X / gfx950 GPU; there is no CPU path)";
+        return fail(HADROS_ERR_DEVICE, g_device_msg);
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_device_ok = 0;
+        g_device_msg = std::string("device is not gfx950: ") + prop.gcnArchName;
+        return fail(HADROS_ERR_DEVICE, g_device_msg);
+    }
+    g_device_ok = 1;
+    return HADROS_OK;
+}
+
+// ---------------------------------------------------------------- params
+DevMedium make_medium(const hadros_medium& m)
+{
+    DevMedium d{};
+    d.kind = m.kind;
+    d.powerlaw_disk = (m.kind == HADROS_PROFILE_POWERLAW || m.kind == HADROS_PROFILE_POWERLAW_FUNNEL ||
+                       m.kind == HADROS_PROFILE_POWERLAW_ENVELOPE || m.kind == HADROS_PROFILE_POWERLAW_FUNNEL_ENVELOPE);
+    d.funnel = (m.kind == HADROS_PROFILE_GAUSSIAN_FUNNEL || m.kind == HADROS_PROFILE_POWERLAW_FUNNEL ||
+                m.kind == HADROS_PROFILE_POWERLAW_FUNNEL_ENVELOPE);
+    d.envelope = (m.kind == HADROS_PROFILE_GAUSSIAN_ENVELOPE || m.kind == HADROS_PROFILE_POWERLAW_ENVELOPE ||
+                  m.kind == HADROS_PROFILE_POWERLAW_FUNNEL_ENVELOPE);
+    // TorusProfile constructor clamps (src/torus_profile.cpp:12-42)
+    d.rho0 = m.rho0;
+    d.r0 = m.r0;
+    d.sr = m.sigma_r;
+    d.H = m.h_over_r;
+    d.p = m.radial_power;
+    d.fd = dclamp(m.funnel_depletion, 0.0, 1.0);
+    d.ft = dmax(m.funnel_theta_rad, 1.0e-6);
+    d.env0 = dmax(m.envelope_rho0, 0.0);
+    d.ealpha = m.envelope_alpha;
+    d.rmin = m.r_min;
+    d.rmax = dmax(m.r_max, m.r_min);
+    d.floor_ = dmax(m.rho_floor, 0.0);
+    return d;
+}
+
+int check_kinds(const hadros_scene& s)
+{
+    if (s.medium.kind < 0 || s.medium.kind > 7) return fail(HADROS_ERR_KIND, "unknown density profile kind");
+    if (s.source.kind < 0 || s.source.kind > 4) return fail(HADROS_ERR_KIND, "unknown UHE source kind");
+    if (s.spectrum.kind < 0 || s.spectrum.kind > 2) return fail(HADROS_ERR_KIND, "unknown UHE spectral kind");
+    return HADROS_OK;
+}
+
+// uhe_spectral_weight (src/radiative_transfer.cpp:172-207), host side for the bin weights
+double spectral_weight_host(const hadros_scene& s, double E)
+{
+    if (E <= 0.0) return 0.0;
+    const double safe_E = dmax(E, 1.0e-300);
+    if (s.spectrum.kind == HADROS_SPECTRUM_MONOCHROMATIC) {
+        if (s.source.emax_GeV <= 0.0) return 0.0;
+        return std::pow(safe_E, -s.source.powerlaw) * std::exp(-safe_E / s.source.emax_GeV);
+    }
+    if (s.spectrum.kind == HADROS_SPECTRUM_POWERLAW) return std::pow(safe_E, -s.spectrum.gamma);
+    if (s.spectrum.kind == HADROS_SPECTRUM_POWERLAW_CUTOFF) {
+        if (s.spectrum.ecut_GeV <= 0.0) return 0.0;
+        return std::pow(safe_E, -s.spectrum.gamma) * std::exp(-safe_E / s.spectrum.ecut_GeV);
+    }
+    return 0.0;
+}
+
+int check_sigma(const double* e, const double* s, int n)
+{
+    if (!e || !s) return fail(HADROS_ERR_ARG, "sigma table pointer is NULL");
+    if (n < 2) return fail(HADROS_ERR_TABLE, "Sigma table must contain at least two valid data lines.");
+    if (n > HADROS_MAX_SIGMA) return fail(HADROS_ERR_ARG, "sigma table longer than HADROS_MAX_SIGMA");
+    for (int k = 0; k < n; ++k)
+        if (!(e[k] > 0.0) || !(s[k] > 0.0)) return fail(HADROS_ERR_TABLE, "sigma table has non-positive entries");
+    for (int k = 1; k < n; ++k)
+        if (e[k] <= e[k - 1]) return fail(HADROS_ERR_TABLE, "Energy grid in sigma table must be strictly increasing.");
+    return HADROS_OK;
+}
+
+// DevRt of one scene: integrate_kerr_ray_spectral's bins (src/radiative_transfer.cpp:687-761),
+// spectral factor, r_g, horizon of the transfer, sigma range.
+int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, double a_bh, DevRt& P)
+{
+    int rc = check_kinds(s);
+    if (rc) return rc;
+    rc = check_sigma(se, ss, n);
+    if (rc) return rc;
+    P = DevRt{};
+    P.med = make_medium(s.medium);
+    const hadros_source& so = s.source;
+    P.src.kind = so.kind;
+    P.src.ring_ok = !(so.r_center <= 0.0 || so.sigma_r <= 0.0 || so.theta_width_rad <= 0.0 || so.emax_GeV <= 0.0);
+    P.src.rc = so.r_center;
+    P.src.sr = so.sigma_r;
+    P.src.tw = so.theta_width_rad;
+    P.src.norm = so.norm;
+    P.src.ft = so.funnel_theta_rad;
+    P.src.q = so.density_q;
+    P.src.s = so.radial_s;
+    P.src.rho_ref = so.rho_ref;
+    P.src.cmin = so.cutoff_min;
+    P.src.cmax = so.cutoff_max;
+    P.src.gdr = so.gradient_dr;
+    P.src.gdth = so.gradient_dtheta_rad;
+    P.src.rho_norm_c = rho(P.med, so.r_center, 0.5 * hd::PI);
+    P.n_sigma = n;
+    P.e_lo = se[0];
+    P.e_hi = se[n - 1];
+    P.rg_cm = G_CGS * s.m_bh_msun * MSUN_G / (C_CGS * C_CGS);
+    const double a = dclamp(a_bh, 0.0, 0.999999);
+    P.r_h_rt = 1.0 + std::sqrt(1.0 - a * a);
+    const hadros_spectrum& sp = s.spectrum;
+    P.mono = sp.kind == HADROS_SPECTRUM_MONOCHROMATIC;
+    if (P.mono) {
+        P.gam = so.powerlaw;
+        P.spec_ok = so.emax_GeV > 0.0;
+        P.invcut = P.spec_ok ? 1.0 / so.emax_GeV : 0.0;
+        P.nb = 1;
+        P.emid[0] = s.enu_obs_GeV;
+        P.lnemid[0] = std::log(s.enu_obs_GeV);
+        P.de[0] = 1.0;
+        P.w[0] = 1.0;
+        return HADROS_OK;
+    }
+    P.gam = sp.gamma;
+    if (sp.kind == HADROS_SPECTRUM_POWERLAW) {
+        P.spec_ok = 1;
+        P.invcut = 0.0;
+    } else {
+        P.spec_ok = sp.ecut_GeV > 0.0;
+        P.invcut = P.spec_ok ? 1.0 / sp.ecut_GeV : 0.0;
+    }
+    const int n_bins = std::max(sp.n_bins, 1);
+    if (n_bins > HADROS_MAX_BINS) return fail(HADROS_ERR_ARG, "n_bins exceeds HADROS_MAX_BINS (16)");
+    const double e_min = dmax(sp.e_min_GeV, se[0]);
+    const double e_max = dmin(sp.e_max_GeV, se[n - 1]);
+    P.nb = 0;
+    if (e_min <= 0.0 || e_max <= e_min) return HADROS_OK;   // RTResult{} for every ray
+    const double log_min = std::log(e_min);
+    const double log_max = std::log(e_max);
+    const double dlog = (log_max - log_min) / static_cast<double>(n_bins);
+    for (int i = 0; i < n_bins; ++i) {
+        const double log_left = log_min + dlog * static_cast<double>(i);
+        const double log_right = log_left + dlog;
+        const double E_mid = std::exp(0.5 * (log_left + log_right));
+        const double dE = std::exp(log_right) - std::exp(log_left);
+        const double w = spectral_weight_host(s, E_mid) * dE;
+        if (w <= 0.0 || !std::isfinite(w)) continue;
+        P.emid[P.nb] = E_mid;
+        P.lnemid[P.nb] = std::log(E_mid);
+        P.de[P.nb] = dE;
+        P.w[P.nb] = w;
+        ++P.nb;
+    }
+    return HADROS_OK;
+}
+
+void sigma_tables(const double* se, const double* ss, int n, std::vector<double>& d, std::vector<float>& f)
+{
+    d.assign(3 * size_t(n), 0.0);
+    f.assign(3 * size_t(n), 0.0f);
+    for (int k = 0; k < n; ++k) {
+        d[k] = std::log(se[k]);
+        d[n + k] = std::log(ss[k]);
+    }
+    for (int k = 1; k < n; ++k) d[2 * n + k] = (d[n + k] - d[n + k - 1]) / (d[k] - d[k - 1]);
+    for (size_t k = 0; k < d.size(); ++k) f[k] = float(d[k]);
+}
+
+// KerrMetric on the host (src/kerr_metric.cpp:10-144) for the observer constants.
+void host_metric(double a, double r, double th, double g[5], double& alpha, double& omega)
+{
+    const double c = std::cos(th);
+    const double sig = r * r + a * a * c * c;
+    const double del = r * r - 2.0 * r + a * a;
+    const double s = std::sin(th);
+    const double s2 = s * s;
+    const double bigA = (r * r + a * a) * (r * r + a * a) - a * a * del * s2;
+    g[0] = -(1.0 - 2.0 * r / sig);
+    g[1] = -2.0 * a * r * s2 / sig;
+    g[2] = sig / del;
+    g[3] = sig;
+    g[4] = (r * r + a * a + 2.0 * a * a * r * s2 / sig) * s2;
+    alpha = std::sqrt(sig * del / bigA);
+    omega = 2.0 * a * r / bigA;
+}
+
+int make_cam(const hadros_camera& c, DevCam& d)
+{
+    if (c.nx <= 0 || c.ny <= 0) return fail(HADROS_ERR_ARG, "camera nx, ny must be positive");
+    if (c.mode != HADROS_MODE_REF && c.mode != HADROS_MODE_CFG) return fail(HADROS_ERR_ARG, "unknown integrator mode");
+    if (!(c.h > 0.0) || !(c.tol > 0.0) || c.max_steps <= 0)
+        return fail(HADROS_ERR_ARG, "camera h, tol and max_steps must be positive");
+    d = DevCam{};
+    d.a = c.a_spin;
+    d.r_obs = c.r_obs;
+    d.th_obs = c.theta_obs_rad;
+    const double fov_rad = c.fov_deg * hd::PI / 180.0;   // src/kerr_camera.cpp:78-81
+    d.tanhalf = std::tan(0.5 * fov_rad);
+    d.r_max = c.r_max;
+    d.h = c.h;
+    d.tol = c.tol;
+    d.nx = c.nx;
+    d.ny = c.ny;
+    d.mode = c.mode;
+    d.max_steps = c.max_steps;
+    double g[5];
+    host_metric(c.a_spin, c.r_obs, c.theta_obs_rad, g, d.alpha, d.omega);
+    d.g_tt = g[0];
+    d.g_tph = g[1];
+    d.g_rr = g[2];
+    d.g_thth = g[3];
+    d.g_phph = g[4];
+    d.r_h = 1.0 + std::sqrt(1.0 - c.a_spin * c.a_spin);
+    return HADROS_OK;
+}
+
+template <class T>
+int dalloc(T** p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return HADROS_OK;
+    HIPC(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+    return HADROS_OK;
+}
+
+template <class T>
+void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+// ================================================================ plans
+struct hadros_tracer {
+    TraceArgs args{};
+    int precision = 0;
+    int block = 0;
+    long npix = 0;
+    double *d_sig = nullptr;
+    float* d_sigf = nullptr;
+    double *d_tau = nullptr, *d_p = nullptr, *d_I = nullptr, *d_alpha = nullptr, *d_beta = nullptr;
+    double *d_bt = nullptr, *d_bp = nullptr, *d_bi = nullptr;
+    int *d_cap = nullptr, *d_np = nullptr, *d_flags = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ran = false;
+};
+
+struct hadros_replay {
+    ReplayArgs args{};
+    int precision = 0;
+    long n_rays = 0, n_points = 0;
+    double* d_sig = nullptr;
+    float* d_sigf = nullptr;
+    long* d_off = nullptr;
+    double *d_r = nullptr, *d_th = nullptr, *d_dl = nullptr, *d_z = nullptr;
+    double *d_tau = nullptr, *d_p = nullptr, *d_I = nullptr, *d_bt = nullptr, *d_bp = nullptr, *d_bi = nullptr;
+    unsigned long long* d_cnt = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ran = false;
+};
+
+extern "C" {
+
+const char* hadros_last_error(void) { return g_err.c_str(); }
+int hadros_abi_version(void) { return HADROS_ABI_VERSION; }
+
+int hadros_device_info(char* name, int name_cap, int* cu_count, int* gcn_arch)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(HADROS_ERR_DEVICE, "no HIP device available");
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, dev));
+    if (name && name_cap > 0) std::snprintf(name, size_t(name_cap), "%s (%s)", prop.name, prop.gcnArchName);
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (gcn_arch) *gcn_arch = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 950 : 0;
+    return HADROS_OK;
+}
+
+void hadros_camera_defaults(hadros_camera* c)
+{
+    if (!c) return;
+    c->a_spin = 0.9;
+    c->r_obs = 60.0;
+    c->theta_obs_rad = M_PI * 74.0 / 180.0;
+    c->fov_deg = 45.0;
+    c->nx = 100;
+    c->ny = 100;
+    c->r_max = 120.0;
+    c->h = 0.001;
+    c->tol = 1.0e-6;
+    c->mode = HADROS_MODE_REF;
+    c->max_steps = 200000;
+}
+
+void hadros_scene_defaults(hadros_scene* s)
+{
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    hadros_medium& m = s->medium;
+    m.kind = HADROS_PROFILE_GAUSSIAN;
+    m.rho0 = 1.0e10;
+    m.r0 = 10.0;
+    m.sigma_r = 5.0;
+    m.h_over_r = 0.01;
+    m.radial_power = 2.0;
+    m.funnel_depletion = 0.0;
+    m.funnel_theta_rad = 15.0 * M_PI / 180.0;
+    m.envelope_rho0 = 0.0;
+    m.envelope_alpha = 2.5;
+    m.r_min = 4.0;
+    m.r_max = 60.0;
+    m.rho_floor = 1.0e-99;
+    hadros_source& so = s->source;
+    so.kind = HADROS_SOURCE_INNER_RING;
+    so.r_center = 3.5;
+    so.sigma_r = 1.0;
+    so.theta_width_rad = 8.0 * 3.141592653589793238462643383279502884 / 180.0;
+    so.powerlaw = 2.0;
+    so.emax_GeV = 1.0e12;
+    so.norm = 1.0;
+    so.funnel_theta_rad = 20.0 * 3.141592653589793238462643383279502884 / 180.0;
+    so.density_q = 1.0;
+    so.radial_s = 2.0;
+    so.rho_ref = -1.0;
+    so.cutoff_min = 0.0;
+    so.cutoff_max = 1.0e2;
+    so.gradient_dr = 0.1;
+    so.gradient_dtheta_rad = 1.0 * 3.141592653589793238462643383279502884 / 180.0;
+    hadros_spectrum& sp = s->spectrum;
+    sp.kind = HADROS_SPECTRUM_MONOCHROMATIC;
+    sp.n_bins = 8;
+    sp.gamma = 2.0;
+    sp.ecut_GeV = 1.0e12;
+    sp.e_min_GeV = 1.0e5;
+    sp.e_max_GeV = 1.0e12;
+    s->enu_obs_GeV = 1.0e9;
+    s->m_bh_msun = 3.0;
+}
+
+int hadros_parse_profile(const char* name, int32_t* kind)
+{
+    if (!name || !kind) return fail(HADROS_ERR_ARG, "NULL argument");
+    const std::string n(name);
+    int k = -1;
+    if (n == "gaussian" || n == "torus" || n == "gaussian_torus") k = 0;
+    else if (n == "powerlaw" || n == "powerlaw_disk") k = 1;
+    else if (n == "funnel" || n == "gaussian_funnel") k = 2;
+    else if (n == "powerlaw_funnel") k = 3;
+    else if (n == "gaussian_envelope") k = 4;
+    else if (n == "powerlaw_envelope") k = 5;
+    else if (n == "powerlaw_funnel_envelope") k = 6;
+    else if (n == "collapsar_ndaf_like" || n == "collapsar_ndaf") k = 7;
+    if (k < 0)
+        return fail(HADROS_ERR_KIND, "Unknown density profile '" + n +
+                                         "'. Use gaussian, powerlaw, gaussian_funnel, powerlaw_funnel, "
+                                         "gaussian_envelope, powerlaw_envelope, powerlaw_funnel_envelope, "
+                                         "or collapsar_ndaf_like.");
+    *kind = k;
+    return HADROS_OK;
+}
+
+int hadros_parse_source(const char* name, int32_t* kind)
+{
+    if (!name || !kind) return fail(HADROS_ERR_ARG, "NULL argument");
+    const std::string n(name);
+    int k = -1;
+    if (n == "inner_ring" || n == "INNER_RING" || n == "ring") k = 0;
+    else if (n == "funnel_wall" || n == "FUNNEL_WALL") k = 1;
+    else if (n == "jet_base" || n == "JET_BASE") k = 2;
+    else if (n == "shock_layer" || n == "SHOCK_LAYER") k = 3;
+    else if (n == "density_weighted" || n == "DENSITY_WEIGHTED") k = 4;
+    if (k < 0)
+        return fail(HADROS_ERR_KIND, "Unknown UHE source model '" + n +
+                                         "'. Use inner_ring, funnel_wall, jet_base, shock_layer, or density_weighted.");
+    *kind = k;
+    return HADROS_OK;
+}
+
+int hadros_parse_spectrum(const char* name, int32_t* kind)
+{
+    if (!name || !kind) return fail(HADROS_ERR_ARG, "NULL argument");
+    const std::string n(name);
+    int k = -1;
+    if (n == "monochromatic" || n == "MONOCHROMATIC" || n == "mono") k = 0;
+    else if (n == "powerlaw" || n == "POWERLAW" || n == "power_law") k = 1;
+    else if (n == "powerlaw_cutoff" || n == "POWERLAW_CUTOFF" || n == "power_law_cutoff") k = 2;
+    if (k < 0)
+        return fail(HADROS_ERR_KIND,
+                    "Unknown UHE spectral model '" + n + "'. Use monochromatic, powerlaw, or powerlaw_cutoff.");
+    *kind = k;
+    return HADROS_OK;
+}
+
+const char* hadros_profile_name(int32_t k)
+{
+    static const char* names[] = {"gaussian", "powerlaw", "gaussian_funnel", "powerlaw_funnel",
+                                  "gaussian_envelope", "powerlaw_envelope", "powerlaw_funnel_envelope",
+                                  "collapsar_ndaf_like"};
+    return (k >= 0 && k < 8) ? names[k] : "unknown";
+}
+const char* hadros_source_name(int32_t k)
+{
+    static const char* names[] = {"inner_ring", "funnel_wall", "jet_base", "shock_layer", "density_weighted"};
+    return (k >= 0 && k < 5) ? names[k] : "unknown";
+}
+const char* hadros_spectrum_name(int32_t k)
+{
+    static const char* names[] = {"monochromatic", "powerlaw", "powerlaw_cutoff"};
+    return (k >= 0 && k < 3) ? names[k] : "unknown";
+}
+
+// SigmaTable::SigmaTable (src/sigma_table.cpp:9-53)
+int hadros_sigma_load(const char* path, double* e, double* sg2, double* scm2, int cap, int* n)
+{
+    if (!path || !n) return fail(HADROS_ERR_ARG, "NULL argument");
+    std::ifstream in(path);
+    if (!in) return fail(HADROS_ERR_IO, std::string("Could not open sigma table: ") + path);
+    std::string line;
+    int cnt = 0;
+    std::vector<double> E, G, S;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        double a, b, c;
+        if (!(iss >> a >> b >> c)) continue;
+        if (a <= 0.0 || b <= 0.0 || c <= 0.0) continue;
+        E.push_back(a);
+        G.push_back(b);
+        S.push_back(c);
+        ++cnt;
+    }
+    if (cnt < 2) return fail(HADROS_ERR_TABLE, "Sigma table must contain at least two valid data lines.");
+    for (int k = 1; k < cnt; ++k)
+        if (E[k] <= E[k - 1]) return fail(HADROS_ERR_TABLE, "Energy grid in sigma table must be strictly increasing.");
+    for (int k = 0; k < cnt && k < cap; ++k) {
+        if (e) e[k] = E[k];
+        if (sg2) sg2[k] = G[k];
+        if (scm2) scm2[k] = S[k];
+    }
+    *n = cnt;
+    return HADROS_OK;
+}
+
+int hadros_sigma_check(const double* e, const double* s, int n) { return check_sigma(e, s, n); }
+
+// ---------------------------------------------------------------- tracer plan
+int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, const double* se, const double* ss,
+                         int n_sigma, int precision, hadros_tracer** out)
+{
+    if (!cam || !scene || !out) return fail(HADROS_ERR_ARG, "NULL argument");
+    if (precision != HADROS_PREC_FP64 && precision != HADROS_PREC_FP32K) return fail(HADROS_ERR_ARG, "unknown precision");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hadros_tracer* t = new hadros_tracer();
+    rc = make_cam(*cam, t->args.cam);
+    if (!rc) rc = make_rt(*scene, se, ss, n_sigma, cam->a_spin, t->args.rt);
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    t->precision = precision;
+    t->block = 128;
+    t->npix = long(cam->nx) * cam->ny;
+    std::vector<double> sd;
+    std::vector<float> sf;
+    sigma_tables(se, ss, n_sigma, sd, sf);
+    const size_t nb = size_t(std::max(t->args.rt.nb, 0));
+    const size_t np = size_t(t->npix);
+    if ((rc = dalloc(&t->d_sig, sd.size())) || (rc = dalloc(&t->d_sigf, sf.size())) || (rc = dalloc(&t->d_tau, np)) ||
+        (rc = dalloc(&t->d_p, np)) || (rc = dalloc(&t->d_I, np)) || (rc = dalloc(&t->d_alpha, np)) ||
+        (rc = dalloc(&t->d_beta, np)) || (rc = dalloc(&t->d_bt, np * nb)) || (rc = dalloc(&t->d_bp, np * nb)) ||
+        (rc = dalloc(&t->d_bi, np * nb)) || (rc = dalloc(&t->d_cap, np)) || (rc = dalloc(&t->d_np, np)) ||
+        (rc = dalloc(&t->d_flags, np)) || (rc = dalloc(&t->d_cnt, 4))) {
+        hadros_tracer_destroy(t);
+        return rc;
+    }
+    HIPC(hipMemcpy(t->d_sig, sd.data(), sd.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(t->d_sigf, sf.data(), sf.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(hipEventCreate(&t->ev0));
+    HIPC(hipEventCreate(&t->ev1));
+    TraceArgs& A = t->args;
+    A.sig = t->d_sig;
+    A.sigf = t->d_sigf;
+    A.out = TraceOut{t->d_tau, t->d_p, t->d_I, t->d_alpha, t->d_beta, t->d_bt, t->d_bp, t->d_bi,
+                     t->d_cap, t->d_np, t->d_flags, t->d_cnt};
+    *out = t;
+    return HADROS_OK;
+}
+
+int hadros_tracer_set_block(hadros_tracer* t, int block)
+{
+    if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
+    if (block == 0) block = 128;
+    if (block < 64 || block > 256 || block % 64) return fail(HADROS_ERR_ARG, "block size must be 64, 128, 192 or 256");
+    t->block = block;
+    return HADROS_OK;
+}
+
+int hadros_tracer_run(hadros_tracer* t, void* stream)
+{
+    if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // counters cleared by a fill kernel (no memset nodes if the caller captures the stream)
+    HIPC(launch_zero_u64(t->d_cnt, 4, s));
+    HIPC(hipEventRecord(t->ev0, s));
+    HIPC(launch_trace_rt(t->args, t->precision, t->block, s));
+    HIPC(hipEventRecord(t->ev1, s));
+    t->ran = true;
+    return HADROS_OK;
+}
+
+int hadros_tracer_last_ms(hadros_tracer* t, double* ms)
+{
+    if (!t || !ms || !t->ran) return fail(HADROS_ERR_ARG, "tracer has not run");
+    HIPC(hipEventSynchronize(t->ev1));
+    float f = 0.0f;
+    HIPC(hipEventElapsedTime(&f, t->ev0, t->ev1));
+    *ms = f;
+    return HADROS_OK;
+}
+
+int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* o, hadros_stats* st)
+{
+    if (!t || !t->ran) return fail(HADROS_ERR_ARG, "tracer has not run");
+    HIPC(hipEventSynchronize(t->ev1));
+    const size_t np = size_t(t->npix);
+    const size_t nb = size_t(std::max(t->args.rt.nb, 0));
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst && src && bytes) HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return HADROS_OK;
+    };
+    int rc = 0;
+    if (o) {
+        if ((rc = cp(o->tau, t->d_tau, np * 8)) || (rc = cp(o->p_surv, t->d_p, np * 8)) ||
+            (rc = cp(o->intensity, t->d_I, np * 8)) || (rc = cp(o->captured, t->d_cap, np * 4)) ||
+            (rc = cp(o->npoints, t->d_np, np * 4)) || (rc = cp(o->flags, t->d_flags, np * 4)) ||
+            (rc = cp(o->alpha, t->d_alpha, np * 8)) || (rc = cp(o->beta, t->d_beta, np * 8)) ||
+            (rc = cp(o->bin_tau, t->d_bt, np * nb * 8)) || (rc = cp(o->bin_p, t->d_bp, np * nb * 8)) ||
+            (rc = cp(o->bin_i, t->d_bi, np * nb * 8)))
+            return rc;
+    }
+    if (st) {
+        unsigned long long c[4];
+        HIPC(hipMemcpy(c, t->d_cnt, sizeof(c), hipMemcpyDeviceToHost));
+        float f = 0.0f;
+        HIPC(hipEventElapsedTime(&f, t->ev0, t->ev1));
+        st->kernel_ms = f;
+        st->total_ms = f;
+        st->total_points = (int64_t)c[0];
+        st->total_attempts = (int64_t)c[1];
+        st->n_captured = int32_t(c[2]);
+        st->n_exhausted = int32_t(c[3]);
+        st->n_bins = t->args.rt.nb;
+        st->block_size = t->block;
+    }
+    return HADROS_OK;
+}
+
+int hadros_tracer_destroy(hadros_tracer* t)
+{
+    if (!t) return HADROS_OK;
+    dfree(t->d_sig);
+    dfree(t->d_sigf);
+    dfree(t->d_tau);
+    dfree(t->d_p);
+    dfree(t->d_I);
+    dfree(t->d_alpha);
+    dfree(t->d_beta);
+    dfree(t->d_bt);
+    dfree(t->d_bp);
+    dfree(t->d_bi);
+    dfree(t->d_cap);
+    dfree(t->d_np);
+    dfree(t->d_flags);
+    dfree(t->d_cnt);
+    if (t->ev0) (void)hipEventDestroy(t->ev0);
+    if (t->ev1) (void)hipEventDestroy(t->ev1);
+    delete t;
+    return HADROS_OK;
+}
+
+int hadros_trace_rt(const hadros_camera* cam, const hadros_scene* scene, const double* se, const double* ss,
+                    int n_sigma, int precision, hadros_image_out* out, hadros_stats* stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    hadros_tracer* t = nullptr;
+    int rc = hadros_tracer_create(cam, scene, se, ss, n_sigma, precision, &t);
+    if (rc) return rc;
+    rc = hadros_tracer_run(t, nullptr);
+    if (!rc) rc = hadros_tracer_fetch(t, out, stats);
+    hadros_tracer_destroy(t);
+    if (!rc && stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+// ---------------------------------------------------------------- replay plan
+int hadros_replay_create(const hadros_cache_view* c, const hadros_scene* scene, const double* se, const double* ss,
+                         int n_sigma, int precision, hadros_replay** out)
+{
+    if (!c || !scene || !out) return fail(HADROS_ERR_ARG, "NULL argument");
+    if (c->n_rays < 0 || (c->n_rays > 0 && !c->offset)) return fail(HADROS_ERR_ARG, "bad cache view");
+    if (precision != HADROS_PREC_FP64 && precision != HADROS_PREC_FP32K) return fail(HADROS_ERR_ARG, "unknown precision");
+    for (int k = 0; k < c->n_rays; ++k)
+        if (c->offset[k + 1] < c->offset[k]) return fail(HADROS_ERR_CACHE_TRUNCATED, "Invalid ray with negative number of points");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hadros_replay* p = new hadros_replay();
+    rc = make_rt(*scene, se, ss, n_sigma, c->a_bh, p->args.rt);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    p->precision = precision;
+    p->n_rays = c->n_rays;
+    p->n_points = c->n_rays ? long(c->offset[c->n_rays] - c->offset[0]) : 0;
+    std::vector<double> sd;
+    std::vector<float> sf;
+    sigma_tables(se, ss, n_sigma, sd, sf);
+    const size_t nr = size_t(p->n_rays), npt = size_t(p->n_points), nb = size_t(std::max(p->args.rt.nb, 0));
+    if ((rc = dalloc(&p->d_sig, sd.size())) || (rc = dalloc(&p->d_sigf, sf.size())) ||
+        (rc = dalloc(&p->d_off, nr + 1)) || (rc = dalloc(&p->d_r, npt)) || (rc = dalloc(&p->d_th, npt)) ||
+        (rc = dalloc(&p->d_dl, npt)) || (rc = dalloc(&p->d_z, npt)) || (rc = dalloc(&p->d_tau, nr)) ||
+        (rc = dalloc(&p->d_p, nr)) || (rc = dalloc(&p->d_I, nr)) || (rc = dalloc(&p->d_bt, nr * nb)) ||
+        (rc = dalloc(&p->d_bp, nr * nb)) || (rc = dalloc(&p->d_bi, nr * nb)) || (rc = dalloc(&p->d_cnt, 1))) {
+        hadros_replay_destroy(p);
+        return rc;
+    }
+    HIPC(hipMemcpy(p->d_sig, sd.data(), sd.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(p->d_sigf, sf.data(), sf.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (nr) {
+        // offsets rebased to 0
+        std::vector<long> off(nr + 1);
+        for (size_t k = 0; k <= nr; ++k) off[k] = long(c->offset[k] - c->offset[0]);
+        HIPC(hipMemcpy(p->d_off, off.data(), (nr + 1) * sizeof(long), hipMemcpyHostToDevice));
+    }
+    if (npt) {
+        const size_t b = size_t(c->offset[0]);
+        HIPC(hipMemcpy(p->d_r, c->r + b, npt * 8, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(p->d_th, c->theta + b, npt * 8, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(p->d_dl, c->dl + b, npt * 8, hipMemcpyHostToDevice));
+        HIPC(hipMemcpy(p->d_z, c->redshift + b, npt * 8, hipMemcpyHostToDevice));
+    }
+    HIPC(hipEventCreate(&p->ev0));
+    HIPC(hipEventCreate(&p->ev1));
+    ReplayArgs& A = p->args;
+    A.sig = p->d_sig;
+    A.sigf = p->d_sigf;
+    A.n_rays = int(nr);
+    A.offset = p->d_off;
+    A.r = p->d_r;
+    A.th = p->d_th;
+    A.dl = p->d_dl;
+    A.z = p->d_z;
+    A.tau = p->d_tau;
+    A.p = p->d_p;
+    A.I = p->d_I;
+    A.btau = p->d_bt;
+    A.bp = p->d_bp;
+    A.bi = p->d_bi;
+    A.counters = p->d_cnt;
+    *out = p;
+    return HADROS_OK;
+}
+
+int hadros_replay_run(hadros_replay* p, void* stream)
+{
+    if (!p) return fail(HADROS_ERR_ARG, "NULL replay");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    HIPC(launch_zero_u64(p->d_cnt, 1, s));
+    HIPC(hipEventRecord(p->ev0, s));
+    if (p->n_rays > 0) HIPC(launch_replay(p->args, p->precision, s));
+    HIPC(hipEventRecord(p->ev1, s));
+    p->ran = true;
+    return HADROS_OK;
+}
+
+int hadros_replay_last_ms(hadros_replay* p, double* ms)
+{
+    if (!p || !ms || !p->ran) return fail(HADROS_ERR_ARG, "replay has not run");
+    HIPC(hipEventSynchronize(p->ev1));
+    float f = 0.0f;
+    HIPC(hipEventElapsedTime(&f, p->ev0, p->ev1));
+    *ms = f;
+    return HADROS_OK;
+}
+
+int hadros_replay_fetch(hadros_replay* p, hadros_image_out* o, hadros_stats* st)
+{
+    if (!p || !p->ran) return fail(HADROS_ERR_ARG, "replay has not run");
+    HIPC(hipEventSynchronize(p->ev1));
+    const size_t nr = size_t(p->n_rays), nb = size_t(std::max(p->args.rt.nb, 0));
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst && src && bytes) HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return HADROS_OK;
+    };
+    int rc = 0;
+    if (o) {
+        if ((rc = cp(o->tau, p->d_tau, nr * 8)) || (rc = cp(o->p_surv, p->d_p, nr * 8)) ||
+            (rc = cp(o->intensity, p->d_I, nr * 8)) || (rc = cp(o->bin_tau, p->d_bt, nr * nb * 8)) ||
+            (rc = cp(o->bin_p, p->d_bp, nr * nb * 8)) || (rc = cp(o->bin_i, p->d_bi, nr * nb * 8)))
+            return rc;
+    }
+    if (st) {
+        unsigned long long c = 0;
+        HIPC(hipMemcpy(&c, p->d_cnt, sizeof(c), hipMemcpyDeviceToHost));
+        float f = 0.0f;
+        HIPC(hipEventElapsedTime(&f, p->ev0, p->ev1));
+        std::memset(st, 0, sizeof(*st));
+        st->kernel_ms = f;
+        st->total_ms = f;
+        st->total_points = (int64_t)c;
+        st->n_bins = p->args.rt.nb;
+        st->block_size = 256;
+    }
+    return HADROS_OK;
+}
+
+int hadros_replay_destroy(hadros_replay* p)
+{
+    if (!p) return HADROS_OK;
+    dfree(p->d_sig);
+    dfree(p->d_sigf);
+    dfree(p->d_off);
+    dfree(p->d_r);
+    dfree(p->d_th);
+    dfree(p->d_dl);
+    dfree(p->d_z);
+    dfree(p->d_tau);
+    dfree(p->d_p);
+    dfree(p->d_I);
+    dfree(p->d_bt);
+    dfree(p->d_bp);
+    dfree(p->d_bi);
+    dfree(p->d_cnt);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    delete p;
+    return HADROS_OK;
+}
+
+int hadros_rt_from_cache(const hadros_cache_view* c, const hadros_scene* scene, const double* se, const double* ss,
+                         int n_sigma, int precision, hadros_image_out* out, hadros_stats* stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    hadros_replay* p = nullptr;
+    int rc = hadros_replay_create(c, scene, se, ss, n_sigma, precision, &p);
+    if (rc) return rc;
+    rc = hadros_replay_run(p, nullptr);
+    if (!rc) rc = hadros_replay_fetch(p, out, stats);
+    hadros_replay_destroy(p);
+    if (!rc && stats)
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+// ---------------------------------------------------------------- geodesic cache
+int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captured, double* points, int64_t cap)
+{
+    if (!cam || !offset) return fail(HADROS_ERR_ARG, "NULL argument");
+    int rc = ensure_device();
+    if (rc) return rc;
+    CacheArgs A{};
+    rc = make_cam(*cam, A.cam);
+    if (rc) return rc;
+    const size_t np = size_t(cam->nx) * size_t(cam->ny);
+    long* d_counts = nullptr;
+    int* d_cap = nullptr;
+    long* d_off = nullptr;
+    double* d_pts = nullptr;
+    if ((rc = dalloc(&d_counts, np)) || (rc = dalloc(&d_cap, np)) || (rc = dalloc(&d_off, np + 1))) {
+        dfree(d_counts);
+        dfree(d_cap);
+        return rc;
+    }
+    // pass 1: point counts (the trace is deterministic, so pass 2 reproduces it)
+    A.counts = d_counts;
+    A.captured = d_cap;
+    hipError_t e = launch_trace_cache(A, false, nullptr);
+    std::vector<long> counts(np);
+    std::vector<int> capv(np);
+    if (e == hipSuccess) e = hipMemcpy(counts.data(), d_counts, np * sizeof(long), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(capv.data(), d_cap, np * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        dfree(d_counts);
+        dfree(d_cap);
+        dfree(d_off);
+        return fail(HADROS_ERR_DEVICE, std::string("trace_cache count pass: ") + hipGetErrorString(e));
+    }
+    std::vector<long> off(np + 1, 0);
+    for (size_t k = 0; k < np; ++k) off[k + 1] = off[k] + counts[k];
+    for (size_t k = 0; k <= np; ++k) offset[k] = off[k];
+    if (captured)
+        for (size_t k = 0; k < np; ++k) captured[k] = capv[k];
+    if (points) {
+        if (cap < off[np]) {
+            dfree(d_counts);
+            dfree(d_cap);
+            dfree(d_off);
+            return fail(HADROS_ERR_MAX_POINTS, "point buffer smaller than offset[nx*ny]");
+        }
+        if ((rc = dalloc(&d_pts, size_t(off[np]) * 7))) {
+            dfree(d_counts);
+            dfree(d_cap);
+            dfree(d_off);
+            return rc;
+        }
+        e = hipMemcpy(d_off, off.data(), (np + 1) * sizeof(long), hipMemcpyHostToDevice);
+        A.offset = d_off;
+        A.points = d_pts;
+        if (e == hipSuccess) e = launch_trace_cache(A, true, nullptr);
+        if (e == hipSuccess && off[np] > 0)
+            e = hipMemcpy(points, d_pts, size_t(off[np]) * 7 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    dfree(d_counts);
+    dfree(d_cap);
+    dfree(d_off);
+    dfree(d_pts);
+    if (e != hipSuccess) return fail(HADROS_ERR_DEVICE, std::string("trace_cache: ") + hipGetErrorString(e));
+    return HADROS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- point-wise hooks
+namespace {
+template <class F>
+int run_eval(const std::vector<std::pair<const double*, size_t>>& ins, double* out, size_t out_n, F&& launch)
+{
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::vector<double*> d(ins.size(), nullptr);
+    double* d_out = nullptr;
+    auto cleanup = [&]() {
+        for (auto& p : d) dfree(p);
+        dfree(d_out);
+    };
+    for (size_t k = 0; k < ins.size(); ++k) {
+        if ((rc = dalloc(&d[k], ins[k].second))) {
+            cleanup();
+            return rc;
+        }
+        if (ins[k].second) {
+            hipError_t e = hipMemcpy(d[k], ins[k].first, ins[k].second * 8, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                cleanup();
+                return fail(HADROS_ERR_DEVICE, hipGetErrorString(e));
+            }
+        }
+    }
+    if ((rc = dalloc(&d_out, out_n))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = launch(d, d_out);
+    if (e == hipSuccess && out_n) e = hipMemcpy(out, d_out, out_n * 8, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(HADROS_ERR_DEVICE, hipGetErrorString(e));
+    return HADROS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hadros_eval_rho(const hadros_medium* m, const double* r, const double* th, double* out, int64_t n)
+{
+    if (!m || n < 0) return fail(HADROS_ERR_ARG, "bad argument");
+    if (m->kind < 0 || m->kind > 7) return fail(HADROS_ERR_KIND, "unknown density profile kind");
+    const DevMedium dm = make_medium(*m);
+    return run_eval({{r, size_t(n)}, {th, size_t(n)}}, out, size_t(n), [&](std::vector<double*>& d, double* o) {
+        return launch_eval_rho(dm, d[0], d[1], o, long(n));
+    });
+}
+
+int hadros_eval_sigma(const double* se, const double* ss, int n_sigma, const double* e, double* out, int64_t n)
+{
+    int rc = check_sigma(se, ss, n_sigma);
+    if (rc) return rc;
+    std::vector<double> sd;
+    std::vector<float> sf;
+    sigma_tables(se, ss, n_sigma, sd, sf);
+    const double lo = se[0], hi = se[n_sigma - 1];
+    return run_eval({{sd.data(), sd.size()}, {e, size_t(n)}}, out, size_t(n),
+                    [&](std::vector<double*>& d, double* o) {
+                        return launch_eval_sigma(d[0], n_sigma, lo, hi, d[1], o, long(n));
+                    });
+}
+
+int hadros_eval_emissivity(const hadros_scene* scene, const double* r, const double* th, const double* e, double* out,
+                           int64_t n)
+{
+    if (!scene) return fail(HADROS_ERR_ARG, "NULL scene");
+    // a two-point dummy table: emissivity does not use sigma
+    const double te[2] = {1.0, 2.0}, ts[2] = {1.0, 1.0};
+    DevRt P;
+    int rc = make_rt(*scene, te, ts, 2, 0.0, P);
+    if (rc) return rc;
+    return run_eval({{r, size_t(n)}, {th, size_t(n)}, {e, size_t(n)}}, out, size_t(n),
+                    [&](std::vector<double*>& d, double* o) { return launch_eval_emis(P, d[0], d[1], d[2], o, long(n)); });
+}
+
+int hadros_eval_rhs(double a, int mode, const double* y, double* dy, int64_t n)
+{
+    if (mode != HADROS_MODE_REF && mode != HADROS_MODE_CFG) return fail(HADROS_ERR_ARG, "unknown mode");
+    return run_eval({{y, size_t(n) * 8}}, dy, size_t(n) * 8,
+                    [&](std::vector<double*>& d, double* o) { return launch_eval_rhs(a, mode, d[0], o, long(n)); });
+}
+
+int hadros_eval_metric(double a, const double* r, const double* th, double* out, int64_t n)
+{
+    return run_eval({{r, size_t(n)}, {th, size_t(n)}}, out, size_t(n) * 10,
+                    [&](std::vector<double*>& d, double* o) { return launch_eval_metric(a, d[0], d[1], o, long(n)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// kernels.h -- kernel argument blocks and launchers shared by the .hip units.
+#pragma once
+
+#include "physics.h"
+
+namespace hd {
+
+struct TraceOut {
+    double *tau, *p, *I, *alpha, *beta, *btau, *bp, *bi;
+    int *captured, *npoints, *flags;
+    unsigned long long* counters;   // [0] points, [1] attempts, [2] captured, [3] exhausted
+};
+
+struct TraceArgs {
+    DevCam cam;
+    DevRt rt;
+    const double* sig;    // 3*n_sigma: ln E, ln sigma, slope (FP64)
+    const float* sigf;    // same, FP32
+    TraceOut out;
+};
+
+struct CacheArgs {
+    DevCam cam;
+    const long* offset;   // [npix+1] (write pass)
+    double* points;       // 7 doubles per point (write pass)
+    long* counts;         // [npix] (count pass)
+    int* captured;        // [npix] (count pass)
+};
+
+struct ReplayArgs {
+    DevRt rt;
+    const double* sig;
+    const float* sigf;
+    int n_rays;
+    const long* offset;
+    const double *r, *th, *dl, *z;
+    double *tau, *p, *I, *btau, *bp, *bi;
+    unsigned long long* counters;
+};
+
+// Sum four 64-bit per-lane values over the wavefront and add them to
+// counters[0..3] from lane 0 (one atomic per wave, not per lane).
+__device__ __forceinline__ void wave_add4(unsigned long long* c, unsigned long long a, unsigned long long b,
+                                          unsigned long long d, unsigned long long e)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        d += __shfl_xor(d, off, 64);
+        e += __shfl_xor(e, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && c) {
+        atomicAdd(c + 0, a);
+        atomicAdd(c + 1, b);
+        atomicAdd(c + 2, d);
+        atomicAdd(c + 3, e);
+    }
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStream_t s);
+hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s);
+hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s);
+
+// point-wise hooks (eval.hip)
+hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th, double* out, long n);
+hipError_t launch_eval_sigma(const double* sig, int n_sigma, double e_lo, double e_hi, const double* e, double* out,
+                             long n);
+hipError_t launch_eval_emis(const DevRt& P, const double* r, const double* th, const double* e, double* out,
+                            long n);
+hipError_t launch_eval_rhs(double a, int mode, const double* y, double* dy, long n);
+hipError_t launch_eval_metric(double a, const double* r, const double* th, double* out, long n);
+
+}  // namespace hd

@@ -1,0 +1,544 @@
+// physics.h -- device-side physics of the per-pixel path for gfx950.
+//
+// One ray per lane: every function here is a register-resident FP64 (or
+// FP32 for the precision form's transfer) routine with no memory traffic
+// except the sigma table, which the kernels stage in LDS.
+//
+// Reference correspondence (file:line of the reference sources):
+//   Kerr metric / inverse / lapse / omega   src/kerr_metric.cpp:10-144
+//   rhs (finite differences, REF mode)      src/kerr_geodesic.cpp:47-135
+//   RKF45 tableau and REF step control      src/kerr_geodesic.cpp:194-329
+//   RK4 fallback                            src/kerr_geodesic.cpp:137-192
+//   camera initial state, dl, redshift      src/kerr_camera.cpp:14-186
+//   density morphologies                    src/torus_profile.cpp:93-272
+//   source / spectral weights, emissivity   src/radiative_transfer.cpp:64-395
+//   transfer per path point                 src/radiative_transfer.cpp:501-672
+//   sigma log-log interpolation             src/sigma_table.cpp:80-124
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/hadros.h"
+
+namespace hd {
+
+constexpr double PI = 3.141592653589793238462643383279502884;
+constexpr double M_U_G = 1.66053906660e-24;   // atomic mass unit [g]
+
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__host__ __device__ __forceinline__ double dclamp(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
+
+// ------------------------------------------------------------------ params
+// Normalised TorusProfile (constructor clamps of src/torus_profile.cpp:12-42
+// applied on the host).
+struct DevMedium {
+    int kind;
+    int powerlaw_disk, funnel, envelope;
+    double rho0, r0, sr, H, p, fd, ft, env0, ealpha, rmin, rmax, floor_;
+};
+
+struct DevSource {
+    int kind;
+    int ring_ok;          // emissivity_collapsar_ring guard (src/radiative_transfer.cpp:343-349)
+    double rc, sr, tw, norm, ft, q, s, rho_ref, cmin, cmax, gdr, gdth;
+    double rho_norm_c;    // torus.rho(r_center, pi/2), a per-scene constant
+};
+
+// Spectral factor of every bin: ln S(E) = -gam ln E - E * invcut, valid if spec_ok.
+// Monochromatic: gam = source.powerlaw, invcut = 1/emax (src/radiative_transfer.cpp:184-192);
+// powerlaw: gam, 0; powerlaw_cutoff: gam, 1/ecut.
+struct DevRt {
+    DevMedium med;
+    DevSource src;
+    int nb;               // bins integrated (0..16)
+    int mono;
+    int spec_ok;
+    int n_sigma;
+    double gam, invcut;
+    double rg_cm;         // r_g in cm (include/constants.hpp:17-20)
+    double r_h_rt;        // 1 + sqrt(1 - clamp(a,0,0.999999)^2) (src/radiative_transfer.cpp:485-489)
+    double e_lo, e_hi;    // sigma table range
+    double emid[HADROS_MAX_BINS], lnemid[HADROS_MAX_BINS], de[HADROS_MAX_BINS], w[HADROS_MAX_BINS];
+};
+
+struct DevCam {
+    double a, r_obs, th_obs, tanhalf, r_max, h, tol;
+    int nx, ny, mode, max_steps;
+    // observer constants of KerrCamera::initial_state (src/kerr_camera.cpp:110-186)
+    double g_tt, g_tph, g_rr, g_thth, g_phph, alpha, omega;
+    double r_h;
+};
+
+// ------------------------------------------------------------------ metric
+struct Metric5 {
+    double tt, tph, rr, thth, phph;
+};
+
+// KerrMetric::inverse_metric (src/kerr_metric.cpp:77-116) given sin, cos of theta.
+__device__ __forceinline__ Metric5 inv_metric(double a, double r, double s, double c)
+{
+    const double sig = r * r + a * a * c * c;
+    const double del = r * r - 2.0 * r + a * a;
+    const double s2 = s * s;
+    const double bigA = (r * r + a * a) * (r * r + a * a) - a * a * del * s2;
+    Metric5 g;
+    g.tt = -bigA / (sig * del);
+    g.tph = -2.0 * a * r / (sig * del);
+    g.rr = del / sig;
+    g.thth = 1.0 / sig;
+    g.phph = (del - a * a * s2) / (sig * del * s2);
+    return g;
+}
+
+// KerrMetric::metric (src/kerr_metric.cpp:34-75).
+__device__ __forceinline__ Metric5 metric(double a, double r, double th)
+{
+    double s, c;
+    sincos(th, &s, &c);
+    const double sig = r * r + a * a * c * c;
+    const double del = r * r - 2.0 * r + a * a;
+    const double s2 = s * s;
+    Metric5 g;
+    g.tt = -(1.0 - 2.0 * r / sig);
+    g.tph = -2.0 * a * r * s2 / sig;
+    g.rr = sig / del;
+    g.thth = sig;
+    g.phph = (r * r + a * a + 2.0 * a * a * r * s2 / sig) * s2;
+    return g;
+}
+
+// lapse and frame-drag together (src/kerr_metric.cpp:118-144)
+__device__ __forceinline__ void lapse_omega(double a, double r, double th, double& alpha, double& omega)
+{
+    double s, c;
+    sincos(th, &s, &c);
+    const double sig = r * r + a * a * c * c;
+    const double del = r * r - 2.0 * r + a * a;
+    const double s2 = s * s;
+    const double bigA = (r * r + a * a) * (r * r + a * a) - a * a * del * s2;
+    alpha = sqrt(sig * del / bigA);
+    omega = 2.0 * a * r / bigA;
+}
+
+// ------------------------------------------------------------------ RHS
+// Evolving part of the state: r, theta, phi, p_r, p_theta.  p_t and p_phi have
+// exactly zero derivative (src/kerr_geodesic.cpp:116-117) and t feeds nothing.
+struct Y5 {
+    double r, th, ph, pr, pth;
+};
+
+// KerrGeodesic::rhs in REF mode: central differences of g^{mu nu} with
+// eps_r = 1e-5 max(1,|r|), eps_th = 1e-5; the accumulation runs over the
+// non-zero (mu,nu) pairs in the reference's mu-major order.
+__device__ __forceinline__ Y5 rhs_ref(double a, const Y5& y, double pt, double pph)
+{
+    const double r = y.r, th = y.th;
+    double s, c;
+    sincos(th, &s, &c);
+    const Metric5 g0 = inv_metric(a, r, s, c);
+    Y5 d;
+    d.r = g0.rr * y.pr;
+    d.th = g0.thth * y.pth;
+    d.ph = g0.tph * pt + g0.phph * pph;
+
+    const double er = 1.0e-5 * dmax(1.0, fabs(r));
+    const double et = 1.0e-5;
+    const Metric5 gp = inv_metric(a, r + er, s, c);
+    const Metric5 gm = inv_metric(a, r - er, s, c);
+    double sp, cp, sm, cm;
+    sincos(th + et, &sp, &cp);
+    sincos(th - et, &sm, &cm);
+    const Metric5 tp = inv_metric(a, r, sp, cp);
+    const Metric5 tm = inv_metric(a, r, sm, cm);
+    const double two_er = 2.0 * er, two_et = 2.0 * et;
+    const double dr_tt = (gp.tt - gm.tt) / two_er, dt_tt = (tp.tt - tm.tt) / two_et;
+    const double dr_tp = (gp.tph - gm.tph) / two_er, dt_tp = (tp.tph - tm.tph) / two_et;
+    const double dr_rr = (gp.rr - gm.rr) / two_er, dt_rr = (tp.rr - tm.rr) / two_et;
+    const double dr_hh = (gp.thth - gm.thth) / two_er, dt_hh = (tp.thth - tm.thth) / two_et;
+    const double dr_pp = (gp.phph - gm.phph) / two_er, dt_pp = (tp.phph - tm.phph) / two_et;
+    double a_r = 0.0, a_t = 0.0;
+    a_r -= 0.5 * dr_tt * pt * pt;      a_t -= 0.5 * dt_tt * pt * pt;
+    a_r -= 0.5 * dr_tp * pt * pph;     a_t -= 0.5 * dt_tp * pt * pph;
+    a_r -= 0.5 * dr_rr * y.pr * y.pr;  a_t -= 0.5 * dt_rr * y.pr * y.pr;
+    a_r -= 0.5 * dr_hh * y.pth * y.pth; a_t -= 0.5 * dt_hh * y.pth * y.pth;
+    a_r -= 0.5 * dr_tp * pph * pt;     a_t -= 0.5 * dt_tp * pph * pt;
+    a_r -= 0.5 * dr_pp * pph * pph;    a_t -= 0.5 * dt_pp * pph * pph;
+    d.pr = a_r;
+    d.pth = a_t;
+    return d;
+}
+
+// Analytic Hamiltonian derivative (CFG mode).  With W = (r^2+a^2) p_t + a p_phi
+// and V = p_phi/sin + a sin p_t, 2 Sigma H = N = Delta p_r^2 + p_th^2 - W^2/Delta + V^2
+// (the contraction g^{mu nu} p_mu p_nu of src/kerr_metric.cpp:77-116 times Sigma).
+__device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double pph)
+{
+    const double r = y.r;
+    double s, c;
+    sincos(y.th, &s, &c);
+    const double r2 = r * r, a2 = a * a;
+    const double sig = r2 + a2 * c * c;
+    const double del = r2 - 2.0 * r + a2;
+    const double ra = r2 + a2;
+    const double W = ra * pt + a * pph;
+    const double inv_s = 1.0 / s;
+    const double V = pph * inv_s + a * s * pt;
+    const double inv_sig = 1.0 / sig;
+    const double Wd = W / del;
+    const double N = del * y.pr * y.pr + y.pth * y.pth - W * Wd + V * V;
+    const double ddel = 2.0 * r - 2.0;
+    const double dNr = ddel * y.pr * y.pr - 4.0 * r * pt * Wd + Wd * Wd * ddel;
+    const double Vth = c * (a * pt - pph * inv_s * inv_s);
+    const double dNth = 2.0 * V * Vth;
+    const double h_is = 0.5 * inv_sig;
+    const double n_over = N * h_is * inv_sig;
+    Y5 d;
+    d.r = del * y.pr * inv_sig;
+    d.th = y.pth * inv_sig;
+    d.ph = (V * inv_s - a * Wd) * inv_sig;
+    d.pr = -(dNr * h_is - n_over * (2.0 * r));
+    d.pth = -(dNth * h_is - n_over * (-2.0 * a2 * c * s));
+    return d;
+}
+
+template <int MODE>
+__device__ __forceinline__ Y5 rhs(double a, const Y5& y, double pt, double pph)
+{
+    if constexpr (MODE == HADROS_MODE_CFG) return rhs_cfg(a, y, pt, pph);
+    else return rhs_ref(a, y, pt, pph);
+}
+
+__device__ __forceinline__ Y5 axpy1(const Y5& y, double a1, const Y5& k1)
+{
+    return {y.r + a1 * k1.r, y.th + a1 * k1.th, y.ph + a1 * k1.ph, y.pr + a1 * k1.pr, y.pth + a1 * k1.pth};
+}
+
+// Fehlberg 4(5) attempt (tableau of src/kerr_geodesic.cpp:239-308), stage
+// sums left to right as the reference evaluates them.
+template <int MODE>
+__device__ __forceinline__ void rkf45(double a, const Y5& y, double pt, double pph, double h, Y5& y4, Y5& y5)
+{
+    const Y5 k1 = rhs<MODE>(a, y, pt, pph);
+    const double a21 = h * 1.0 / 4.0;
+    const Y5 k2 = rhs<MODE>(a, axpy1(y, a21, k1), pt, pph);
+    const double a31 = h * 3.0 / 32.0, a32 = h * 9.0 / 32.0;
+    Y5 t;
+#define HD_ST3(F) t.F = y.F + a31 * k1.F + a32 * k2.F
+    HD_ST3(r); HD_ST3(th); HD_ST3(ph); HD_ST3(pr); HD_ST3(pth);
+    const Y5 k3 = rhs<MODE>(a, t, pt, pph);
+    const double a41 = h * 1932.0 / 2197.0, a42 = h * -7200.0 / 2197.0, a43 = h * 7296.0 / 2197.0;
+#define HD_ST4(F) t.F = y.F + a41 * k1.F + a42 * k2.F + a43 * k3.F
+    HD_ST4(r); HD_ST4(th); HD_ST4(ph); HD_ST4(pr); HD_ST4(pth);
+    const Y5 k4 = rhs<MODE>(a, t, pt, pph);
+    const double a51 = h * 439.0 / 216.0, a52 = h * -8.0, a53 = h * 3680.0 / 513.0, a54 = h * -845.0 / 4104.0;
+#define HD_ST5(F) t.F = y.F + a51 * k1.F + a52 * k2.F + a53 * k3.F + a54 * k4.F
+    HD_ST5(r); HD_ST5(th); HD_ST5(ph); HD_ST5(pr); HD_ST5(pth);
+    const Y5 k5 = rhs<MODE>(a, t, pt, pph);
+    const double a61 = h * -8.0 / 27.0, a62 = h * 2.0, a63 = h * -3544.0 / 2565.0, a64 = h * 1859.0 / 4104.0,
+                 a65 = h * -11.0 / 40.0;
+#define HD_ST6(F) t.F = y.F + a61 * k1.F + a62 * k2.F + a63 * k3.F + a64 * k4.F + a65 * k5.F
+    HD_ST6(r); HD_ST6(th); HD_ST6(ph); HD_ST6(pr); HD_ST6(pth);
+    const Y5 k6 = rhs<MODE>(a, t, pt, pph);
+#define HD_Y4(F) y4.F = y.F + h * (25.0 / 216.0 * k1.F + 1408.0 / 2565.0 * k3.F + 2197.0 / 4104.0 * k4.F - 1.0 / 5.0 * k5.F)
+#define HD_Y5(F) y5.F = y.F + h * (16.0 / 135.0 * k1.F + 6656.0 / 12825.0 * k3.F + 28561.0 / 56430.0 * k4.F - 9.0 / 50.0 * k5.F + 2.0 / 55.0 * k6.F)
+    HD_Y4(r); HD_Y4(th); HD_Y4(ph); HD_Y4(pr); HD_Y4(pth);
+    HD_Y5(r); HD_Y5(th); HD_Y5(ph); HD_Y5(pr); HD_Y5(pth);
+#undef HD_ST3
+#undef HD_ST4
+#undef HD_ST5
+#undef HD_ST6
+#undef HD_Y4
+#undef HD_Y5
+}
+
+// KerrGeodesic::step_rk4 at the base step (src/kerr_geodesic.cpp:137-192).
+__device__ __noinline__ Y5 rk4_ref(double a, const Y5& y, double pt, double pph, double h)
+{
+    const Y5 k1 = rhs_ref(a, y, pt, pph);
+    const Y5 k2 = rhs_ref(a, axpy1(y, 0.5 * h, k1), pt, pph);
+    const Y5 k3 = rhs_ref(a, axpy1(y, 0.5 * h, k2), pt, pph);
+    const Y5 k4 = rhs_ref(a, axpy1(y, h, k3), pt, pph);
+    Y5 o;
+#define HD_RK4(F) o.F = y.F + h * (k1.F + 2.0 * k2.F + 2.0 * k3.F + k4.F) / 6.0
+    HD_RK4(r); HD_RK4(th); HD_RK4(ph); HD_RK4(pr); HD_RK4(pth);
+#undef HD_RK4
+    return o;
+}
+
+// error_norm (src/kerr_geodesic.cpp:219-233); std::max semantics keep err on NaN.
+__device__ __forceinline__ double err_ref(const Y5& a, const Y5& b)
+{
+    double e = 0.0;
+    e = dmax(e, fabs(a.r - b.r));
+    e = dmax(e, fabs(a.th - b.th));
+    e = dmax(e, fabs(a.ph - b.ph));
+    e = dmax(e, fabs(a.pr - b.pr));
+    e = dmax(e, fabs(a.pth - b.pth));
+    return e;
+}
+
+// CFG mixed error: max |y5-y4| / (tol (1 + max(|y|,|y5|))); NaN forces rejection.
+__device__ __forceinline__ double err_cfg(const Y5& y, const Y5& y4, const Y5& y5, double tol)
+{
+    double e = 0.0;
+#define HD_E(F)                                                                          \
+    {                                                                                    \
+        const double sc = tol * (1.0 + fmax(fabs(y.F), fabs(y5.F)));                    \
+        const double q = fabs(y5.F - y4.F) / sc;                                         \
+        if (!(q <= e)) e = q;                                                            \
+    }
+    HD_E(r); HD_E(th); HD_E(ph); HD_E(pr); HD_E(pth);
+#undef HD_E
+    return e;
+}
+
+constexpr double CFG_H_MIN = 1.0e-9;
+
+// ------------------------------------------------------------------ camera
+// src/kerr_camera.cpp:14-29
+__device__ __forceinline__ double zamo_energy(double a, double r, double th, double pt, double pph)
+{
+    double alpha, omega;
+    lapse_omega(a, r, th, alpha, omega);
+    return -(pt + omega * pph) / alpha;
+}
+
+// src/kerr_camera.cpp:31-44 (the reference's while loops, bounded)
+__device__ __forceinline__ double wrapped_dphi(double pn, double po)
+{
+    double d = pn - po;
+    for (int it = 0; it < 64 && d > PI; ++it) d -= 2.0 * PI;
+    for (int it = 0; it < 64 && d < -PI; ++it) d += 2.0 * PI;
+    return d;
+}
+
+// src/kerr_camera.cpp:46-76
+__device__ __forceinline__ double zamo_dl(double a, const Y5& cur, const Y5& prev)
+{
+    const Metric5 g = metric(a, 0.5 * (cur.r + prev.r), 0.5 * (cur.th + prev.th));
+    const double dr = cur.r - prev.r;
+    const double dth = cur.th - prev.th;
+    const double dph = wrapped_dphi(cur.ph, prev.ph);
+    const double dl2 = g.rr * dr * dr + g.thth * dth * dth + g.phph * dph * dph;
+    return sqrt(dmax(dl2, 0.0));
+}
+
+// ------------------------------------------------------------------ medium
+__host__ __device__ __forceinline__ double sq(double x) { return x * x; }
+
+// TorusProfile::rho (src/torus_profile.cpp:150-272); pow(x, 2.0) is x*x.
+__host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, double th)
+{
+    double shape;
+    if (m.kind == HADROS_PROFILE_COLLAPSAR_NDAF_LIKE) {
+        if (!(r >= m.rmin && r <= m.rmax) || r <= 0.0 || m.r0 <= 0.0) {
+            shape = 0.0;
+        } else {
+            const double mu = cos(th);
+            const double vertical = exp(-sq(mu / dmax(m.H, 1.0e-6)));
+            const double inner_taper = 1.0 / (1.0 + exp(-(r - m.rmin) / dmax(0.35 * m.sr, 1.0e-6)));
+            const double outer_taper = 1.0 / (1.0 + exp((r - m.rmax) / dmax(0.60 * m.sr, 1.0e-6)));
+            const double inner_core = 0.45 * exp(-sq((r - 0.75 * m.r0) / dmax(0.75 * m.sr, 1.0e-6)));
+            const double radial = pow(dmax(r / m.r0, 1.0e-300), -dmax(m.p, 0.0));
+            const double floor_tail = 0.08 * exp(-sq((r - m.r0) / dmax(1.8 * m.sr, 1.0e-6)));
+            shape = (radial + inner_core + floor_tail) * vertical * inner_taper * outer_taper;
+        }
+    } else if (m.powerlaw_disk) {
+        if (!(r >= m.rmin && r <= m.rmax) || r <= 0.0 || m.r0 <= 0.0) {
+            shape = 0.0;
+        } else {
+            const double vertical = exp(-sq(cos(th) / m.H));
+            const double inner_taper = 1.0 - exp(-sq((r - m.rmin) / m.sr));
+            const double q = r / m.rmax;
+            const double outer_taper = exp(-(q * q) * (q * q));
+            const double radial = pow(r / m.r0, -m.p);
+            shape = radial * vertical * inner_taper * outer_taper;
+        }
+    } else {
+        const double delta = th - 0.5 * PI;
+        if (r <= 4.0 || r >= 18.0 || fabs(delta) >= 0.45) {
+            shape = 0.0;
+        } else {
+            shape = exp(-sq((r - m.r0) / m.sr)) * exp(-sq(delta / m.H));
+        }
+    }
+    double F = 1.0;
+    if (m.funnel) {
+        const double north = exp(-sq(th / m.ft));
+        const double south = exp(-sq((PI - th) / m.ft));
+        F = dclamp(1.0 - m.fd * (north + south), 0.0, 1.0);
+    }
+    double env = 0.0;
+    if (m.envelope && !(m.env0 <= 0.0 || r < m.r0 || r > m.rmax)) env = m.env0 * pow(r / m.r0, -m.ealpha);
+    return dmax(m.rho0 * shape * F + env, m.floor_);
+}
+
+// bipolar_gaussian (src/radiative_transfer.cpp:64-76)
+__device__ __forceinline__ double bipolar(double th, double th0, double w)
+{
+    if (w <= 0.0) return 0.0;
+    const double n = th - th0;
+    const double s = th - (PI - th0);
+    return exp(-(n / w) * (n / w)) + exp(-(s / w) * (s / w));
+}
+
+// norm x spatial factor of emissivity_uhe (src/radiative_transfer.cpp:209-395):
+// bin-independent, so evaluated once per path point.
+__device__ __forceinline__ double source_factor(const DevSource& so, const DevMedium& m, double r, double th)
+{
+    if (so.kind == HADROS_SOURCE_INNER_RING) {
+        if (!so.ring_ok) return 0.0;
+        const double dr = (r - so.rc) / so.sr;
+        const double dt = (th - 0.5 * PI) / so.tw;
+        return so.norm * exp(-dr * dr) * exp(-dt * dt);
+    }
+    if (r <= 0.0 || so.sr <= 0.0 || so.tw <= 0.0) return 0.0;
+    double W = 0.0;
+    if (so.kind == HADROS_SOURCE_FUNNEL_WALL) {
+        const double dr = (r - so.rc) / so.sr;
+        W = exp(-dr * dr) * bipolar(th, so.ft, so.tw);
+    } else if (so.kind == HADROS_SOURCE_JET_BASE) {
+        W = exp(-sq(r / so.rc)) * bipolar(th, 0.0, so.tw);
+    } else if (so.kind == HADROS_SOURCE_DENSITY_WEIGHTED) {
+        const double rh = rho(m, r, th);
+        const double rho_norm = dmax(so.rho_ref > 0.0 ? so.rho_ref : so.rho_norm_c, 1.0e-300);
+        const double dw = pow(dmax(rh / rho_norm, 0.0), so.q);
+        const double rw = pow(dmax(r / dmax(so.rc, 1.0e-300), 1.0e-300), -so.s);
+        W = dclamp(dw * rw, dmax(so.cmin, 0.0), dmax(so.cmax, dmax(so.cmin, 0.0)));
+    } else if (so.kind == HADROS_SOURCE_SHOCK_LAYER) {
+        const double dr = dmax(so.gdr, 1.0e-4);
+        const double dth = dmax(so.gdth, 1.0e-5);
+        const double rp = rho(m, r + dr, th);
+        const double rm = rho(m, dmax(r - dr, 1.0e-6), th);
+        const double tp = rho(m, r, dclamp(th + dth, 0.0, PI));
+        const double tm = rho(m, r, dclamp(th - dth, 0.0, PI));
+        const double rho_norm = dmax(so.rho_norm_c, 1.0e-300);
+        const double d_r = (rp - rm) / (2.0 * dr);
+        const double d_t = (tp - tm) / (2.0 * dth * dmax(r, 1.0e-6));
+        const double grad = sqrt(d_r * d_r + d_t * d_t) / rho_norm;
+        const double taper = exp(-sq(r / dmax(so.rc, 1.0e-300)));
+        W = dclamp(grad * taper, dmax(so.cmin, 0.0), dmax(so.cmax, dmax(so.cmin, 0.0)));
+    }
+    return so.norm * W;
+}
+
+// ------------------------------------------------------------------ sigma
+// LDS-staged table: ln E_k, ln sigma_k, slope_k = d ln sigma / d ln E on
+// [k-1, k] (slope_0 unused).  lower_bound over ln E as std::lower_bound
+// over E (src/sigma_table.cpp:100-124).
+__device__ __forceinline__ double sigma_ln(const double* __restrict__ lnE, const double* __restrict__ lnS,
+                                           const double* __restrict__ slope, int n, double lx)
+{
+    int lo = 0, cnt = n;
+    while (cnt > 0) {
+        const int half = cnt >> 1;
+        if (lnE[lo + half] < lx) {
+            lo += half + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    if (lo == 0) return lnS[0];
+    if (lo >= n) return lnS[n - 1];
+    return lnS[lo - 1] + (lx - lnE[lo - 1]) * slope[lo];
+}
+
+__device__ __forceinline__ float sigma_ln_f(const float* __restrict__ lnE, const float* __restrict__ lnS,
+                                            const float* __restrict__ slope, int n, float lx)
+{
+    int lo = 0, cnt = n;
+    while (cnt > 0) {
+        const int half = cnt >> 1;
+        if (lnE[lo + half] < lx) {
+            lo += half + 1;
+            cnt -= half + 1;
+        } else {
+            cnt = half;
+        }
+    }
+    if (lo == 0) return lnS[0];
+    if (lo >= n) return lnS[n - 1];
+    return lnS[lo - 1] + (lx - lnE[lo - 1]) * slope[lo];
+}
+
+// ------------------------------------------------------------------ transfer
+// Per-path-point quantities shared by all bins (integrate_kerr_ray,
+// src/radiative_transfer.cpp:501-672): g = 1/z, g^3, rho, n_N dl, emissivity
+// spatial factor.  Returns false for points the reference skips (r < r_+ + 1e-3).
+struct PointQ {
+    double z, lnz, g3Adl, nbar_dl, rho;
+};
+
+__device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, double dl, double z, PointQ& q)
+{
+    if (r < P.r_h_rt + 1.0e-3) return false;
+    const double zc = dmax(z, 1.0e-300);
+    const double g = 1.0 / zc;
+    const double dl_cm = dl * P.rg_cm;
+    q.rho = rho(P.med, r, th);
+    const double A = source_factor(P.src, P.med, r, th);
+    q.z = zc;
+    q.lnz = log(zc);
+    q.g3Adl = g * g * g * A * dl_cm;
+    q.nbar_dl = q.rho / M_U_G * dl_cm;
+    return true;
+}
+
+// FP64 transfer update of every bin for one path point:
+//   I_b   += g^3 j(E_b) e^{-tau_b} dl       (emission uses tau before the point)
+//   tau_b += n_N sigma(E_b) dl              if rho > 0 and E_b inside the table
+// with E_b = E_mid,b z and j = A S(E_b), S = exp(-gam ln E - E invcut).
+template <int NBMAX>
+__device__ __forceinline__ void point_rt64(const DevRt& P, const PointQ& q, const double* lnE, const double* lnS,
+                                           const double* slope, double* tau, double* I)
+{
+#pragma unroll
+    for (int b = 0; b < NBMAX; ++b) {
+        if (b >= P.nb) break;
+        const double E = P.emid[b] * q.z;
+        const double lnEb = P.lnemid[b] + q.lnz;
+        if (P.spec_ok && q.g3Adl != 0.0) I[b] += q.g3Adl * exp(-P.gam * lnEb - E * P.invcut - tau[b]);
+        if (q.rho > 0.0 && E >= P.e_lo && E <= P.e_hi) {
+            const double ls = sigma_ln(lnE, lnS, slope, P.n_sigma, lnEb);
+            tau[b] += q.nbar_dl * exp(ls);
+        }
+    }
+}
+
+// Kahan-compensated FP32 accumulator.
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ void add(float x)
+    {
+        const float y = x - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// I accumulator with a per-bin power-of-e offset so that FP32 holds values
+// like 1e-60: value = (s) * e^{off}.
+struct ScaledKahan {
+    Kahan k;
+    float off;   // 0 until the first non-zero term
+    int set;
+    __device__ __forceinline__ void add_log(float L)
+    {
+        if (!set) {
+            off = ceilf(L);
+            set = 1;
+        } else if (L - off > 40.0f) {
+            const float noff = ceilf(L);
+            const float sc = __expf(off - noff);
+            k.s *= sc;
+            k.c *= sc;
+            off = noff;
+        }
+        k.add(__expf(L - off));
+    }
+};
+
+}  // namespace hd

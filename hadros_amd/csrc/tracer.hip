@@ -1,0 +1,402 @@
+// tracer.hip -- fused per-pixel kernel: Kerr geodesic + tau + per-bin transfer.
+//
+// One ray per lane, state in registers, sigma table in LDS, nothing
+// materialised: the per-pixel loop of KerrCamera::trace_pixel
+// (src/kerr_camera.cpp:188-283) with integrate_kerr_ray_spectral
+// (src/radiative_transfer.cpp:457-780) folded into every emitted path point.
+// A wavefront owns an 8x8 pixel tile so that its 64 rays have similar
+// lengths (less tail divergence than a 64-pixel image row).
+#include "kernels.h"
+
+namespace hd {
+
+template <int MODE, int PREC, int NBMAX>
+__global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
+{
+    extern __shared__ double smem[];
+    const DevRt& P = A.rt;
+    const int n = P.n_sigma;
+    // ---- stage the sigma table in LDS
+    const double* lnE = smem;
+    const double* lnS = smem + n;
+    const double* slope = smem + 2 * n;
+    const float* lnEf = reinterpret_cast<const float*>(smem);
+    const float* lnSf = lnEf + n;
+    const float* slopef = lnEf + 2 * n;
+    if constexpr (PREC == HADROS_PREC_FP64) {
+        for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) smem[t] = A.sig[t];
+    } else {
+        float* sf = reinterpret_cast<float*>(smem);
+        for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) sf[t] = A.sigf[t];
+    }
+    __syncthreads();
+
+    const DevCam& C = A.cam;
+    const int lane = threadIdx.x & 63;
+    const long gw = (long(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int tiles_y = (C.ny + 7) >> 3;
+    const int ti = int(gw / tiles_y), tj = int(gw % tiles_y);
+    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    const bool valid = (i < C.nx) && (j < C.ny) && (ti * 8 < C.nx);
+    const double a = C.a;
+
+    // ---- initial state, src/kerr_camera.cpp:110-186 (observer metric from host)
+    const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
+    const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;
+    const double norm = sqrt(1.0 + u * u + v * v);
+    const double n_r = -1.0 / norm, n_th = v / norm, n_ph = u / norm;
+    const double pt_c = 1.0 / C.alpha;
+    const double pr_c = n_r / sqrt(C.g_rr);
+    const double pth_c = n_th / sqrt(C.g_thth);
+    const double pph_c = n_ph / sqrt(C.g_phph) + C.omega * pt_c;
+    const double pt = C.g_tt * pt_c + C.g_tph * pph_c;
+    const double pph = C.g_tph * pt_c + C.g_phph * pph_c;
+    Y5 y{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
+    const double e_obs = -(pt + C.omega * pph) / C.alpha;
+    const double e_obs_c = dmax(e_obs, 1.0e-300);
+
+    const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
+
+    double tau[NBMAX], I[NBMAX];
+    Kahan ktau[NBMAX];
+    ScaledKahan kI[NBMAX];
+#pragma unroll
+    for (int b = 0; b < NBMAX; ++b) {
+        tau[b] = 0.0;
+        I[b] = 0.0;
+        ktau[b] = {0.0f, 0.0f};
+        kI[b] = {{0.0f, 0.0f}, 0.0f, 0};
+    }
+
+    int step = 0, captured = 0, exhausted = 0, npts = 0;
+    long attempts = 0;
+    bool done = !valid;
+    // step 0: termination checks, then the observer point (dl = 0 adds nothing)
+    if (!done) {
+        if ((MODE == HADROS_MODE_CFG) ? (y.r < r_stop) : (y.r <= r_stop)) {
+            captured = 1;
+            done = true;
+        } else {
+            npts = 1;
+        }
+    }
+    Y5 yp = y;
+    double h = C.h;
+    const double h_min = C.h * 1.0e-5;
+    int att_in_step = 0;
+    const long max_attempts = 8L * C.max_steps + 64;
+
+    while (!done) {
+        Y5 y4, y5;
+        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        ++attempts;
+        bool acc;
+        if constexpr (MODE == HADROS_MODE_REF) {
+            // KerrGeodesic::step_adaptive (src/kerr_geodesic.cpp:235-328)
+            const double err = err_ref(y4, y5);
+            acc = (err < C.tol) || (h <= h_min);
+            if (!acc) {
+                const double factor = 0.8 * pow(C.tol / dmax(err, 1.0e-30), 0.25);
+                h *= dclamp(factor, 0.1, 5.0);
+                if (h < h_min) h = h_min;
+                if (++att_in_step == 50) {
+                    y5 = rk4_ref(a, y, pt, pph, C.h);
+                    acc = true;
+                }
+            }
+            if (acc) {
+                h = C.h;
+                att_in_step = 0;
+            }
+        } else {
+            const double err = err_cfg(y, y4, y5, C.tol);
+            acc = (err <= 1.0) || (h <= CFG_H_MIN);
+            if (acc) {
+                double fac = (err > 0.0) ? 0.9 * pow(err, -0.2) : 5.0;
+                fac = fmin(5.0, fmax(0.2, fac));
+                h = fmax(h * fac, CFG_H_MIN);
+            } else {
+                double fac = (err == err) ? 0.9 * pow(err, -0.2) : 0.1;
+                fac = fmin(0.9, fmax(0.1, fac));
+                h = fmax(h * fac, CFG_H_MIN);
+                if (attempts >= max_attempts) {
+                    exhausted = 1;
+                    break;
+                }
+            }
+        }
+        if (!acc) continue;
+        yp = y;
+        y = y5;
+        ++step;
+        if (step >= C.max_steps) {
+            exhausted = 1;
+            break;
+        }
+        if ((MODE == HADROS_MODE_CFG) ? (y.r < r_stop) : (y.r <= r_stop)) {
+            captured = 1;
+            break;
+        }
+        if ((MODE == HADROS_MODE_CFG) ? (y.r > C.r_max) : (y.r >= C.r_max)) {
+            if (step > 10) break;
+        }
+        // ---- emit path point `step` (src/kerr_camera.cpp:236-273) and integrate it
+        ++npts;
+        const double dl = zamo_dl(a, y, yp);
+        const double z = zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c;
+        PointQ q;
+        if (point_q(P, y.r, y.th, dl, z, q)) {
+            if constexpr (PREC == HADROS_PREC_FP64) {
+                point_rt64<NBMAX>(P, q, lnE, lnS, slope, tau, I);
+            } else {
+                const float lnz = float(q.lnz);
+                const float lP = (q.g3Adl > 0.0) ? float(log(q.g3Adl)) : 0.0f;
+                const float lN = (q.nbar_dl > 0.0) ? float(log(q.nbar_dl)) : 0.0f;
+                const float gam = float(P.gam), invcut = float(P.invcut);
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    if (b >= P.nb) break;
+                    const double E = P.emid[b] * q.z;
+                    const float lnEb = float(P.lnemid[b]) + lnz;
+                    if (P.spec_ok && q.g3Adl > 0.0)
+                        kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
+                    if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
+                        ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb)));
+                }
+            }
+        }
+    }
+
+    // ---- outputs
+    const unsigned long long c_pts = valid ? (unsigned long long)npts : 0ull;
+    unsigned long long c_att = valid ? (unsigned long long)attempts : 0ull;
+    unsigned long long c_cap = (valid && captured) ? 1ull : 0ull;
+    unsigned long long c_ex = (valid && exhausted) ? 1ull : 0ull;
+    if (valid) {
+        const long k = long(i) * C.ny + j;
+        // integrate_kerr_ray_spectral aggregation (src/radiative_transfer.cpp:711-769)
+        double bt[NBMAX], bp[NBMAX], bi[NBMAX];
+#pragma unroll
+        for (int b = 0; b < NBMAX; ++b) {
+            if (PREC == HADROS_PREC_FP64) {
+                bt[b] = tau[b];
+                bi[b] = I[b];
+            } else {
+                bt[b] = double(ktau[b].s) - double(ktau[b].c);
+                bi[b] = kI[b].set ? (double(kI[b].k.s) - double(kI[b].k.c)) * exp(double(kI[b].off)) : 0.0;
+            }
+            bp[b] = exp(-bt[b]);
+        }
+        double agg_t = 0.0, agg_p = 1.0, agg_i = 0.0;
+        if (P.mono) {
+            if (P.nb > 0) {
+                agg_t = bt[0];
+                agg_p = bp[0];
+                agg_i = bi[0];
+            }
+        } else {
+            double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
+#pragma unroll
+            for (int b = 0; b < NBMAX; ++b) {
+                if (b >= P.nb) break;
+                ws += P.w[b];
+                ts += P.w[b] * bt[b];
+                ps += P.w[b] * bp[b];
+                is += bi[b] * P.de[b];
+            }
+            if (ws > 0.0) {
+                agg_t = ts / ws;
+                agg_p = ps / ws;
+                agg_i = is;
+            }
+        }
+        const TraceOut& O = A.out;
+        if (O.tau) O.tau[k] = agg_t;
+        if (O.p) O.p[k] = agg_p;
+        if (O.I) O.I[k] = agg_i;
+        if (O.captured) O.captured[k] = captured;
+        if (O.npoints) O.npoints[k] = int(c_pts);
+        if (O.flags) O.flags[k] = exhausted ? HADROS_FLAG_EXHAUSTED : 0;
+        if (O.alpha) O.alpha[k] = u;
+        if (O.beta) O.beta[k] = v;
+#pragma unroll
+        for (int b = 0; b < NBMAX; ++b) {
+            if (b >= P.nb) break;
+            if (O.btau) O.btau[k * P.nb + b] = bt[b];
+            if (O.bp) O.bp[k * P.nb + b] = bp[b];
+            if (O.bi) O.bi[k * P.nb + b] = bi[b];
+        }
+    }
+    wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
+}
+
+// ---- geodesic cache mode: same trace, PathPoints written to [offset[k], ...)
+template <int MODE, bool WRITE>
+__global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
+{
+    const DevCam& C = A.cam;
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= long(C.nx) * C.ny) return;
+    const int i = int(k / C.ny), j = int(k % C.ny);
+    const double a = C.a;
+    const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
+    const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;
+    const double norm = sqrt(1.0 + u * u + v * v);
+    const double n_r = -1.0 / norm, n_th = v / norm, n_ph = u / norm;
+    const double pt_c = 1.0 / C.alpha;
+    const double pr_c = n_r / sqrt(C.g_rr);
+    const double pth_c = n_th / sqrt(C.g_thth);
+    const double pph_c = n_ph / sqrt(C.g_phph) + C.omega * pt_c;
+    const double pt = C.g_tt * pt_c + C.g_tph * pph_c;
+    const double pph = C.g_tph * pt_c + C.g_phph * pph_c;
+    Y5 y{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
+    const double e_obs_c = dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);
+    const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
+    double* out = nullptr;
+    long cap = 0;
+    if constexpr (WRITE) {
+        out = A.points + 7 * A.offset[k];
+        cap = A.offset[k + 1] - A.offset[k];
+    }
+    auto emit = [&](long idx, const Y5& yy, double dl, double z) {
+        if constexpr (WRITE) {
+            if (idx < cap) {
+                double st, ct, sp, cp;
+                sincos(yy.th, &st, &ct);
+                sincos(yy.ph, &sp, &cp);
+                double* o = out + 7 * idx;
+                o[0] = yy.r;
+                o[1] = yy.th;
+                o[2] = yy.r * st * cp;
+                o[3] = yy.r * st * sp;
+                o[4] = yy.r * ct;
+                o[5] = dl;
+                o[6] = z;
+            }
+        }
+    };
+    int step = 0, captured = 0;
+    long npts = 0;
+    bool done = false;
+    if ((MODE == HADROS_MODE_CFG) ? (y.r < r_stop) : (y.r <= r_stop)) {
+        captured = 1;
+        done = true;
+    } else {
+        emit(npts++, y, 0.0, zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+    }
+    Y5 yp = y;
+    double h = C.h;
+    const double h_min = C.h * 1.0e-5;
+    int att_in_step = 0;
+    long attempts = 0;
+    const long max_attempts = 8L * C.max_steps + 64;
+    while (!done) {
+        Y5 y4, y5;
+        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        ++attempts;
+        bool acc;
+        if constexpr (MODE == HADROS_MODE_REF) {
+            const double err = err_ref(y4, y5);
+            acc = (err < C.tol) || (h <= h_min);
+            if (!acc) {
+                const double factor = 0.8 * pow(C.tol / dmax(err, 1.0e-30), 0.25);
+                h *= dclamp(factor, 0.1, 5.0);
+                if (h < h_min) h = h_min;
+                if (++att_in_step == 50) {
+                    y5 = rk4_ref(a, y, pt, pph, C.h);
+                    acc = true;
+                }
+            }
+            if (acc) {
+                h = C.h;
+                att_in_step = 0;
+            }
+        } else {
+            const double err = err_cfg(y, y4, y5, C.tol);
+            acc = (err <= 1.0) || (h <= CFG_H_MIN);
+            if (acc) {
+                double fac = (err > 0.0) ? 0.9 * pow(err, -0.2) : 5.0;
+                fac = fmin(5.0, fmax(0.2, fac));
+                h = fmax(h * fac, CFG_H_MIN);
+            } else {
+                double fac = (err == err) ? 0.9 * pow(err, -0.2) : 0.1;
+                fac = fmin(0.9, fmax(0.1, fac));
+                h = fmax(h * fac, CFG_H_MIN);
+                if (attempts >= max_attempts) break;
+            }
+        }
+        if (!acc) continue;
+        yp = y;
+        y = y5;
+        ++step;
+        if (step >= C.max_steps) break;
+        if ((MODE == HADROS_MODE_CFG) ? (y.r < r_stop) : (y.r <= r_stop)) {
+            captured = 1;
+            break;
+        }
+        if ((MODE == HADROS_MODE_CFG) ? (y.r > C.r_max) : (y.r >= C.r_max)) {
+            if (step > 10) break;
+        }
+        emit(npts++, y, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+    }
+    if constexpr (!WRITE) {
+        A.counts[k] = npts;
+        A.captured[k] = captured;
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ launchers
+template <int MODE, int PREC>
+static hipError_t launch_nb(const TraceArgs& A, int block, size_t lds, hipStream_t s, int grid)
+{
+    if (A.rt.nb <= 1) {
+        hipLaunchKernelGGL((k_trace_rt<MODE, PREC, 1>), dim3(grid), dim3(block), lds, s, A);
+    } else if (A.rt.nb <= 8) {
+        hipLaunchKernelGGL((k_trace_rt<MODE, PREC, 8>), dim3(grid), dim3(block), lds, s, A);
+    } else {
+        hipLaunchKernelGGL((k_trace_rt<MODE, PREC, 16>), dim3(grid), dim3(block), lds, s, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStream_t s)
+{
+    const int tiles = ((A.cam.nx + 7) / 8) * ((A.cam.ny + 7) / 8);
+    const long threads = long(tiles) * 64;
+    const int grid = int((threads + block - 1) / block);
+    const size_t lds = size_t(3) * A.rt.n_sigma * (precision == HADROS_PREC_FP64 ? sizeof(double) : sizeof(float));
+    if (A.cam.mode == HADROS_MODE_CFG) {
+        return precision == HADROS_PREC_FP64 ? launch_nb<HADROS_MODE_CFG, HADROS_PREC_FP64>(A, block, lds, s, grid)
+                                             : launch_nb<HADROS_MODE_CFG, HADROS_PREC_FP32K>(A, block, lds, s, grid);
+    }
+    return precision == HADROS_PREC_FP64 ? launch_nb<HADROS_MODE_REF, HADROS_PREC_FP64>(A, block, lds, s, grid)
+                                         : launch_nb<HADROS_MODE_REF, HADROS_PREC_FP32K>(A, block, lds, s, grid);
+}
+
+hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s)
+{
+    const long n = long(A.cam.nx) * A.cam.ny;
+    const int block = 64;
+    const int grid = int((n + block - 1) / block);
+    if (A.cam.mode == HADROS_MODE_CFG) {
+        if (write) hipLaunchKernelGGL((k_trace_cache<HADROS_MODE_CFG, true>), dim3(grid), dim3(block), 0, s, A);
+        else hipLaunchKernelGGL((k_trace_cache<HADROS_MODE_CFG, false>), dim3(grid), dim3(block), 0, s, A);
+    } else {
+        if (write) hipLaunchKernelGGL((k_trace_cache<HADROS_MODE_REF, true>), dim3(grid), dim3(block), 0, s, A);
+        else hipLaunchKernelGGL((k_trace_cache<HADROS_MODE_REF, false>), dim3(grid), dim3(block), 0, s, A);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hd

@@ -1,0 +1,359 @@
+"""ctypes binding of libhadros.so (the C-ABI of include/hadros.h).
+
+This is the Python face of the drop-in boundary: the reference's Python
+drivers reach the hot path only through `subprocess` calls of the CLIs
+(scripts/run_robustness_scans.py:60-70,147-165); here the same path is a
+library call.  There is no CPU fallback: if the library is missing or no
+gfx950 device is present, every compute call raises HadrosError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhadros.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_lib = None
+
+# every symbol include/hadros.h declares
+EXPORTS = [
+    "hadros_last_error", "hadros_abi_version", "hadros_device_info", "hadros_camera_defaults",
+    "hadros_scene_defaults", "hadros_parse_profile", "hadros_parse_source", "hadros_parse_spectrum",
+    "hadros_profile_name", "hadros_source_name", "hadros_spectrum_name", "hadros_sigma_load",
+    "hadros_sigma_check", "hadros_trace_rt", "hadros_tracer_create", "hadros_tracer_set_block",
+    "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_fetch", "hadros_tracer_destroy",
+    "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
+    "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_eval_rho",
+    "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
+]
+
+
+class HadrosError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hadros error {code}: {msg}")
+        self.code = code
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def lib():
+    """Load libhadros.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HadrosError(abi.ERR_DEVICE, f"{LIB_PATH} not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    L.hadros_last_error.restype = C.c_char_p
+    for f in ("hadros_profile_name", "hadros_source_name", "hadros_spectrum_name"):
+        getattr(L, f).restype = C.c_char_p
+        getattr(L, f).argtypes = [C.c_int32]
+    L.hadros_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.hadros_camera_defaults.argtypes = [C.POINTER(abi.Camera)]
+    L.hadros_scene_defaults.argtypes = [C.POINTER(abi.Scene)]
+    for f in ("hadros_parse_profile", "hadros_parse_source", "hadros_parse_spectrum"):
+        getattr(L, f).argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
+    L.hadros_sigma_load.argtypes = [C.c_char_p, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_int)]
+    L.hadros_sigma_check.argtypes = [_dp, _dp, C.c_int]
+    L.hadros_trace_rt.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
+                                  C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
+    L.hadros_tracer_create.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p)]
+    L.hadros_tracer_set_block.argtypes = [C.c_void_p, C.c_int]
+    L.hadros_tracer_run.argtypes = [C.c_void_p, C.c_void_p]
+    L.hadros_tracer_last_ms.argtypes = [C.c_void_p, _dp]
+    L.hadros_tracer_fetch.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
+    L.hadros_tracer_destroy.argtypes = [C.c_void_p]
+    L.hadros_rt_from_cache.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
+                                       C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
+    L.hadros_replay_create.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p)]
+    L.hadros_replay_run.argtypes = [C.c_void_p, C.c_void_p]
+    L.hadros_replay_last_ms.argtypes = [C.c_void_p, _dp]
+    L.hadros_replay_fetch.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
+    L.hadros_replay_destroy.argtypes = [C.c_void_p]
+    L.hadros_trace_cache.argtypes = [C.POINTER(abi.Camera), _lp, _ip, _dp, C.c_int64]
+    L.hadros_eval_rho.argtypes = [C.POINTER(abi.Medium), _dp, _dp, _dp, C.c_int64]
+    L.hadros_eval_sigma.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int64]
+    L.hadros_eval_emissivity.argtypes = [C.POINTER(abi.Scene), _dp, _dp, _dp, _dp, C.c_int64]
+    L.hadros_eval_rhs.argtypes = [C.c_double, C.c_int, _dp, _dp, C.c_int64]
+    L.hadros_eval_metric.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int64]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != abi.OK:
+        raise HadrosError(rc, lib().hadros_last_error().decode(errors="replace"))
+
+
+def device_info():
+    L = lib()
+    buf = C.create_string_buffer(256)
+    cu, arch = C.c_int(), C.c_int()
+    check(L.hadros_device_info(buf, 256, C.byref(cu), C.byref(arch)))
+    return buf.value.decode(), cu.value, arch.value
+
+
+def parse_profile(name):
+    k = C.c_int32()
+    check(lib().hadros_parse_profile(name.encode(), C.byref(k)))
+    return k.value
+
+
+def parse_source(name):
+    k = C.c_int32()
+    check(lib().hadros_parse_source(name.encode(), C.byref(k)))
+    return k.value
+
+
+def parse_spectrum(name):
+    k = C.c_int32()
+    check(lib().hadros_parse_spectrum(name.encode(), C.byref(k)))
+    return k.value
+
+
+def sigma_load(path):
+    """SigmaTable(path): returns (E_GeV, sigma_GeV^-2, sigma_cm2) arrays."""
+    L = lib()
+    n = C.c_int()
+    check(L.hadros_sigma_load(path.encode(), None, None, None, 0, C.byref(n)))
+    e, g, s = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+    check(L.hadros_sigma_load(path.encode(), dptr(e), dptr(g), dptr(s), n.value, C.byref(n)))
+    return e, g, s
+
+
+class Image:
+    """Host arrays of one camera's (or cache's) outputs, pixel k = i*ny + j."""
+
+    def __init__(self, npix, nbins):
+        self.nbins = max(nbins, 0)
+        self.tau = np.zeros(npix)
+        self.p = np.zeros(npix)
+        self.I = np.zeros(npix)
+        self.captured = np.zeros(npix, np.int32)
+        self.npoints = np.zeros(npix, np.int32)
+        self.flags = np.zeros(npix, np.int32)
+        self.alpha = np.zeros(npix)
+        self.beta = np.zeros(npix)
+        self.bin_tau = np.zeros((npix, self.nbins))
+        self.bin_p = np.zeros((npix, self.nbins))
+        self.bin_i = np.zeros((npix, self.nbins))
+
+    def out(self):
+        o = abi.ImageOut()
+        o.tau, o.p_surv, o.intensity = dptr(self.tau), dptr(self.p), dptr(self.I)
+        o.captured, o.npoints, o.flags = iptr(self.captured), iptr(self.npoints), iptr(self.flags)
+        o.alpha, o.beta = dptr(self.alpha), dptr(self.beta)
+        if self.nbins:
+            o.bin_tau, o.bin_p, o.bin_i = dptr(self.bin_tau), dptr(self.bin_p), dptr(self.bin_i)
+        return o
+
+
+def _sig(sig_e, sig_s):
+    return np.ascontiguousarray(sig_e, np.float64), np.ascontiguousarray(sig_s, np.float64)
+
+
+class Tracer:
+    """Re-usable plan of hadros_trace_rt (device buffers allocated once)."""
+
+    def __init__(self, cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0):
+        L = lib()
+        self._se, self._ss = _sig(sig_e, sig_s)
+        self.cam = cam
+        h = C.c_void_p()
+        check(L.hadros_tracer_create(C.byref(cam), C.byref(scene), dptr(self._se), dptr(self._ss), self._se.size,
+                                     precision, C.byref(h)))
+        self.h = h
+        if block:
+            check(L.hadros_tracer_set_block(self.h, block))
+
+    def run(self, stream=None):
+        check(lib().hadros_tracer_run(self.h, stream))
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().hadros_tracer_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def fetch(self):
+        st = abi.Stats()
+        check(lib().hadros_tracer_fetch(self.h, None, C.byref(st)))
+        img = Image(self.cam.nx * self.cam.ny, st.n_bins)
+        o = img.out()
+        check(lib().hadros_tracer_fetch(self.h, C.byref(o), C.byref(st)))
+        return img, st
+
+    def close(self):
+        if self.h:
+            lib().hadros_tracer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def trace_rt(cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0):
+    t = Tracer(cam, scene, sig_e, sig_s, precision, block)
+    try:
+        t.run()
+        return t.fetch()
+    finally:
+        t.close()
+
+
+class Cache:
+    """A geodesic cache as SoA numpy arrays (what a KGEO file holds)."""
+
+    def __init__(self, a_bh, offset, r, theta, dl, redshift, captured=None, pixel_i=None, pixel_j=None,
+                 alpha=None, beta=None, x=None, y=None, z=None, nx=0, ny=0):
+        self.a_bh = float(a_bh)
+        self.offset = np.ascontiguousarray(offset, np.int64)
+        self.r = np.ascontiguousarray(r, np.float64)
+        self.theta = np.ascontiguousarray(theta, np.float64)
+        self.dl = np.ascontiguousarray(dl, np.float64)
+        self.redshift = np.ascontiguousarray(redshift, np.float64)
+        n = self.offset.size - 1
+        self.captured = captured if captured is not None else np.zeros(n, np.int32)
+        self.pixel_i, self.pixel_j = pixel_i, pixel_j
+        self.alpha, self.beta = alpha, beta
+        self.x, self.y, self.z = x, y, z
+        self.nx, self.ny = nx, ny
+
+    @property
+    def n_rays(self):
+        return self.offset.size - 1
+
+    def view(self):
+        v = abi.CacheView()
+        v.n_rays = self.n_rays
+        v.a_bh = self.a_bh
+        v.offset = self.offset.ctypes.data_as(_lp)
+        v.r, v.theta, v.dl, v.redshift = dptr(self.r), dptr(self.theta), dptr(self.dl), dptr(self.redshift)
+        return v
+
+
+class Replay:
+    """Re-usable plan of hadros_rt_from_cache with the cache resident in HBM."""
+
+    def __init__(self, cache, scene, sig_e, sig_s, precision=abi.PREC_FP64):
+        L = lib()
+        self._se, self._ss = _sig(sig_e, sig_s)
+        self.n = cache.n_rays
+        self._v = cache.view()
+        self._keep = cache
+        h = C.c_void_p()
+        check(L.hadros_replay_create(C.byref(self._v), C.byref(scene), dptr(self._se), dptr(self._ss),
+                                     self._se.size, precision, C.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        check(lib().hadros_replay_run(self.h, stream))
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().hadros_replay_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def fetch(self):
+        st = abi.Stats()
+        check(lib().hadros_replay_fetch(self.h, None, C.byref(st)))
+        img = Image(self.n, st.n_bins)
+        o = img.out()
+        o.captured = o.npoints = o.flags = None
+        o.alpha = o.beta = None
+        check(lib().hadros_replay_fetch(self.h, C.byref(o), C.byref(st)))
+        return img, st
+
+    def close(self):
+        if self.h:
+            lib().hadros_replay_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rt_from_cache(cache, scene, sig_e, sig_s, precision=abi.PREC_FP64):
+    p = Replay(cache, scene, sig_e, sig_s, precision)
+    try:
+        p.run()
+        return p.fetch()
+    finally:
+        p.close()
+
+
+def trace_cache(cam):
+    """GPU geodesic cache of a camera: returns a Cache with x, y, z as well."""
+    L = lib()
+    npix = cam.nx * cam.ny
+    off = np.zeros(npix + 1, np.int64)
+    cap = np.zeros(npix, np.int32)
+    check(L.hadros_trace_cache(C.byref(cam), off.ctypes.data_as(_lp), iptr(cap), None, 0))
+    total = int(off[-1])
+    pts = np.zeros((max(total, 1), 7))
+    check(L.hadros_trace_cache(C.byref(cam), off.ctypes.data_as(_lp), iptr(cap), dptr(pts), total))
+    pts = pts[:total]
+    k = np.arange(npix)
+    i, j = k // cam.ny, k % cam.ny
+    th = np.tan(0.5 * cam.fov_deg * np.pi / 180.0)
+    alpha = (2.0 * (i + 0.5) / cam.nx - 1.0) * th
+    beta = (2.0 * (j + 0.5) / cam.ny - 1.0) * th
+    return Cache(cam.a_spin, off, pts[:, 0], pts[:, 1], pts[:, 5], pts[:, 6], captured=cap, pixel_i=i, pixel_j=j,
+                 alpha=alpha, beta=beta, x=pts[:, 2], y=pts[:, 3], z=pts[:, 4], nx=cam.nx, ny=cam.ny)
+
+
+def eval_rho(medium, r, th):
+    r = np.ascontiguousarray(r, np.float64)
+    th = np.ascontiguousarray(th, np.float64)
+    out = np.zeros(r.size)
+    check(lib().hadros_eval_rho(C.byref(medium), dptr(r), dptr(th), dptr(out), r.size))
+    return out
+
+
+def eval_sigma(sig_e, sig_s, e):
+    se, ss = _sig(sig_e, sig_s)
+    e = np.ascontiguousarray(e, np.float64)
+    out = np.zeros(e.size)
+    check(lib().hadros_eval_sigma(dptr(se), dptr(ss), se.size, dptr(e), dptr(out), e.size))
+    return out
+
+
+def eval_emissivity(scene, r, th, e):
+    r, th, e = (np.ascontiguousarray(x, np.float64) for x in (r, th, e))
+    out = np.zeros(r.size)
+    check(lib().hadros_eval_emissivity(C.byref(scene), dptr(r), dptr(th), dptr(e), dptr(out), r.size))
+    return out
+
+
+def eval_rhs(a, mode, y):
+    y = np.ascontiguousarray(y, np.float64).reshape(-1, 8)
+    out = np.zeros_like(y)
+    check(lib().hadros_eval_rhs(a, mode, dptr(y), dptr(out), y.shape[0]))
+    return out
+
+
+def eval_metric(a, r, th):
+    r = np.ascontiguousarray(r, np.float64)
+    th = np.ascontiguousarray(th, np.float64)
+    out = np.zeros((r.size, 10))
+    check(lib().hadros_eval_metric(a, dptr(r), dptr(th), dptr(out), r.size))
+    return out

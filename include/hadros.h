@@ -1,0 +1,335 @@
+/*
+ * hadros.h -- C-ABI of the MI355X (gfx950) per-pixel Kerr tracer:
+ * null-geodesic integration, DIS optical depth tau = int n_N sigma_CC(E) dl,
+ * and per-energy-bin transfer I = int j e^{-tau} dl.
+ *
+ * Plain C: structs of doubles and int32 pairs, caller-owned buffers, no C++
+ * types, no exceptions across the boundary.  Every entry point returns an
+ * int status (HADROS_OK == 0); the message of the last failure on the calling
+ * thread is available from hadros_last_error().
+ *
+ * The reference exposes this path as C++ value classes plus positional-argv
+ * programs; each declaration below names the reference interface it replaces.
+ * All compute entry points run on the current HIP device (gfx950).  There is
+ * no CPU fallback: without a usable device they fail with HADROS_ERR_DEVICE.
+ */
+#ifndef HADROS_H
+#define HADROS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HADROS_ABI_VERSION 1
+
+/* ---- status codes (reference failure modes, see each entry point) ------- */
+enum {
+    HADROS_OK = 0,
+    HADROS_ERR_ARG = 1,              /* invalid argument / shape               */
+    HADROS_ERR_TABLE = 2,            /* sigma table: <2 rows or E not increasing
+                                        (src/sigma_table.cpp:42-52)            */
+    HADROS_ERR_RANGE = 3,            /* energy outside sigma table
+                                        (src/sigma_table.cpp:86-94)            */
+    HADROS_ERR_KIND = 4,             /* unknown profile/source/spectrum name
+                                        (src/torus_profile.cpp:85-90,
+                                         src/radiative_transfer.cpp:108-111,152-155) */
+    HADROS_ERR_CACHE_MAGIC = 5,      /* KGEO bad magic
+                                        (apps/compute_kerr_image_from_cache.cpp:388-391) */
+    HADROS_ERR_CACHE_VERSION = 6,    /* KGEO version != 1 (:393-397)            */
+    HADROS_ERR_CACHE_TRUNCATED = 7,  /* EOF mid-ray or npoints < 0 (:672-700)   */
+    HADROS_ERR_DEVICE = 8,           /* HIP runtime failure / no gfx950 device  */
+    HADROS_ERR_IO = 9,               /* file open/write failure                 */
+    HADROS_ERR_MAX_POINTS = 10       /* a traced ray exceeded the point buffer
+                                        (apps/compute_kerr_image_cuda.cu:1787-1823) */
+};
+
+/* ---- integrator semantics ---------------------------------------------- */
+enum {
+    /* Reference semantics (src/kerr_geodesic.cpp:194-329, src/kerr_camera.cpp:257-351):
+       finite-difference d g^{mu nu}, RKF45 restarted from h every step,
+       absolute max-norm error < tol, 50 attempts then RK4, stop at
+       r <= r_+ + 1e-3 or (r >= r_max and step > 10), max_steps points. */
+    HADROS_MODE_REF = 0,
+    /* Error-controlled semantics: analytic Hamiltonian derivatives, RKF45
+       with step growth, mixed error |y5-y4| <= tol*(1+|y|), stop at
+       r < 1.01 r_+ or (r > r_max and step > 10), max_steps points. */
+    HADROS_MODE_CFG = 1
+};
+
+enum {
+    HADROS_PREC_FP64 = 0,            /* everything FP64                        */
+    HADROS_PREC_FP32K = 1            /* FP64 geodesic + FP32 Kahan tau / I     */
+};
+
+/* density morphologies, src/torus_profile.cpp:44-91 (include/torus_profile.hpp:6-15) */
+enum {
+    HADROS_PROFILE_GAUSSIAN = 0,
+    HADROS_PROFILE_POWERLAW = 1,
+    HADROS_PROFILE_GAUSSIAN_FUNNEL = 2,
+    HADROS_PROFILE_POWERLAW_FUNNEL = 3,
+    HADROS_PROFILE_GAUSSIAN_ENVELOPE = 4,
+    HADROS_PROFILE_POWERLAW_ENVELOPE = 5,
+    HADROS_PROFILE_POWERLAW_FUNNEL_ENVELOPE = 6,
+    HADROS_PROFILE_COLLAPSAR_NDAF_LIKE = 7
+};
+
+/* UHE source morphologies, include/radiative_transfer.hpp:39-45 */
+enum {
+    HADROS_SOURCE_INNER_RING = 0,
+    HADROS_SOURCE_FUNNEL_WALL = 1,
+    HADROS_SOURCE_JET_BASE = 2,
+    HADROS_SOURCE_SHOCK_LAYER = 3,
+    HADROS_SOURCE_DENSITY_WEIGHTED = 4
+};
+
+/* UHE spectral models, include/radiative_transfer.hpp:23-27 */
+enum {
+    HADROS_SPECTRUM_MONOCHROMATIC = 0,
+    HADROS_SPECTRUM_POWERLAW = 1,
+    HADROS_SPECTRUM_POWERLAW_CUTOFF = 2
+};
+
+#define HADROS_MAX_BINS 16
+#define HADROS_MAX_SIGMA 1024
+
+/* Camera: replaces KerrCamera(a, r_obs, theta_obs, fov_deg, nx, ny, r_max, h)
+   (include/kerr_camera.hpp:10-19) plus the integrator knobs that the
+   reference hard-codes (tolerance include/kerr_geodesic.hpp:12,
+   max steps src/kerr_camera.cpp:222). */
+typedef struct hadros_camera {
+    double a_spin;          /* dimensionless spin a                          */
+    double r_obs;           /* observer radius [r_g]                         */
+    double theta_obs_rad;   /* observer inclination [rad]                    */
+    double fov_deg;         /* full field of view [deg]                      */
+    int32_t nx, ny;         /* pixels                                        */
+    double r_max;           /* escape radius [r_g]                           */
+    double h;               /* (initial) affine step                         */
+    double tol;             /* REF: absolute error tol; CFG: relative tol    */
+    int32_t mode;           /* HADROS_MODE_*                                 */
+    int32_t max_steps;      /* max emitted points per ray                    */
+} hadros_camera;
+
+/* Density: replaces TorusProfile(...) (include/torus_profile.hpp:19-33). */
+typedef struct hadros_medium {
+    int32_t kind;           /* HADROS_PROFILE_*                              */
+    int32_t reserved;
+    double rho0;            /* g/cm^3                                        */
+    double r0;              /* r_g                                           */
+    double sigma_r;         /* r_g                                           */
+    double h_over_r;
+    double radial_power;
+    double funnel_depletion;
+    double funnel_theta_rad;
+    double envelope_rho0;   /* g/cm^3                                        */
+    double envelope_alpha;
+    double r_min, r_max;    /* r_g                                           */
+    double rho_floor;       /* g/cm^3                                        */
+} hadros_medium;
+
+/* UHE source: replaces UHESourceParams (include/radiative_transfer.hpp:47-64). */
+typedef struct hadros_source {
+    int32_t kind;           /* HADROS_SOURCE_*                               */
+    int32_t reserved;
+    double r_center;
+    double sigma_r;
+    double theta_width_rad;
+    double powerlaw;
+    double emax_GeV;
+    double norm;
+    double funnel_theta_rad;
+    double density_q;
+    double radial_s;
+    double rho_ref;
+    double cutoff_min, cutoff_max;
+    double gradient_dr;
+    double gradient_dtheta_rad;
+} hadros_source;
+
+/* UHE spectrum: replaces UHESpectralParams (include/radiative_transfer.hpp:29-37). */
+typedef struct hadros_spectrum {
+    int32_t kind;           /* HADROS_SPECTRUM_*                             */
+    int32_t n_bins;
+    double gamma;
+    double ecut_GeV;
+    double e_min_GeV, e_max_GeV;
+} hadros_spectrum;
+
+/* Everything the transfer integral needs besides the ray and the sigma table:
+   the argv block of apps/compute_kerr_image_from_cache.cpp:57-205 that is on
+   the UHE path. */
+typedef struct hadros_scene {
+    hadros_medium medium;
+    hadros_source source;
+    hadros_spectrum spectrum;
+    double enu_obs_GeV;     /* observed energy (monochromatic)               */
+    double m_bh_msun;       /* black-hole mass [M_sun] -> r_g in cm           */
+} hadros_scene;
+
+/* Per-pixel outputs (caller-owned, host memory unless stated otherwise).
+   Pixel index k = i*ny + j (ray_id of apps/compute_kerr_geodesics.cpp:110).
+   tau/p_surv/intensity are the aggregates written to the image rows
+   (src/radiative_transfer.cpp:763-769); bin_* hold the per-bin values
+   (k*n_bins + b), bins skipped by the reference (w <= 0) hold tau=0, P=1, I=0.
+   Any pointer may be NULL to skip that output. */
+typedef struct hadros_image_out {
+    double* tau;
+    double* p_surv;
+    double* intensity;
+    int32_t* captured;
+    int32_t* npoints;       /* points emitted along the ray                  */
+    int32_t* flags;         /* bit0: ray exhausted max_steps                 */
+    double* alpha;          /* tan-plane u (alpha_rg of the reference)       */
+    double* beta;           /* tan-plane v (beta_rg)                         */
+    double* bin_tau;
+    double* bin_p;
+    double* bin_i;
+} hadros_image_out;
+
+#define HADROS_FLAG_EXHAUSTED 1
+
+typedef struct hadros_stats {
+    double kernel_ms;       /* device time of the compute kernels (HIP events) */
+    double total_ms;        /* wall time of the call                          */
+    int64_t total_points;   /* accepted steps over all rays                   */
+    int64_t total_attempts; /* RKF45 attempts over all rays                   */
+    int32_t n_captured;
+    int32_t n_exhausted;
+    int32_t n_bins;         /* bins actually integrated                       */
+    int32_t block_size;     /* workgroup size used                            */
+} hadros_stats;
+
+/* Geodesic cache (KGEO content) as structure-of-arrays.
+   Replaces RayPath/PathPoint (include/ray.hpp:6-29) streamed from the KGEO
+   file (apps/compute_kerr_image_from_cache.cpp:660-711).  Ray k owns points
+   [offset[k], offset[k+1]).  Only r, theta, dl, redshift are read by the
+   transfer; x, y, z are not needed. */
+typedef struct hadros_cache_view {
+    int32_t n_rays;
+    int32_t reserved;
+    double a_bh;            /* spin stored in the cache header               */
+    const int64_t* offset;  /* [n_rays+1]                                    */
+    const double* r;        /* [n_points]                                    */
+    const double* theta;
+    const double* dl;
+    const double* redshift;
+} hadros_cache_view;
+
+/* ---- library / errors --------------------------------------------------- */
+const char* hadros_last_error(void);
+int hadros_abi_version(void);
+/* Name and compute-unit count of the current device. */
+int hadros_device_info(char* name, int name_cap, int* cu_count, int* gcn_arch);
+
+/* ---- defaults and name parsing ------------------------------------------ */
+/* argv defaults of apps/compute_kerr_geodesics.cpp:26-34 (ref mode, tol 1e-6
+   as KerrCamera passes no tolerance: include/kerr_geodesic.hpp:12). */
+void hadros_camera_defaults(hadros_camera* cam);
+/* argv defaults of apps/compute_kerr_image_from_cache.cpp:33-55. */
+void hadros_scene_defaults(hadros_scene* scene);
+/* TorusProfile::parse_profile_kind (src/torus_profile.cpp:44-91). */
+int hadros_parse_profile(const char* name, int32_t* kind);
+/* parse_uhe_source_model (src/radiative_transfer.cpp:80-112). */
+int hadros_parse_source(const char* name, int32_t* kind);
+/* parse_uhe_spectral_model (src/radiative_transfer.cpp:132-156). */
+int hadros_parse_spectrum(const char* name, int32_t* kind);
+const char* hadros_profile_name(int32_t kind);
+const char* hadros_source_name(int32_t kind);
+const char* hadros_spectrum_name(int32_t kind);
+
+/* ---- sigma table -------------------------------------------------------- */
+/* SigmaTable::SigmaTable (src/sigma_table.cpp:9-53): parse a 3-column
+   "E_GeV sigma_GeV^-2 sigma_cm2" file, skip '#', blank, malformed and
+   non-positive rows; require >= 2 rows and strictly increasing E.
+   Writes up to cap rows; *n receives the row count. */
+int hadros_sigma_load(const char* path, double* e_gev, double* sigma_gev2,
+                      double* sigma_cm2, int cap, int* n);
+/* Validation part of the same constructor for in-memory tables. */
+int hadros_sigma_check(const double* e_gev, const double* sigma_cm2, int n);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* Fused trace + tau + per-bin transfer for a whole camera, one ray per lane:
+   KerrCamera::trace_pixel (src/kerr_camera.cpp:257-351) followed by
+   radiative_transfer::integrate_kerr_ray_spectral
+   (src/radiative_transfer.cpp:687-780) without materialising the path --
+   the loop of apps/compute_kerr_geodesics.cpp:106-140 +
+   apps/compute_kerr_image_from_cache.cpp:605-658 in one launch, and the
+   direct mode of apps/compute_kerr_image_cuda.cu:2239-2606. */
+int hadros_trace_rt(const hadros_camera* cam, const hadros_scene* scene,
+                    const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                    int precision, hadros_image_out* out, hadros_stats* stats);
+
+/* Re-usable plan of hadros_trace_rt: device buffers are allocated once,
+   run() only launches (asynchronously on `stream`, a hipStream_t or NULL),
+   fetch() synchronises and copies results to host. */
+typedef struct hadros_tracer hadros_tracer;
+int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene,
+                         const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                         int precision, hadros_tracer** out);
+/* block_size 0 = autotuned default; must be a multiple of 64, <= 1024. */
+int hadros_tracer_set_block(hadros_tracer* t, int block_size);
+int hadros_tracer_run(hadros_tracer* t, void* stream);
+/* Device time (ms) of the last run's kernel, measured with HIP events on the
+   launch stream; waits for the run to finish. */
+int hadros_tracer_last_ms(hadros_tracer* t, double* ms);
+int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* out, hadros_stats* stats);
+int hadros_tracer_destroy(hadros_tracer* t);
+
+/* Cache replay: radiative_transfer::integrate_kerr_ray_spectral over every ray
+   of a KGEO cache (apps/compute_kerr_image_from_cache.cpp:605-658;
+   apps/compute_kerr_image_cuda.cu:1293-1335).  Host arrays in, host results
+   out; rays with zero points give tau=0, P=1, I=0 (:611-618). */
+int hadros_rt_from_cache(const hadros_cache_view* cache, const hadros_scene* scene,
+                         const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                         int precision, hadros_image_out* out, hadros_stats* stats);
+
+/* Re-usable plan of the replay with the cache resident in HBM. */
+typedef struct hadros_replay hadros_replay;
+int hadros_replay_create(const hadros_cache_view* cache, const hadros_scene* scene,
+                         const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                         int precision, hadros_replay** out);
+int hadros_replay_run(hadros_replay* p, void* stream);
+int hadros_replay_last_ms(hadros_replay* p, double* ms);
+int hadros_replay_fetch(hadros_replay* p, hadros_image_out* out, hadros_stats* stats);
+int hadros_replay_destroy(hadros_replay* p);
+
+/* Geodesic cache on the GPU: the --geodesic-cache mode of
+   apps/compute_kerr_image_cuda.cu:1478-1535,1624-1826 and the per-pixel
+   output of apps/compute_kerr_geodesics.cpp:106-140.  Traces every pixel and
+   writes its PathPoints {r, theta, x, y, z, dl, redshift} (7 doubles each,
+   include/ray.hpp:6-14) into `points` at row offset[k], where offset is an
+   exclusive prefix sum of the per-ray point counts computed on the device.
+   Call first with points == NULL to obtain counts/offsets only (offset has
+   nx*ny+1 entries; offset[nx*ny] is the total), then with a buffer of
+   offset[nx*ny]*7 doubles.  captured[k] as in BinaryRayHeader. */
+int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captured,
+                       double* points, int64_t points_cap);
+
+/* ---- point-wise device evaluations (parity hooks for each physics term) -- */
+/* TorusProfile::rho (src/torus_profile.cpp:261-264), n points. */
+int hadros_eval_rho(const hadros_medium* m, const double* r, const double* th,
+                    double* rho, int64_t n);
+/* SigmaTable::sigma_cm2 (src/sigma_table.cpp:55-58,80-124); out-of-range
+   energies give NaN (the reference throws HADROS_ERR_RANGE-equivalent). */
+int hadros_eval_sigma(const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                      const double* e, double* out, int64_t n);
+/* radiative_transfer::emissivity_uhe (src/radiative_transfer.cpp:369-395)
+   with the spectral model of `scene`. */
+int hadros_eval_emissivity(const hadros_scene* scene, const double* r, const double* th,
+                           const double* e_local, double* out, int64_t n);
+/* KerrGeodesic::rhs (src/kerr_geodesic.cpp:83-135) for mode REF (finite
+   differences) or the analytic derivative for mode CFG; y and dy are n x 8
+   (t, r, theta, phi, p_t, p_r, p_theta, p_phi). */
+int hadros_eval_rhs(double a_spin, int mode, const double* y, double* dy, int64_t n);
+/* KerrMetric::metric / inverse_metric (src/kerr_metric.cpp:34-116):
+   5 distinct components each (tt, tphi, rr, thth, phph) per point, n x 10. */
+int hadros_eval_metric(double a_spin, const double* r, const double* th,
+                       double* out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HADROS_H */
