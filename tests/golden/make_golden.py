@@ -1,0 +1,299 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of tests/golden/ from the compiled reference.
+
+Run in the development container, where /root/reference exists:
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Everything here comes from the reference's own code, built from its sources
+by oracle/Makefile into oracle/_ref/ (the harness library binds the
+reference's C++ classes; the programs are the reference's own apps).  The
+fixtures are data only: inputs and the reference's outputs.
+
+  known_answers.json   values printed by apps/test_kerr_metric.cpp,
+                       apps/test_sigma.cpp, apps/test_torus.cpp
+  pointwise.npz        metric, rhs, step_adaptive, rho, source/spectral
+                       weights, emissivity and sigma on seeded random inputs
+  kgeo_small.npz       a 4x4 KGEO cache written by the reference's
+                       compute_kerr_geodesics (sampled points + column sums)
+  image_rows.npz       image rows of the reference's
+                       compute_kerr_image_from_cache on that cache for several
+                       (profile, source, spectrum) argv sets, with the header
+  sigma_GBW.npz        the reference's GBW DIS table (data file)
+"""
+import ctypes as C
+import json
+import math
+import os
+import re
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from hadros_amd import abi  # noqa: E402
+from oracle import py_oracle as po  # noqa: E402
+
+REF = "/root/reference"
+REF_BIN = po.REF_DIR
+GBW = os.path.join(REF, "data/sigma/sigma_nuN_CC_GBW.dat")
+
+
+def run(cmd, cwd):
+    return subprocess.run(cmd, cwd=cwd, check=True, capture_output=True, text=True).stdout
+
+
+def known_answers():
+    out = {}
+    txt = run([os.path.join(REF_BIN, "test_kerr_metric")], REF)
+    for key, pat in [("horizon", r"Horizon radius = (\S+)"), ("g_tt", r"g_tt\s+= (\S+)"),
+                     ("g_tphi", r"g_tphi = (\S+)"), ("g_rr", r"g_rr\s+= (\S+)"), ("g_thth", r"g_thth = (\S+)"),
+                     ("g_phph", r"g_phph = (\S+)"), ("gtt", r"\ngtt\s+= (\S+)"), ("gtphi", r"gtphi\s+= (\S+)"),
+                     ("grr", r"\ngrr\s+= (\S+)"), ("gthth", r"\ngthth\s+= (\S+)"), ("gphph", r"\ngphph\s+= (\S+)"),
+                     ("lapse", r"Lapse = (\S+)"), ("omega", r"(?:Omega|omega)[^=]*= (\S+)")]:
+        m = re.search(pat, txt)
+        if m:
+            out["kerr_metric_a0.9_r5_th_pi2." + key] = float(m.group(1))
+    out["kerr_metric_stdout"] = txt
+    txt = run([os.path.join(REF_BIN, "test_sigma")], REF)
+    out["sigma_GBW_1e9_cm2"] = float(re.search(r"sigma\(\S+ GeV\) = (\S+)", txt).group(1))
+    out["sigma_stdout"] = txt
+    txt = run([os.path.join(REF_BIN, "test_torus")], REF)
+    out["torus_default_rho_10_pi2"] = float(re.search(r"rho = (\S+)", txt).group(1))
+    out["torus_stdout"] = txt
+    return out
+
+
+def pointwise(R):
+    rng = np.random.default_rng(20261203)
+    d = {}
+    # metric
+    n = 400
+    a = rng.uniform(-0.2, 0.999, n)
+    r = rng.uniform(1.0, 300.0, n)
+    th = rng.uniform(1e-3, math.pi - 1e-3, n)
+    g = np.zeros((n, 16))
+    gi = np.zeros((n, 16))
+    lo = np.zeros((n, 3))
+    for k in range(n):
+        R.ref_metric(a[k], r[k], th[k], po.dptr(g[k]), po.dptr(gi[k]))
+        R.ref_lapse_omega(a[k], r[k], th[k], po.dptr(lo[k]))
+    d.update(metric_a=a, metric_r=r, metric_th=th, metric_g=g, metric_gi=gi, metric_lapse_omega_horizon=lo)
+    # rhs and one adaptive step
+    n = 300
+    a = rng.uniform(0.0, 0.998, n)
+    y = np.zeros((n, 8))
+    y[:, 0] = rng.uniform(0, 10, n)
+    y[:, 1] = rng.uniform(2.2, 200.0, n)
+    y[:, 2] = rng.uniform(0.05, math.pi - 0.05, n)
+    y[:, 3] = rng.uniform(-3, 3, n)
+    y[:, 4] = -rng.uniform(0.5, 1.5, n)
+    y[:, 5] = rng.normal(size=n)
+    y[:, 6] = 5 * rng.normal(size=n)
+    y[:, 7] = 5 * rng.normal(size=n)
+    dy = np.zeros_like(y)
+    H = np.zeros(n)
+    ys = y.copy()
+    hs = rng.uniform(0.01, 0.5, n)
+    tols = 10 ** rng.uniform(-10, -5, n)
+    for k in range(n):
+        R.ref_rhs(a[k], po.dptr(y[k]), po.dptr(dy[k]))
+        H[k] = R.ref_hamiltonian(a[k], po.dptr(y[k]))
+        R.ref_step_adaptive(a[k], hs[k], tols[k], po.dptr(ys[k]))
+    d.update(rhs_a=a, rhs_y=y, rhs_dy=dy, rhs_H=H, step_h=hs, step_tol=tols, step_y=ys)
+    # rho, emissivity for every profile / source / spectrum
+    n = 500
+    r = rng.uniform(0.5, 90.0, n)
+    th = rng.uniform(0.0, math.pi, n)
+    e = 10 ** rng.uniform(2, 14, n)
+    d.update(pt_r=r, pt_th=th, pt_e=e)
+    for pk, prof in enumerate(abi.PROFILE_NAMES):
+        tp = np.array([10 ** rng.uniform(-3, 11), rng.uniform(5, 30), rng.uniform(1, 6), rng.uniform(0.1, 0.6),
+                       rng.uniform(0.5, 3), rng.uniform(0, 1.2), rng.uniform(0.1, 0.6), 10 ** rng.uniform(-3, 8),
+                       rng.uniform(1, 3), rng.uniform(2, 6), rng.uniform(30, 80), 10 ** rng.uniform(-5, -1)])
+        out = np.zeros(n)
+        assert R.ref_rho(prof.encode(), po.dptr(tp), po.dptr(r), po.dptr(th), po.dptr(out), n) == 0
+        d[f"rho_{prof}_params"] = tp
+        d[f"rho_{prof}"] = out
+        for sk, src in enumerate(abi.SOURCE_NAMES):
+            for spk, spec in enumerate(abi.SPECTRUM_NAMES):
+                if (pk + sk + spk) % 3:
+                    continue   # a third of the 120 combinations keeps the fixture small
+                sp = np.array([rng.uniform(2, 10), rng.uniform(0.5, 3), rng.uniform(0.05, 0.5), rng.uniform(1, 3),
+                               10 ** rng.uniform(10, 13), 10 ** rng.uniform(-2, 2), rng.uniform(0.1, 0.7),
+                               rng.uniform(0.5, 2), rng.uniform(1, 3), -1.0 if rng.uniform() < 0.5 else 1e5,
+                               0.0, 1e2, rng.uniform(0.05, 0.3), rng.uniform(0.01, 0.05)])
+                spp = np.array([rng.uniform(1.5, 2.7), 10 ** rng.uniform(9, 13), 1e5, 1e12, 8.0])
+                j = np.zeros(n)
+                w = np.zeros(n)
+                s = np.zeros(n)
+                rc = R.ref_emissivity(prof.encode(), po.dptr(tp), src.encode(), po.dptr(sp), spec.encode(),
+                                      po.dptr(spp), po.dptr(r), po.dptr(th), po.dptr(e), po.dptr(j), po.dptr(w),
+                                      po.dptr(s), n)
+                assert rc == 0
+                key = f"emis_{prof}_{src}_{spec}"
+                d[key + "_src"] = sp
+                d[key + "_spec"] = spp
+                d[key + "_j"] = j
+                d[key + "_w"] = w
+                d[key + "_s"] = s
+    # sigma on the GBW table (energies inside and outside the range)
+    es = 10 ** rng.uniform(2.5, 14.5, 2000)
+    out = np.zeros(es.size)
+    assert R.ref_sigma(GBW.encode(), po.dptr(es), po.dptr(out), es.size) == 0
+    d.update(sigma_e=es, sigma_gbw=out)
+    return d
+
+
+def read_kgeo(path):
+    with open(path, "rb") as f:
+        buf = f.read()
+    magic, ver, nx, ny, a = struct.unpack_from("<iiiid", buf, 0)
+    pos = 24
+    rays = []
+    while pos < len(buf):
+        rid, i, j, cap, npts, al, be = struct.unpack_from("<5i4xdd", buf, pos)
+        pos += 40
+        pts = np.frombuffer(buf, np.float64, npts * 7, pos).reshape(npts, 7)
+        pos += npts * 56
+        rays.append((rid, i, j, cap, npts, al, be, pts))
+    rays.sort(key=lambda t: t[0])
+    return (magic, ver, nx, ny, a), rays
+
+
+CACHE_ARGS = ["0.9", "60", "80", "25", "4", "4", "120", "0.1"]
+
+
+def kgeo_fixture(tmp):
+    path = os.path.join(tmp, "kgeo.bin")
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([os.path.join(REF_BIN, "compute_kerr_geodesics")] + CACHE_ARGS + [path], cwd=tmp, check=True,
+                   capture_output=True, env=env)
+    hdr, rays = read_kgeo(path)
+    d = {"header": np.array(hdr[:4], np.int64), "a": np.array([hdr[4]]), "argv": np.array(CACHE_ARGS)}
+    meta = np.array([[r[0], r[1], r[2], r[3], r[4]] for r in rays], np.int64)
+    d["rays"] = meta
+    d["alpha_beta"] = np.array([[r[5], r[6]] for r in rays])
+    samples = []
+    sums = []
+    for r in rays:
+        pts = r[7]
+        idx = sorted(set(list(range(0, pts.shape[0], 97)) + list(range(max(0, pts.shape[0] - 5), pts.shape[0]))))
+        samples.append(np.concatenate([np.array(idx, float)[:, None], pts[idx]], 1))
+        sums.append([math.fsum(pts[:, c]) for c in range(7)])
+    d["sample_offsets"] = np.cumsum([0] + [s.shape[0] for s in samples])
+    d["samples"] = np.concatenate(samples)
+    d["column_fsums"] = np.array(sums)
+    return d, path, rays
+
+
+IMAGE_CASES = [
+    # ENU ASPIN MBH RHO0 R0 SIGMA HOR SRC_R SRC_SIG SRC_TH SRC_PL SRC_EMAX SRC_NORM MEV_ENU MEV_NORM CAMTH
+    # SIGMA PROFILE RPOW FDEPL FTHETA ENVRHO ENVALPHA RMIN RMAX FLOOR ROBS USEF3 CACHE SOURCE... SPECTRAL...
+    dict(name="gaussian_ring_mono", profile="gaussian", source="inner_ring", spectrum="monochromatic", rho0="1e-2"),
+    dict(name="pfe_funnel_cutoff", profile="powerlaw_funnel_envelope", source="funnel_wall",
+         spectrum="powerlaw_cutoff", rho0="1e1"),
+    dict(name="collapsar_jet_powerlaw", profile="collapsar_ndaf_like", source="jet_base", spectrum="powerlaw",
+         rho0="1e3"),
+    dict(name="genv_density_cutoff", profile="gaussian_envelope", source="density_weighted",
+         spectrum="powerlaw_cutoff", rho0="1e0"),
+    dict(name="pfunnel_shock_mono", profile="powerlaw_funnel", source="shock_layer", spectrum="monochromatic",
+         rho0="1e2"),
+]
+
+
+def image_argv(c, cache):
+    return ["1e9", "0.9", "3.0", c["rho0"], "10.0", "5.0", "0.25", "6.0", "1.5", "15.0", "2.0", "1e12", "1.0",
+            "10.0", "1.0", "80.0", "SIGMA", c["profile"], "2.0", "0.9", "20.0", "1e-1", "2.5", "4.0", "60.0",
+            "1e-3", "60.0", "1", cache, c["source"], "20.0", "1.0", "2.0", "0.1", "1.0", "-1.0", "0.0", "100.0",
+            c["spectrum"], "2.0", "1e12", "1e5", "1e12", "8"]
+
+
+def image_fixture(tmp, cache_path, sigma_path):
+    os.makedirs(os.path.join(tmp, "output", "images"), exist_ok=True)
+    d = {}
+    for c in IMAGE_CASES:
+        argv = [sigma_path if x == "SIGMA" else x for x in image_argv(c, cache_path)]
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        out = subprocess.run([os.path.join(REF_BIN, "compute_kerr_image_from_cache")] + argv, cwd=tmp, check=True,
+                             capture_output=True, text=True, env=env).stdout
+        saved = re.search(r"Saved: (\S+)", out).group(1)
+        with open(os.path.join(tmp, saved)) as f:
+            lines = f.read().splitlines()
+        header = [ln for ln in lines if ln.startswith("#")]
+        rows = np.array([[float(x) for x in ln.split()] for ln in lines if not ln.startswith("#")])
+        d[c["name"] + "_argv"] = np.array(["SIGMA" if x == sigma_path else "CACHE" if x == cache_path else x
+                                           for x in argv])
+        d[c["name"] + "_rows"] = rows
+        d[c["name"] + "_header"] = np.array(header)
+        d[c["name"] + "_filename"] = np.array([saved])
+        d[c["name"] + "_stdout"] = np.array([out])
+    return d
+
+
+def integrate_fixture(R, rays, sigma_path):
+    """integrate_kerr_ray_spectral on the reference's own paths, exact doubles."""
+    d = {}
+    for c in IMAGE_CASES:
+        argv = image_argv(c, "x")
+        sc = abi.scene(enu=1e9, m_bh=3.0, rho0=float(c["rho0"]), r0=10.0, sigma_r=5.0, h_over_r=0.25,
+                       profile=c["profile"], radial_power=2.0, funnel_depletion=0.9, funnel_theta_deg=20.0,
+                       envelope_rho0=0.1, envelope_alpha=2.5, r_min=4.0, r_max=60.0, rho_floor=1e-3,
+                       source=c["source"], source_r=6.0, source_sigma=1.5, source_theta_deg=15.0,
+                       source_powerlaw=2.0, source_emax=1e12, source_norm=1.0, source_funnel_theta_deg=20.0,
+                       source_q=1.0, source_s=2.0, source_gradient_dr=0.1, source_gradient_dtheta_deg=1.0,
+                       source_rho_ref=-1.0, source_cutoff_min=0.0, source_cutoff_max=100.0, spectrum=c["spectrum"],
+                       gamma=2.0, ecut=1e12, e_min=1e5, e_max=1e12, n_bins=8)
+        del argv
+        tp = np.array(abi.medium_params(sc.medium))
+        sp = np.array(abi.source_params(sc.source))
+        spp = np.array(abi.spectrum_params(sc.spectrum))
+        res = np.zeros((len(rays), 8))
+        for k, r in enumerate(rays):
+            pts = np.ascontiguousarray(r[7])
+            rc = R.ref_integrate(sigma_path.encode(), abi.PROFILE_NAMES[sc.medium.kind].encode(), po.dptr(tp),
+                                 c["source"].encode(), po.dptr(sp), c["spectrum"].encode(), po.dptr(spp), 1e9, 3.0,
+                                 0.9, po.dptr(pts), pts.shape[0], po.dptr(res[k]))
+            assert rc == 0
+        d[c["name"] + "_rt"] = res
+    return d
+
+
+def main():
+    if not po.have_ref():
+        sys.exit("oracle/_ref not built: run `make -C oracle ref` (needs /root/reference)")
+    R = po.ref()
+    ka = known_answers()
+    with open(os.path.join(HERE, "known_answers.json"), "w") as f:
+        json.dump(ka, f, indent=1, sort_keys=True)
+    np.savez_compressed(os.path.join(HERE, "pointwise.npz"), **pointwise(R))
+    e, g, s = [], [], []
+    with open(GBW) as f:
+        for ln in f:
+            if ln.startswith("#") or not ln.strip():
+                continue
+            a, b, c = map(float, ln.split()[:3])
+            e.append(a)
+            g.append(b)
+            s.append(c)
+    np.savez_compressed(os.path.join(HERE, "sigma_GBW.npz"), e=np.array(e), g=np.array(g), s=np.array(s))
+    with tempfile.TemporaryDirectory() as tmp:
+        kd, cache_path, rays = kgeo_fixture(tmp)
+        np.savez_compressed(os.path.join(HERE, "kgeo_small.npz"), **kd)
+        sig_path = os.path.join(tmp, "sigma_synthetic.dat")
+        from hadros_amd import workloads
+        workloads.write_sigma_file(sig_path, *workloads.synthetic_sigma())
+        imd = image_fixture(tmp, cache_path, sig_path)
+        imd.update(integrate_fixture(R, rays, sig_path))
+        np.savez_compressed(os.path.join(HERE, "image_rows.npz"), **imd)
+    for fn in sorted(os.listdir(HERE)):
+        print(fn, os.path.getsize(os.path.join(HERE, fn)))
+
+
+if __name__ == "__main__":
+    main()

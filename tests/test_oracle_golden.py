@@ -1,0 +1,368 @@
+"""CPU: pin the FP64 oracle against the reference's golden fixtures (bit-exact)
+and against what the mathematics fixes (closed forms, invariants, special cases).
+
+Fixtures: tests/golden/*.npz|json, written by tests/golden/make_golden.py from
+the reference's own sources compiled in this container.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from hadros_amd import abi, workloads
+from oracle import py_oracle as po
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def pw():
+    return np.load(os.path.join(GOLD, "pointwise.npz"))
+
+
+@pytest.fixture(scope="module")
+def gbw():
+    d = np.load(os.path.join(GOLD, "sigma_GBW.npz"))
+    return d["e"], d["s"]
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    same = (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b)) | ((a == 0) & (b == 0))
+    return bool(same.all())
+
+
+# ------------------------------------------------------------------ known answers
+def test_known_answers_of_reference_test_programs(gbw):
+    ka = json.load(open(os.path.join(GOLD, "known_answers.json")))
+    L = po.lib()
+    g = np.zeros(5)
+    gi = np.zeros(5)
+    L.or_metric(0.9, 5.0, math.pi / 2.0, po.dptr(g))
+    L.or_inverse_metric(0.9, 5.0, math.pi / 2.0, po.dptr(gi))
+    k = "kerr_metric_a0.9_r5_th_pi2."
+    pairs = [("g_tt", g[0]), ("g_tphi", g[1]), ("g_rr", g[2]), ("g_thth", g[3]), ("g_phph", g[4]),
+             ("gtt", gi[0]), ("gtphi", gi[1]), ("grr", gi[2]), ("gthth", gi[3]), ("gphph", gi[4]),
+             ("horizon", L.or_horizon(0.9)), ("lapse", L.or_lapse(0.9, 5.0, math.pi / 2)),
+             ("omega", L.or_omega(0.9, 5.0, math.pi / 2))]
+    for name, val in pairs:
+        assert val == pytest.approx(ka[k + name], rel=1e-8, abs=1e-12), name
+    e, s = gbw
+    assert po.lib().or_sigma(po.dptr(e), po.dptr(s), e.size, 1.0e9) == pytest.approx(ka["sigma_GBW_1e9_cm2"],
+                                                                                    rel=1e-5)
+    m = abi.Medium(kind=0, rho0=1e11, r0=10.0, sigma_r=5.0, h_over_r=0.3, radial_power=2.0, funnel_depletion=0.0,
+                   funnel_theta_rad=0.25, envelope_rho0=0.0, envelope_alpha=2.5, r_min=4.0, r_max=60.0,
+                   rho_floor=1e-99)
+    assert po.lib().or_rho(m, 10.0, math.pi / 2) == ka["torus_default_rho_10_pi2"]
+
+
+# ------------------------------------------------------------------ point-wise, bit-exact
+def test_metric_bitexact(pw):
+    L = po.lib()
+    a, r, th = pw["metric_a"], pw["metric_r"], pw["metric_th"]
+    g16, gi16, lo = pw["metric_g"], pw["metric_gi"], pw["metric_lapse_omega_horizon"]
+    idx = [0, 3, 5, 10, 15]
+    for k in range(a.size):
+        g = np.zeros(5)
+        gi = np.zeros(5)
+        L.or_metric(a[k], r[k], th[k], po.dptr(g))
+        L.or_inverse_metric(a[k], r[k], th[k], po.dptr(gi))
+        assert bits_equal(g, g16[k, idx]) and bits_equal(gi, gi16[k, idx])
+        assert bits_equal([L.or_lapse(a[k], r[k], th[k]), L.or_omega(a[k], r[k], th[k]), L.or_horizon(a[k])], lo[k])
+
+
+def test_rhs_hamiltonian_step_bitexact(pw):
+    L = po.lib()
+    a, y, dy, H = pw["rhs_a"], pw["rhs_y"], pw["rhs_dy"], pw["rhs_H"]
+    for k in range(a.size):
+        d = np.zeros(8)
+        yy = np.ascontiguousarray(y[k])
+        L.or_rhs(a[k], abi.MODE_REF, po.dptr(yy), po.dptr(d))
+        assert bits_equal(d[1:], dy[k, 1:]), k     # t is not integrated (it feeds nothing)
+        assert bits_equal([L.or_hamiltonian(a[k], po.dptr(yy))], [H[k]])
+        ys = yy.copy()
+        L.or_step_ref(a[k], pw["step_h"][k], pw["step_tol"][k], po.dptr(ys))
+        assert bits_equal(ys[1:], pw["step_y"][k, 1:]), k
+
+
+def test_rho_and_emissivity_bitexact(pw):
+    L = po.lib()
+    r, th, e = pw["pt_r"], pw["pt_th"], pw["pt_e"]
+    n_emis = 0
+    for pk, prof in enumerate(abi.PROFILE_NAMES):
+        tp = pw[f"rho_{prof}_params"]
+        sc = abi.Scene()
+        m = sc.medium
+        m.kind = pk
+        (m.rho0, m.r0, m.sigma_r, m.h_over_r, m.radial_power, m.funnel_depletion, m.funnel_theta_rad,
+         m.envelope_rho0, m.envelope_alpha, m.r_min, m.r_max, m.rho_floor) = tp
+        got = np.array([L.or_rho(m, r[q], th[q]) for q in range(r.size)])
+        assert bits_equal(got, pw[f"rho_{prof}"]), prof
+        for sk, src in enumerate(abi.SOURCE_NAMES):
+            for spk, spec in enumerate(abi.SPECTRUM_NAMES):
+                key = f"emis_{prof}_{src}_{spec}"
+                if key + "_j" not in pw.files:
+                    continue
+                n_emis += 1
+                so = sc.source
+                so.kind = sk
+                (so.r_center, so.sigma_r, so.theta_width_rad, so.powerlaw, so.emax_GeV, so.norm,
+                 so.funnel_theta_rad, so.density_q, so.radial_s, so.rho_ref, so.cutoff_min, so.cutoff_max,
+                 so.gradient_dr, so.gradient_dtheta_rad) = pw[key + "_src"]
+                sp = sc.spectrum
+                sp.kind = spk
+                sp.gamma, sp.ecut_GeV, sp.e_min_GeV, sp.e_max_GeV, nb = pw[key + "_spec"]
+                sp.n_bins = int(nb)
+                j = np.array([L.or_emissivity(sc, r[q], th[q], e[q]) for q in range(r.size)])
+                w = np.array([L.or_source_weight(sc, r[q], th[q]) for q in range(r.size)])
+                s = np.array([L.or_spectral_weight(sc, e[q]) for q in range(r.size)])
+                assert bits_equal(j, pw[key + "_j"]), key
+                assert bits_equal(s, pw[key + "_s"]), key
+                if sk != 0:   # inner_ring's spatial weight is reached only through emissivity
+                    assert bits_equal(w, pw[key + "_w"]), key
+    assert n_emis == 40
+
+
+def test_sigma_bitexact_and_range(pw, gbw):
+    e, s = gbw
+    got = np.array([po.lib().or_sigma(po.dptr(e), po.dptr(s), e.size, x) for x in pw["sigma_e"]])
+    ref = pw["sigma_gbw"]
+    assert bits_equal(got, ref)           # NaN where the reference throws (out of range)
+    assert np.isnan(ref).any() and np.isfinite(ref).any()
+
+
+# ------------------------------------------------------------------ traced paths
+def test_kgeo_paths_bitexact():
+    d = np.load(os.path.join(GOLD, "kgeo_small.npz"))
+    argv = [float(x) for x in d["argv"]]
+    a, r_obs, th_deg, fov, nx, ny, r_max, h = argv
+    cam = abi.camera(a_spin=a, r_obs=r_obs, theta_obs_deg=th_deg, fov_deg=fov, nx=int(nx), ny=int(ny), r_max=r_max,
+                     h=h, mode=abi.MODE_REF)
+    assert list(d["header"]) == [0x4B47454F, 1, int(nx), int(ny)]
+    off = d["sample_offsets"]
+    for k, (rid, i, j, cap, npts) in enumerate(d["rays"]):
+        assert rid == i * int(ny) + j
+        pts, cp, ex, at = po.trace_pixel(cam, int(i), int(j))
+        assert pts.shape[0] == npts and cp == cap
+        smp = d["samples"][off[k]:off[k + 1]]
+        idx = smp[:, 0].astype(int)
+        assert bits_equal(pts[idx], smp[:, 1:])
+        assert bits_equal([math.fsum(pts[:, c]) for c in range(7)], d["column_fsums"][k])
+
+
+def test_integrate_and_image_rows_bitexact():
+    """integrate_kerr_ray_spectral on the reference's paths, and the UHE columns
+    of the reference CLI's image rows (%.8e) reproduced by the oracle."""
+    d = np.load(os.path.join(GOLD, "kgeo_small.npz"))
+    im = np.load(os.path.join(GOLD, "image_rows.npz"))
+    a, r_obs, th_deg, fov, nx, ny, r_max, h = [float(x) for x in d["argv"]]
+    cam = abi.camera(a_spin=a, r_obs=r_obs, theta_obs_deg=th_deg, fov_deg=fov, nx=int(nx), ny=int(ny), r_max=r_max,
+                     h=h, mode=abi.MODE_REF)
+    e, _, s = workloads.synthetic_sigma()
+    names = sorted({k[:-3] for k in im.files if k.endswith("_rt")})
+    assert len(names) == 5
+    paths = [po.trace_pixel(cam, int(i), int(j))[0] for (_, i, j, _, _) in d["rays"]]
+    for name in names:
+        argv = list(im[name + "_argv"])
+        sc = scene_from_argv(argv)
+        rt = im[name + "_rt"]
+        rows = im[name + "_rows"]
+        by_pix = {(int(rw[0]), int(rw[1])): rw for rw in rows}
+        for k, (_, i, j, cap, npts) in enumerate(d["rays"]):
+            agg, bt, bp, bi = po.integrate_ray(sc, e, s, a, paths[k])
+            assert bits_equal(agg, rt[k, :3]), (name, k)
+            rw = by_pix[(int(i), int(j))]
+            for c, v in ((4, agg[0]), (5, agg[1]), (6, agg[2])):
+                assert float("%.8e" % v) == rw[c], (name, k, c)
+
+
+def scene_from_argv(argv):
+    """Scene of a compute_kerr_image_from_cache argv (positions of
+    apps/compute_kerr_image_from_cache.cpp:57-205)."""
+    A = ["prog"] + argv
+    f = lambda i: float(A[i])  # noqa: E731
+    return abi.scene(enu=f(1), m_bh=f(3), rho0=f(4), r0=f(5), sigma_r=f(6), h_over_r=f(7), source_r=f(8),
+                     source_sigma=f(9), source_theta_deg=f(10), source_powerlaw=f(11), source_emax=f(12),
+                     source_norm=f(13), profile=A[18], radial_power=f(19), funnel_depletion=f(20),
+                     funnel_theta_deg=f(21), envelope_rho0=f(22), envelope_alpha=f(23), r_min=f(24), r_max=f(25),
+                     rho_floor=f(26), source=A[30], source_funnel_theta_deg=f(31), source_q=f(32), source_s=f(33),
+                     source_gradient_dr=f(34), source_gradient_dtheta_deg=f(35), source_rho_ref=f(36),
+                     source_cutoff_min=f(37), source_cutoff_max=f(38), spectrum=A[39], gamma=f(40), ecut=f(41),
+                     e_min=f(42), e_max=f(43), n_bins=int(A[44]))
+
+
+# ------------------------------------------------------------------ mathematics
+def test_metric_inverse_identity():
+    L = po.lib()
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        a, r, th = rng.uniform(0, 0.999), rng.uniform(2.5, 500), rng.uniform(0.05, 3.09)
+        g = np.zeros(5)
+        gi = np.zeros(5)
+        L.or_metric(a, r, th, po.dptr(g))
+        L.or_inverse_metric(a, r, th, po.dptr(gi))
+        G = np.array([[g[0], 0, 0, g[1]], [0, g[2], 0, 0], [0, 0, g[3], 0], [g[1], 0, 0, g[4]]])
+        Gi = np.array([[gi[0], 0, 0, gi[1]], [0, gi[2], 0, 0], [0, 0, gi[3], 0], [gi[1], 0, 0, gi[4]]])
+        assert np.allclose(G @ Gi, np.eye(4), atol=1e-9)
+
+
+def test_cfg_rhs_is_hamiltonian_gradient():
+    """dp_r = -dH/dr, dp_th = -dH/dth, dx/dl = dH/dp: central differences of the
+    reference's Hamiltonian (src/kerr_geodesic.cpp:17-45)."""
+    L = po.lib()
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        a = rng.uniform(0, 0.998)
+        y = np.array([0, rng.uniform(3, 100), rng.uniform(0.2, 2.9), 0.3, -1.0, rng.normal(), 3 * rng.normal(),
+                      3 * rng.normal()])
+        d = np.zeros(8)
+        L.or_rhs(a, abi.MODE_CFG, po.dptr(y), po.dptr(d))
+
+        def H(yy):
+            yy = np.ascontiguousarray(yy)
+            return L.or_hamiltonian(a, po.dptr(yy))
+
+        def grad(c, eps):
+            yp, ym = y.copy(), y.copy()
+            yp[c] += eps
+            ym[c] -= eps
+            return (H(yp) - H(ym)) / (2 * eps)
+
+        for c_dot, c_der, sign in ((1, 5, 1), (2, 6, 1), (3, 7, 1), (5, 1, -1), (6, 2, -1)):
+            num = sign * grad(c_der, 1e-6 * max(1.0, abs(y[c_der])))
+            assert d[c_dot] == pytest.approx(num, rel=1e-6, abs=1e-9 * max(1, abs(num))), (c_dot, a, y)
+
+
+def carter_q(a, y):
+    c = math.cos(y[2])
+    s = math.sin(y[2])
+    return y[6] ** 2 + c * c * (y[7] ** 2 / (s * s) - a * a * y[4] ** 2)
+
+
+def test_camera_null_and_path_bounds():
+    """H = 0 at the camera; CFG paths stay outside 1.01 r_+, end beyond r_max
+    unless captured, and have dl >= 0."""
+    L = po.lib()
+    cam = workloads.c1_camera(nx=7, ny=7)
+    for (i, j) in [(0, 0), (3, 3), (6, 2), (1, 5)]:
+        y0 = np.zeros(8)
+        L.or_initial_state(cam, i, j, po.dptr(y0))
+        p2 = sum(v * v for v in y0[4:])
+        assert abs(L.or_hamiltonian(cam.a_spin, po.dptr(y0))) < 1e-12 * p2
+        pts, cp, ex, at = po.trace_pixel(cam, i, j)
+        assert ex == 0 and pts.shape[0] > 20
+        # the path stays outside the horizon until capture and reaches r_max otherwise
+        r_h = L.or_horizon(cam.a_spin)
+        assert pts[:, 0].min() >= 1.01 * r_h
+        if not cp:
+            assert pts[-1, 0] > 900
+        assert np.all(pts[:, 5] >= 0)
+
+
+def test_analytic_rhs_conserves_constants():
+    """Integrating the analytic right-hand side with an independent fixed-step
+    RK4 keeps H ~ 0, the Carter constant and E = -p_t."""
+    L = po.lib()
+    a = 0.9
+    cam = workloads.c1_camera(nx=5, ny=5)
+    y = np.zeros(8)
+    L.or_initial_state(cam, 2, 1, po.dptr(y))
+    q0 = carter_q(a, y)
+    e0 = -y[4]
+    # explicit RK4 with small steps using the analytic rhs (independent of the stepper)
+    h = 0.5
+    def f(v):
+        v = np.ascontiguousarray(v)
+        d = np.zeros(8)
+        L.or_rhs(a, abi.MODE_CFG, po.dptr(v), po.dptr(d))
+        return d
+
+    for _ in range(3000):
+        if y[1] < 2.5 or y[1] > 1100:
+            break
+        hh = h * min(1.0, y[1] / 20.0)
+        k1 = f(y)
+        k2 = f(y + 0.5 * hh * k1)
+        k3 = f(y + 0.5 * hh * k2)
+        k4 = f(y + hh * k3)
+        y = y + hh * (k1 + 2 * k2 + 2 * k3 + k4) / 6
+    p2 = sum(v * v for v in y[4:])
+    assert abs(L.or_hamiltonian(a, po.dptr(np.ascontiguousarray(y)))) < 1e-6 * p2
+    assert carter_q(a, y) == pytest.approx(q0, rel=1e-5, abs=1e-6)
+    assert -y[4] == e0
+
+
+def test_schwarzschild_shadow_edge():
+    """a = 0: a photon with impact parameter b is captured iff b < sqrt(27) M."""
+    L = po.lib()
+    r_obs = 2000.0
+    nx = 41
+    cam = abi.camera(a_spin=0.0, r_obs=r_obs, theta_obs_deg=90.0,
+                     fov_deg=workloads.fov_for_half_width(8.0, r_obs), nx=nx, ny=1, r_max=2200.0, h=1.0,
+                     mode=abi.MODE_CFG)
+    bc = math.sqrt(27.0)
+    n_checked = 0
+    for i in range(nx):
+        y = np.zeros(8)
+        L.or_initial_state(cam, i, 0, po.dptr(y))
+        b = abs(y[7] / y[4])
+        if abs(b - bc) < 0.02:
+            continue
+        pts, cp, ex, at = po.trace_pixel(cam, i, 0)
+        assert cp == (1 if b < bc else 0), (i, b)
+        n_checked += 1
+    assert n_checked > 30
+
+
+def test_powerlaw_sigma_table_is_exact():
+    e, _, s = workloads.synthetic_sigma()
+    L = po.lib()
+    rng = np.random.default_rng(5)
+    for x in 10 ** rng.uniform(4, 12, 2000):
+        assert L.or_sigma(po.dptr(e), po.dptr(s), e.size, x) == pytest.approx(7.84e-36 * x ** 0.363, rel=1e-13)
+
+
+def test_transfer_linearity_and_single_bin():
+    """I is linear in source.norm and tau in rho0 (floor off): doubling is exact
+    in binary floating point; a one-bin spectrum's tau equals the monochromatic
+    tau at that bin's energy."""
+    cam = workloads.c1_camera(nx=6, ny=6)
+    e, _, s = workloads.synthetic_sigma()
+    sc = workloads.c1_scene()
+    sc.medium.rho0 = 10.0
+    sc.medium.rho_floor = 0.0
+    base, _ = po.trace_image(cam, sc, e, s)
+    sc.source.norm = 2.0
+    sc.medium.rho0 = 20.0
+    dbl, _ = po.trace_image(cam, sc, e, s)
+    assert bits_equal(dbl.tau, 2 * base.tau)
+    # emission sees e^{-tau} with the doubled tau, so only where tau ~ 0 is I exactly doubled
+    thin = base.tau == 0
+    assert bits_equal(dbl.I[thin], 2 * base.I[thin])
+    one = workloads.c1_scene()
+    one.medium.rho0 = 10.0
+    one.spectrum.kind = abi.SPECTRUM_KINDS["powerlaw"]
+    one.spectrum.n_bins = 1
+    one.spectrum.e_min_GeV, one.spectrum.e_max_GeV = 1e6, 1e10
+    emid, de, w = po.bins(one, e)
+    assert emid.size == 1
+    a1, _ = po.trace_image(cam, one, e, s)
+    mono = workloads.c1_scene()
+    mono.medium.rho0 = 10.0
+    mono.enu_obs_GeV = emid[0]
+    a2, _ = po.trace_image(cam, mono, e, s)
+    assert bits_equal(a1.bin_tau[:, 0], a2.tau)
+    assert np.allclose(a1.tau, a2.tau, rtol=1e-15, atol=0)
+
+
+def test_empty_spectral_range_gives_defaults():
+    cam = workloads.c1_camera(nx=3, ny=3)
+    e, _, s = workloads.synthetic_sigma()
+    sc = workloads.c2_scene()
+    sc.spectrum.e_min_GeV, sc.spectrum.e_max_GeV = 1e13, 1e14   # above the table
+    img, st = po.trace_image(cam, sc, e, s)
+    assert st.n_bins == -1 or st.n_bins == 0
+    assert np.all(img.tau == 0) and np.all(img.p == 1) and np.all(img.I == 0)
