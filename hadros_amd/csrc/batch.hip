@@ -1,0 +1,350 @@
+// batch.hip -- persistent multi-camera tracer with a global ray queue.
+//
+// Many cameras of uneven resolution, spin, inclination, medium, source and
+// spectrum in one launch.  Ray lengths differ by orders of magnitude
+// (captured vs escaping vs photon-ring rays), so a fixed grid leaves most
+// lanes idle behind the longest ray of their wavefront.  Here a resident grid
+// of wavefronts pulls rays from one global counter: every `steps_per_chunk`
+// RKF45 attempts each wavefront takes a ballot of its finished lanes, claims
+// that many ray indices with one atomic add, and hands them out in lane order
+// (mbcnt), so the lanes stay busy until the queue is empty.
+//
+// Per ray the arithmetic is the plain tracer's (same device functions, same
+// operation order), so every image equals the per-camera plain run bit for
+// bit.  Camera c owns global ray indices [pix0[c], pix0[c+1]); pixel
+// k = i*ny + j within it, as in the plain tracer.
+#include "kernels.h"
+
+#ifndef HD_BATCH_WAVES_PER_EU
+#define HD_BATCH_WAVES_PER_EU 2
+#endif
+
+namespace hd {
+
+struct BatchLane {
+    long ray;        // global ray index, -1 = idle
+    int cam;
+    int step, npts, captured, exhausted, att_in_step;
+    long attempts;
+    double pt, pph, e_obs_c, h, r_stop;
+    Y5 y, yp;
+};
+
+template <int MODE>
+__device__ __forceinline__ void batch_init_ray(const BatchArgs& A, const long* __restrict__ pix0, BatchLane& L,
+                                               long ray)
+{
+    // camera of this ray: last c with pix0[c] <= ray
+    int lo = 0, hi = A.n_cams - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pix0[mid] <= ray) lo = mid;
+        else hi = mid - 1;
+    }
+    L.ray = ray;
+    L.cam = lo;
+    const DevCam& C = A.cams[lo];
+    const long k = ray - pix0[lo];
+    const int i = int(k / C.ny), j = int(k % C.ny);
+    // src/kerr_camera.cpp:110-186 with the observer metric from the host
+    const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
+    const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;
+    const double norm = sqrt(1.0 + u * u + v * v);
+    const double n_r = -1.0 / norm, n_th = v / norm, n_ph = u / norm;
+    const double pt_c = 1.0 / C.alpha;
+    const double pr_c = n_r / sqrt(C.g_rr);
+    const double pth_c = n_th / sqrt(C.g_thth);
+    const double pph_c = n_ph / sqrt(C.g_phph) + C.omega * pt_c;
+    L.pt = C.g_tt * pt_c + C.g_tph * pph_c;
+    L.pph = C.g_tph * pt_c + C.g_phph * pph_c;
+    L.y = Y5{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
+    L.yp = L.y;
+    L.e_obs_c = dmax(-(L.pt + C.omega * L.pph) / C.alpha, 1.0e-300);
+    L.r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
+    L.h = C.h;
+    L.step = 0;
+    L.npts = 0;
+    L.captured = 0;
+    L.exhausted = 0;
+    L.att_in_step = 0;
+    L.attempts = 0;
+}
+
+template <int MODE, int PREC, int NBMAX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HD_BATCH_WAVES_PER_EU)))
+k_trace_batch(BatchArgs A)
+{
+    extern __shared__ double smem[];
+    const int n = A.n_sigma;
+    const double* lnE = smem;
+    const double* lnS = smem + n;
+    const double* slope = smem + 2 * n;
+    const float* lnEf = reinterpret_cast<const float*>(smem);
+    const float* lnSf = lnEf + n;
+    const float* slopef = lnEf + 2 * n;
+    long* pix0 = reinterpret_cast<long*>(smem + 3 * n);   // [n_cams + 1]
+    if constexpr (PREC == HADROS_PREC_FP64) {
+        for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) smem[t] = A.sig[t];
+    } else {
+        float* sf = reinterpret_cast<float*>(smem);
+        for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) sf[t] = A.sigf[t];
+    }
+    for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0[t] = A.pix0[t];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    BatchLane L;
+    L.ray = -1;
+    double tau[NBMAX], I[NBMAX];
+    Kahan ktau[NBMAX];
+    ScaledKahan kI[NBMAX];
+    bool drained = false;   // wave-uniform: the queue is empty
+    unsigned long long c_pts = 0, c_att = 0, c_cap = 0, c_ex = 0;
+    int since_refill = A.steps_per_chunk;
+
+    while (true) {
+        // ---- refill finished lanes from the global queue (wave-uniform control)
+        if (!drained && since_refill >= A.steps_per_chunk) {
+            since_refill = 0;
+            const unsigned long long idle = __ballot(L.ray < 0);
+            if (idle) {
+                const int cnt = __popcll(idle);
+                const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(A.queue, (unsigned long long)cnt);
+                base = __shfl(base, leader, 64);
+                if (base + cnt >= (unsigned long long)A.n_rays) drained = true;
+                if (L.ray < 0) {
+                    const unsigned long long below = idle & ((1ull << lane) - 1ull);
+                    const long my = long(base) + __popcll(below);
+                    if (my < A.n_rays) {
+                        batch_init_ray<MODE>(A, pix0, L, my);
+#pragma unroll
+                        for (int b = 0; b < NBMAX; ++b) {
+                            tau[b] = 0.0;
+                            I[b] = 0.0;
+                            ktau[b] = {0.0f, 0.0f};
+                            kI[b] = {{0.0f, 0.0f}, 0.0f, 0};
+                        }
+                        const bool cap0 = (MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop);
+                        if (cap0) L.captured = 1;
+                        else L.npts = 1;
+                    }
+                }
+            }
+        }
+        if (__ballot(L.ray >= 0) == 0ull) {
+            if (drained) break;
+            since_refill = A.steps_per_chunk;
+            continue;
+        }
+        ++since_refill;
+
+        bool finished = false;
+        if (L.ray >= 0 && !(L.captured && L.step == 0)) {
+            const DevCam& C = A.cams[L.cam];
+            const double a = C.a;
+            Y5 y4, y5;
+            rkf45<MODE>(a, L.y, L.pt, L.pph, L.h, y4, y5);
+            ++L.attempts;
+            bool acc;
+            if constexpr (MODE == HADROS_MODE_REF) {
+                const double h_min = C.h * 1.0e-5;
+                const double err = err_ref(y4, y5);
+                acc = (err < C.tol) || (L.h <= h_min);
+                if (!acc) {
+                    const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
+                    L.h *= dclamp(factor, 0.1, 5.0);
+                    if (L.h < h_min) L.h = h_min;
+                    if (++L.att_in_step == 50) {
+                        y5 = rk4_ref(a, L.y, L.pt, L.pph, C.h);
+                        acc = true;
+                    }
+                }
+                if (acc) {
+                    L.h = C.h;
+                    L.att_in_step = 0;
+                }
+            } else {
+                const double err = err_cfg(L.y, y4, y5, C.tol);
+                acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
+                if (acc) {
+                    double fac = (err > 0.0) ? 0.9 * exp(-0.2 * log(err)) : 5.0;
+                    fac = fmin(5.0, fmax(0.2, fac));
+                    L.h = fmax(L.h * fac, CFG_H_MIN);
+                } else {
+                    double fac = (err == err) ? 0.9 * exp(-0.2 * log(err)) : 0.1;
+                    fac = fmin(0.9, fmax(0.1, fac));
+                    L.h = fmax(L.h * fac, CFG_H_MIN);
+                    if (L.attempts >= 8L * C.max_steps + 64) {
+                        L.exhausted = 1;
+                        finished = true;
+                    }
+                }
+            }
+            if (acc) {
+                L.yp = L.y;
+                L.y = y5;
+                ++L.step;
+                if (L.step >= C.max_steps) {
+                    L.exhausted = 1;
+                    finished = true;
+                } else if ((MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop)) {
+                    L.captured = 1;
+                    finished = true;
+                } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
+                    finished = true;
+                } else {
+                    ++L.npts;
+                    const double dl = zamo_dl(a, L.y, L.yp);
+                    const double z = zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) / L.e_obs_c;
+                    const DevRt& P = A.rts[C.pad_rt];
+                    PointQ q;
+                    if (point_q(P, L.y.r, L.y.th, dl, z, q)) {
+                        if constexpr (PREC == HADROS_PREC_FP64) {
+                            point_rt64<NBMAX>(P, q, lnE, lnS, slope, tau, I);
+                        } else {
+                            const float lnz = float(q.lnz);
+                            const float lP = (q.g3Adl > 0.0) ? float(log(q.g3Adl)) : 0.0f;
+                            const float lN = (q.nbar_dl > 0.0) ? float(log(q.nbar_dl)) : 0.0f;
+                            const float gam = float(P.gam), invcut = float(P.invcut);
+#pragma unroll
+                            for (int b = 0; b < NBMAX; ++b) {
+                                if (b >= P.nb) continue;
+                                const double E = P.emid[b] * q.z;
+                                const float lnEb = float(P.lnemid[b]) + lnz;
+                                if (P.spec_ok && q.g3Adl > 0.0)
+                                    kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
+                                if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
+                                    ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb)));
+                            }
+                        }
+                    }
+                }
+            }
+        } else if (L.ray >= 0) {
+            finished = true;   // captured at the camera: no points
+        }
+
+        if (finished) {
+            // ---- outputs of ray L.ray (aggregation of src/radiative_transfer.cpp:711-769)
+            const DevRt& P = A.rts[A.cams[L.cam].pad_rt];
+            const long k = L.ray;
+            double bt[NBMAX], bp[NBMAX], bi[NBMAX];
+#pragma unroll
+            for (int b = 0; b < NBMAX; ++b) {
+                if (PREC == HADROS_PREC_FP64) {
+                    bt[b] = tau[b];
+                    bi[b] = I[b];
+                } else {
+                    bt[b] = double(ktau[b].s) - double(ktau[b].c);
+                    bi[b] = kI[b].set ? (double(kI[b].k.s) - double(kI[b].k.c)) * exp(double(kI[b].off)) : 0.0;
+                }
+                bp[b] = exp(-bt[b]);
+            }
+            double agg_t = 0.0, agg_p = 1.0, agg_i = 0.0;
+            if (P.mono) {
+                if (P.nb > 0) {
+                    agg_t = bt[0];
+                    agg_p = bp[0];
+                    agg_i = bi[0];
+                }
+            } else {
+                double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    if (b >= P.nb) continue;
+                    ws += P.w[b];
+                    ts += P.w[b] * bt[b];
+                    ps += P.w[b] * bp[b];
+                    is += bi[b] * P.de[b];
+                }
+                if (ws > 0.0) {
+                    agg_t = ts / ws;
+                    agg_p = ps / ws;
+                    agg_i = is;
+                }
+            }
+            const TraceOut& O = A.out;
+            if (O.tau) O.tau[k] = agg_t;
+            if (O.p) O.p[k] = agg_p;
+            if (O.I) O.I[k] = agg_i;
+            if (O.captured) O.captured[k] = L.captured;
+            if (O.npoints) O.npoints[k] = L.npts;
+            if (O.flags) O.flags[k] = L.exhausted ? HADROS_FLAG_EXHAUSTED : 0;
+#pragma unroll
+            for (int b = 0; b < NBMAX; ++b) {
+                if (b >= A.nb_out) continue;
+                const bool on = b < P.nb;
+                if (O.btau) O.btau[k * A.nb_out + b] = on ? bt[b] : 0.0;
+                if (O.bp) O.bp[k * A.nb_out + b] = on ? bp[b] : 1.0;
+                if (O.bi) O.bi[k * A.nb_out + b] = on ? bi[b] : 0.0;
+            }
+            c_pts += (unsigned long long)L.npts;
+            c_att += (unsigned long long)L.attempts;
+            c_cap += (unsigned long long)L.captured;
+            c_ex += (unsigned long long)L.exhausted;
+            L.ray = -1;
+        }
+    }
+    wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
+}
+
+template <int MODE, int PREC>
+static void launch_b(const BatchArgs& A, int nb, int grid, int block, size_t lds, hipStream_t s)
+{
+    if (nb <= 1) hipLaunchKernelGGL((k_trace_batch<MODE, PREC, 1>), dim3(grid), dim3(block), lds, s, A);
+    else if (nb <= 8) hipLaunchKernelGGL((k_trace_batch<MODE, PREC, 8>), dim3(grid), dim3(block), lds, s, A);
+    else hipLaunchKernelGGL((k_trace_batch<MODE, PREC, 16>), dim3(grid), dim3(block), lds, s, A);
+}
+
+size_t batch_lds_bytes(int n_sigma, int n_cams, int precision)
+{
+    return size_t(3) * n_sigma * sizeof(double) + size_t(n_cams + 1) * sizeof(long) +
+           (precision == HADROS_PREC_FP64 ? 0 : 0);
+}
+
+hipError_t batch_resident_blocks(int mode, int precision, int nb, int block, int n_sigma, int n_cams, int* blocks_per_cu)
+{
+    const size_t lds = batch_lds_bytes(n_sigma, n_cams, precision);
+    const void* f = nullptr;
+#define HD_PICK(M, P, N) f = reinterpret_cast<const void*>(&k_trace_batch<M, P, N>)
+    if (mode == HADROS_MODE_CFG) {
+        if (precision == HADROS_PREC_FP64) {
+            if (nb <= 1) HD_PICK(HADROS_MODE_CFG, HADROS_PREC_FP64, 1);
+            else if (nb <= 8) HD_PICK(HADROS_MODE_CFG, HADROS_PREC_FP64, 8);
+            else HD_PICK(HADROS_MODE_CFG, HADROS_PREC_FP64, 16);
+        } else {
+            if (nb <= 1) HD_PICK(HADROS_MODE_CFG, HADROS_PREC_FP32K, 1);
+            else if (nb <= 8) HD_PICK(HADROS_MODE_CFG, HADROS_PREC_FP32K, 8);
+            else HD_PICK(HADROS_MODE_CFG, HADROS_PREC_FP32K, 16);
+        }
+    } else {
+        if (precision == HADROS_PREC_FP64) {
+            if (nb <= 1) HD_PICK(HADROS_MODE_REF, HADROS_PREC_FP64, 1);
+            else if (nb <= 8) HD_PICK(HADROS_MODE_REF, HADROS_PREC_FP64, 8);
+            else HD_PICK(HADROS_MODE_REF, HADROS_PREC_FP64, 16);
+        } else {
+            if (nb <= 1) HD_PICK(HADROS_MODE_REF, HADROS_PREC_FP32K, 1);
+            else if (nb <= 8) HD_PICK(HADROS_MODE_REF, HADROS_PREC_FP32K, 8);
+            else HD_PICK(HADROS_MODE_REF, HADROS_PREC_FP32K, 16);
+        }
+    }
+#undef HD_PICK
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, lds);
+}
+
+hipError_t launch_trace_batch(const BatchArgs& A, int mode, int precision, int nb, int grid, int block, hipStream_t s)
+{
+    const size_t lds = batch_lds_bytes(A.n_sigma, A.n_cams, precision);
+    if (mode == HADROS_MODE_CFG) {
+        if (precision == HADROS_PREC_FP64) launch_b<HADROS_MODE_CFG, HADROS_PREC_FP64>(A, nb, grid, block, lds, s);
+        else launch_b<HADROS_MODE_CFG, HADROS_PREC_FP32K>(A, nb, grid, block, lds, s);
+    } else {
+        if (precision == HADROS_PREC_FP64) launch_b<HADROS_MODE_REF, HADROS_PREC_FP64>(A, nb, grid, block, lds, s);
+        else launch_b<HADROS_MODE_REF, HADROS_PREC_FP32K>(A, nb, grid, block, lds, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hd

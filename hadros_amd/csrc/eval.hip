@@ -48,7 +48,7 @@ __global__ void k_eval_rhs(double a, int mode, const double* y, double* dy, long
     double* o = dy + 8 * k;
     // dt/dlambda = g^tt p_t + g^tphi p_phi
     double sn, cs;
-    sincos(s[2], &sn, &cs);
+    sincos_m(s[2], &sn, &cs);
     const Metric5 g = inv_metric(a, s[1], sn, cs);
     o[0] = g.tt * s[4] + g.tph * s[7];
     o[1] = d.r;
@@ -66,7 +66,7 @@ __global__ void k_eval_metric(double a, const double* r, const double* th, doubl
     if (k >= n) return;
     const Metric5 g = metric(a, r[k], th[k]);
     double sn, cs;
-    sincos(th[k], &sn, &cs);
+    sincos_m(th[k], &sn, &cs);
     const Metric5 gi = inv_metric(a, r[k], sn, cs);
     double* o = out + 10 * k;
     o[0] = g.tt; o[1] = g.tph; o[2] = g.rr; o[3] = g.thth; o[4] = g.phph;

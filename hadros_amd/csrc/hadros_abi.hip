@@ -297,6 +297,7 @@ struct hadros_tracer {
     double *d_bt = nullptr, *d_bp = nullptr, *d_bi = nullptr;
     int *d_cap = nullptr, *d_np = nullptr, *d_flags = nullptr;
     unsigned long long* d_cnt = nullptr;
+    DevRt* d_rt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ran = false;
 };
@@ -311,6 +312,24 @@ struct hadros_replay {
     double *d_r = nullptr, *d_th = nullptr, *d_dl = nullptr, *d_z = nullptr;
     double *d_tau = nullptr, *d_p = nullptr, *d_I = nullptr, *d_bt = nullptr, *d_bp = nullptr, *d_bi = nullptr;
     unsigned long long* d_cnt = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ran = false;
+};
+
+struct hadros_batch {
+    BatchArgs args{};
+    int mode = 0, precision = 0, nb_max = 0, block = 128, spc = 8, cu = 0;
+    long n_rays = 0;
+    std::vector<long> pix0;
+    std::vector<int> nbc;
+    DevCam* d_cams = nullptr;
+    long* d_pix0 = nullptr;
+    DevRt* d_rts = nullptr;
+    double* d_sig = nullptr;
+    float* d_sigf = nullptr;
+    unsigned long long *d_queue = nullptr, *d_cnt = nullptr;
+    double *d_tau = nullptr, *d_p = nullptr, *d_I = nullptr, *d_bt = nullptr, *d_bp = nullptr, *d_bi = nullptr;
+    int *d_cap = nullptr, *d_np = nullptr, *d_flags = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ran = false;
 };
@@ -528,10 +547,11 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
         (rc = dalloc(&t->d_p, np)) || (rc = dalloc(&t->d_I, np)) || (rc = dalloc(&t->d_alpha, np)) ||
         (rc = dalloc(&t->d_beta, np)) || (rc = dalloc(&t->d_bt, np * nb)) || (rc = dalloc(&t->d_bp, np * nb)) ||
         (rc = dalloc(&t->d_bi, np * nb)) || (rc = dalloc(&t->d_cap, np)) || (rc = dalloc(&t->d_np, np)) ||
-        (rc = dalloc(&t->d_flags, np)) || (rc = dalloc(&t->d_cnt, 4))) {
+        (rc = dalloc(&t->d_flags, np)) || (rc = dalloc(&t->d_cnt, 4)) || (rc = dalloc(&t->d_rt, 1))) {
         hadros_tracer_destroy(t);
         return rc;
     }
+    HIPC(hipMemcpy(t->d_rt, &t->args.rt, sizeof(DevRt), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(t->d_sig, sd.data(), sd.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(t->d_sigf, sf.data(), sf.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPC(hipEventCreate(&t->ev0));
@@ -539,6 +559,7 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
     TraceArgs& A = t->args;
     A.sig = t->d_sig;
     A.sigf = t->d_sigf;
+    A.rtg = t->d_rt;
     A.out = TraceOut{t->d_tau, t->d_p, t->d_I, t->d_alpha, t->d_beta, t->d_bt, t->d_bp, t->d_bi,
                      t->d_cap, t->d_np, t->d_flags, t->d_cnt};
     *out = t;
@@ -631,6 +652,7 @@ int hadros_tracer_destroy(hadros_tracer* t)
     dfree(t->d_np);
     dfree(t->d_flags);
     dfree(t->d_cnt);
+    dfree(t->d_rt);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
     delete t;
@@ -812,8 +834,255 @@ int hadros_rt_from_cache(const hadros_cache_view* c, const hadros_scene* scene, 
     return rc;
 }
 
+
+// ---------------------------------------------------------------- batched tracer
+int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, int n_cams, const double* se,
+                        const double* ss, int n_sigma, int precision, hadros_batch** out)
+{
+    if (!cams || !scenes || !out || n_cams <= 0) return fail(HADROS_ERR_ARG, "bad batch arguments");
+    if (precision != HADROS_PREC_FP64 && precision != HADROS_PREC_FP32K) return fail(HADROS_ERR_ARG, "unknown precision");
+    for (int c = 1; c < n_cams; ++c)
+        if (cams[c].mode != cams[0].mode) return fail(HADROS_ERR_ARG, "all cameras of a batch must share one integrator mode");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hadros_batch* b = new hadros_batch();
+    std::vector<DevCam> dc(static_cast<size_t>(n_cams));
+    std::vector<DevRt> dr(static_cast<size_t>(n_cams));
+    b->pix0.assign(size_t(n_cams) + 1, 0);
+    b->nbc.assign(size_t(n_cams), 0);
+    for (int c = 0; c < n_cams; ++c) {
+        if ((rc = make_cam(cams[c], dc[c])) || (rc = make_rt(scenes[c], se, ss, n_sigma, cams[c].a_spin, dr[c]))) {
+            delete b;
+            return rc;
+        }
+        dc[c].pad_rt = c;
+        b->nbc[c] = dr[c].nb;
+        b->nb_max = std::max(b->nb_max, dr[c].nb);
+        b->pix0[c + 1] = b->pix0[c] + long(cams[c].nx) * cams[c].ny;
+    }
+    b->mode = cams[0].mode;
+    b->precision = precision;
+    b->n_rays = b->pix0[n_cams];
+    std::vector<double> sd;
+    std::vector<float> sf;
+    sigma_tables(se, ss, n_sigma, sd, sf);
+    const size_t nr = size_t(b->n_rays), nb = size_t(b->nb_max);
+    if ((rc = dalloc(&b->d_cams, size_t(n_cams))) || (rc = dalloc(&b->d_pix0, size_t(n_cams) + 1)) ||
+        (rc = dalloc(&b->d_rts, size_t(n_cams))) || (rc = dalloc(&b->d_sig, sd.size())) ||
+        (rc = dalloc(&b->d_sigf, sf.size())) || (rc = dalloc(&b->d_queue, 1)) || (rc = dalloc(&b->d_cnt, 4)) ||
+        (rc = dalloc(&b->d_tau, nr)) || (rc = dalloc(&b->d_p, nr)) || (rc = dalloc(&b->d_I, nr)) ||
+        (rc = dalloc(&b->d_bt, nr * nb)) || (rc = dalloc(&b->d_bp, nr * nb)) || (rc = dalloc(&b->d_bi, nr * nb)) ||
+        (rc = dalloc(&b->d_cap, nr)) || (rc = dalloc(&b->d_np, nr)) || (rc = dalloc(&b->d_flags, nr))) {
+        hadros_batch_destroy(b);
+        return rc;
+    }
+    HIPC(hipMemcpy(b->d_cams, dc.data(), dc.size() * sizeof(DevCam), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(b->d_pix0, b->pix0.data(), b->pix0.size() * sizeof(long), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(b->d_rts, dr.data(), dr.size() * sizeof(DevRt), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(b->d_sig, sd.data(), sd.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(b->d_sigf, sf.data(), sf.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(hipEventCreate(&b->ev0));
+    HIPC(hipEventCreate(&b->ev1));
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    HIPC(hipGetDeviceProperties(&prop, dev));
+    b->cu = prop.multiProcessorCount;
+    BatchArgs& A = b->args;
+    A.cams = b->d_cams;
+    A.pix0 = b->d_pix0;
+    A.rts = b->d_rts;
+    A.sig = b->d_sig;
+    A.sigf = b->d_sigf;
+    A.n_sigma = n_sigma;
+    A.n_cams = n_cams;
+    A.nb_out = b->nb_max;
+    A.steps_per_chunk = b->spc;
+    A.n_rays = b->n_rays;
+    A.queue = b->d_queue;
+    A.out = TraceOut{b->d_tau, b->d_p, b->d_I, nullptr, nullptr, b->d_bt, b->d_bp, b->d_bi,
+                     b->d_cap, b->d_np, b->d_flags, b->d_cnt};
+    *out = b;
+    return HADROS_OK;
+}
+
+int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk)
+{
+    if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
+    if (block < 64 || block > 256 || block % 64) return fail(HADROS_ERR_ARG, "block size must be 64, 128, 192 or 256");
+    if (steps_per_chunk < 1) return fail(HADROS_ERR_ARG, "steps_per_chunk must be >= 1");
+    b->block = block;
+    b->spc = steps_per_chunk;
+    b->args.steps_per_chunk = steps_per_chunk;
+    return HADROS_OK;
+}
+
+static int batch_launch(hadros_batch* b, hipStream_t s, long ray_limit)
+{
+    int per_cu = 0;
+    HIPC(batch_resident_blocks(b->mode, b->precision, b->nb_max, b->block, b->args.n_sigma, b->args.n_cams, &per_cu));
+    const int grid = std::max(1, per_cu) * b->cu;
+    BatchArgs A = b->args;
+    A.n_rays = ray_limit;
+    HIPC(launch_zero_u64(b->d_queue, 1, s));
+    HIPC(launch_zero_u64(b->d_cnt, 4, s));
+    HIPC(hipEventRecord(b->ev0, s));
+    HIPC(launch_trace_batch(A, b->mode, b->precision, b->nb_max, grid, b->block, s));
+    HIPC(hipEventRecord(b->ev1, s));
+    b->ran = true;
+    return HADROS_OK;
+}
+
+int hadros_batch_run(hadros_batch* b, void* stream)
+{
+    if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
+    return batch_launch(b, reinterpret_cast<hipStream_t>(stream), b->n_rays);
+}
+
+int hadros_batch_last_ms(hadros_batch* b, double* ms)
+{
+    if (!b || !ms || !b->ran) return fail(HADROS_ERR_ARG, "batch has not run");
+    HIPC(hipEventSynchronize(b->ev1));
+    float f = 0.0f;
+    HIPC(hipEventElapsedTime(&f, b->ev0, b->ev1));
+    *ms = f;
+    return HADROS_OK;
+}
+
+// Times every (block, steps_per_chunk) pair on the first eighth of the rays
+// (at least 2^18) and keeps the fastest.
+int hadros_batch_autotune(hadros_batch* b, int* block_out, int* spc_out)
+{
+    if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
+    const long limit = std::min(b->n_rays, std::max(b->n_rays / 8, 1L << 18));
+    const int blocks[] = {64, 128, 256};
+    const int spcs[] = {1, 4, 16, 64};
+    double best = 1e300;
+    int bb = b->block, bs = b->spc;
+    for (int blk : blocks)
+        for (int spc : spcs) {
+            b->block = blk;
+            b->args.steps_per_chunk = spc;
+            double ms = 0.0;
+            int rc = batch_launch(b, nullptr, limit);
+            if (!rc) rc = hadros_batch_last_ms(b, &ms);
+            if (!rc) rc = batch_launch(b, nullptr, limit);
+            if (!rc) rc = hadros_batch_last_ms(b, &ms);
+            if (rc) return rc;
+            if (ms < best) {
+                best = ms;
+                bb = blk;
+                bs = spc;
+            }
+        }
+    b->block = bb;
+    b->spc = bs;
+    b->args.steps_per_chunk = bs;
+    b->ran = false;
+    if (block_out) *block_out = bb;
+    if (spc_out) *spc_out = bs;
+    return HADROS_OK;
+}
+
+int hadros_batch_fetch(hadros_batch* b, int cam, hadros_image_out* o)
+{
+    if (!b || !b->ran) return fail(HADROS_ERR_ARG, "batch has not run");
+    if (cam < 0 || cam >= b->args.n_cams) return fail(HADROS_ERR_ARG, "camera index out of range");
+    HIPC(hipEventSynchronize(b->ev1));
+    const size_t k0 = size_t(b->pix0[cam]), n = size_t(b->pix0[cam + 1] - b->pix0[cam]);
+    const size_t nbo = size_t(b->nb_max), nbc = size_t(std::max(b->nbc[cam], 0));
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst && src && bytes) HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return HADROS_OK;
+    };
+    int rc = 0;
+    if ((rc = cp(o->tau, b->d_tau + k0, n * 8)) || (rc = cp(o->p_surv, b->d_p + k0, n * 8)) ||
+        (rc = cp(o->intensity, b->d_I + k0, n * 8)) || (rc = cp(o->captured, b->d_cap + k0, n * 4)) ||
+        (rc = cp(o->npoints, b->d_np + k0, n * 4)) || (rc = cp(o->flags, b->d_flags + k0, n * 4)))
+        return rc;
+    if (nbc && (o->bin_tau || o->bin_p || o->bin_i)) {
+        std::vector<double> t(n * nbo);
+        double* dsts[3] = {o->bin_tau, o->bin_p, o->bin_i};
+        const double* srcs[3] = {b->d_bt, b->d_bp, b->d_bi};
+        for (int q = 0; q < 3; ++q) {
+            if (!dsts[q]) continue;
+            HIPC(hipMemcpy(t.data(), srcs[q] + k0 * nbo, n * nbo * 8, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k)
+                for (size_t c = 0; c < nbc; ++c) dsts[q][k * nbc + c] = t[k * nbo + c];
+        }
+    }
+    if (o->alpha || o->beta) {
+        // tan-plane coordinates of the camera's pixels (src/kerr_camera.cpp:196-200)
+        DevCam dc;
+        HIPC(hipMemcpy(&dc, b->d_cams + cam, sizeof(DevCam), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) {
+            const int i = int(k / size_t(dc.ny)), j = int(k % size_t(dc.ny));
+            if (o->alpha) o->alpha[k] = (2.0 * (i + 0.5) / dc.nx - 1.0) * dc.tanhalf;
+            if (o->beta) o->beta[k] = (2.0 * (j + 0.5) / dc.ny - 1.0) * dc.tanhalf;
+        }
+    }
+    return HADROS_OK;
+}
+
+int hadros_batch_stats(hadros_batch* b, hadros_stats* st)
+{
+    if (!b || !st || !b->ran) return fail(HADROS_ERR_ARG, "batch has not run");
+    HIPC(hipEventSynchronize(b->ev1));
+    unsigned long long c[4];
+    HIPC(hipMemcpy(c, b->d_cnt, sizeof(c), hipMemcpyDeviceToHost));
+    float f = 0.0f;
+    HIPC(hipEventElapsedTime(&f, b->ev0, b->ev1));
+    st->kernel_ms = f;
+    st->total_ms = f;
+    st->total_points = (int64_t)c[0];
+    st->total_attempts = (int64_t)c[1];
+    st->n_captured = int32_t(c[2]);
+    st->n_exhausted = int32_t(c[3]);
+    st->n_bins = b->nb_max;
+    st->block_size = b->block;
+    return HADROS_OK;
+}
+
+int hadros_batch_info(hadros_batch* b, int64_t* n_rays, int* block, int* steps_per_chunk, int* grid)
+{
+    if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
+    int per_cu = 0;
+    HIPC(batch_resident_blocks(b->mode, b->precision, b->nb_max, b->block, b->args.n_sigma, b->args.n_cams, &per_cu));
+    if (n_rays) *n_rays = b->n_rays;
+    if (block) *block = b->block;
+    if (steps_per_chunk) *steps_per_chunk = b->spc;
+    if (grid) *grid = std::max(1, per_cu) * b->cu;
+    return HADROS_OK;
+}
+
+int hadros_batch_destroy(hadros_batch* b)
+{
+    if (!b) return HADROS_OK;
+    dfree(b->d_cams);
+    dfree(b->d_pix0);
+    dfree(b->d_rts);
+    dfree(b->d_sig);
+    dfree(b->d_sigf);
+    dfree(b->d_queue);
+    dfree(b->d_cnt);
+    dfree(b->d_tau);
+    dfree(b->d_p);
+    dfree(b->d_I);
+    dfree(b->d_bt);
+    dfree(b->d_bp);
+    dfree(b->d_bi);
+    dfree(b->d_cap);
+    dfree(b->d_np);
+    dfree(b->d_flags);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+    return HADROS_OK;
+}
+
 // ---------------------------------------------------------------- geodesic cache
-int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captured, double* points, int64_t cap)
+int hadros_trace_cache_range(const hadros_camera* cam, int64_t k0, int64_t k1, int64_t* offset, int32_t* captured,
+                             double* points, int64_t cap)
 {
     if (!cam || !offset) return fail(HADROS_ERR_ARG, "NULL argument");
     int rc = ensure_device();
@@ -821,14 +1090,27 @@ int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captu
     CacheArgs A{};
     rc = make_cam(*cam, A.cam);
     if (rc) return rc;
-    const size_t np = size_t(cam->nx) * size_t(cam->ny);
+    const long npix = long(cam->nx) * long(cam->ny);
+    if (k0 < 0 || k1 > npix || k1 < k0) return fail(HADROS_ERR_ARG, "pixel range outside the camera");
+    A.k0 = long(k0);
+    A.k1 = long(k1);
+    const size_t np = size_t(k1 - k0);
+    if (np == 0) {
+        offset[0] = 0;
+        return HADROS_OK;
+    }
     long* d_counts = nullptr;
     int* d_cap = nullptr;
     long* d_off = nullptr;
     double* d_pts = nullptr;
-    if ((rc = dalloc(&d_counts, np)) || (rc = dalloc(&d_cap, np)) || (rc = dalloc(&d_off, np + 1))) {
+    auto cleanup = [&]() {
         dfree(d_counts);
         dfree(d_cap);
+        dfree(d_off);
+        dfree(d_pts);
+    };
+    if ((rc = dalloc(&d_counts, np)) || (rc = dalloc(&d_cap, np)) || (rc = dalloc(&d_off, np + 1))) {
+        cleanup();
         return rc;
     }
     // pass 1: point counts (the trace is deterministic, so pass 2 reproduces it)
@@ -840,9 +1122,7 @@ int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captu
     if (e == hipSuccess) e = hipMemcpy(counts.data(), d_counts, np * sizeof(long), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(capv.data(), d_cap, np * sizeof(int), hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
-        dfree(d_counts);
-        dfree(d_cap);
-        dfree(d_off);
+        cleanup();
         return fail(HADROS_ERR_DEVICE, std::string("trace_cache count pass: ") + hipGetErrorString(e));
     }
     std::vector<long> off(np + 1, 0);
@@ -852,15 +1132,11 @@ int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captu
         for (size_t k = 0; k < np; ++k) captured[k] = capv[k];
     if (points) {
         if (cap < off[np]) {
-            dfree(d_counts);
-            dfree(d_cap);
-            dfree(d_off);
-            return fail(HADROS_ERR_MAX_POINTS, "point buffer smaller than offset[nx*ny]");
+            cleanup();
+            return fail(HADROS_ERR_MAX_POINTS, "point buffer smaller than the traced point count");
         }
         if ((rc = dalloc(&d_pts, size_t(off[np]) * 7))) {
-            dfree(d_counts);
-            dfree(d_cap);
-            dfree(d_off);
+            cleanup();
             return rc;
         }
         e = hipMemcpy(d_off, off.data(), (np + 1) * sizeof(long), hipMemcpyHostToDevice);
@@ -871,12 +1147,15 @@ int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captu
             e = hipMemcpy(points, d_pts, size_t(off[np]) * 7 * sizeof(double), hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipDeviceSynchronize();
     }
-    dfree(d_counts);
-    dfree(d_cap);
-    dfree(d_off);
-    dfree(d_pts);
+    cleanup();
     if (e != hipSuccess) return fail(HADROS_ERR_DEVICE, std::string("trace_cache: ") + hipGetErrorString(e));
     return HADROS_OK;
+}
+
+int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captured, double* points, int64_t cap)
+{
+    if (!cam) return fail(HADROS_ERR_ARG, "NULL camera");
+    return hadros_trace_cache_range(cam, 0, int64_t(cam->nx) * cam->ny, offset, captured, points, cap);
 }
 
 }  // extern "C"

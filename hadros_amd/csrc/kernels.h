@@ -14,6 +14,7 @@ struct TraceOut {
 struct TraceArgs {
     DevCam cam;
     DevRt rt;
+    const DevRt* rtg;     // device copy of rt, read per path point (see k_trace_rt)
     const double* sig;    // 3*n_sigma: ln E, ln sigma, slope (FP64)
     const float* sigf;    // same, FP32
     TraceOut out;
@@ -21,6 +22,7 @@ struct TraceArgs {
 
 struct CacheArgs {
     DevCam cam;
+    long k0, k1;          // pixel range [k0, k1), pixel k = i*ny + j
     const long* offset;   // [npix+1] (write pass)
     double* points;       // 7 doubles per point (write pass)
     long* counts;         // [npix] (count pass)
@@ -36,6 +38,18 @@ struct ReplayArgs {
     const double *r, *th, *dl, *z;
     double *tau, *p, *I, *btau, *bp, *bi;
     unsigned long long* counters;
+};
+
+struct BatchArgs {
+    const DevCam* cams;   // [n_cams], cams[c].pad_rt indexes rts
+    const long* pix0;     // [n_cams+1] first global ray of each camera
+    const DevRt* rts;     // per-scene transfer parameters
+    const double* sig;    // shared sigma table (3*n_sigma, FP64)
+    const float* sigf;    // same, FP32
+    int n_sigma, n_cams, nb_out, steps_per_chunk;
+    long n_rays;
+    unsigned long long* queue;   // next unclaimed ray
+    TraceOut out;                // indexed by global ray (bins: ray*nb_out + b)
 };
 
 // Sum four 64-bit per-lane values over the wavefront and add them to
@@ -61,6 +75,8 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStream_t s);
 hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s);
 hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s);
+hipError_t launch_trace_batch(const BatchArgs& A, int mode, int precision, int nb, int grid, int block, hipStream_t s);
+hipError_t batch_resident_blocks(int mode, int precision, int nb, int block, int n_sigma, int n_cams, int* blocks_per_cu);
 
 // point-wise hooks (eval.hip)
 hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th, double* out, long n);

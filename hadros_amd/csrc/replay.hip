@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
             if constexpr (PREC == HADROS_PREC_FP64) {
 #pragma unroll
                 for (int b = 0; b < NBMAX; ++b) {
-                    if (b >= P.nb) break;
+                    if (b >= P.nb) continue;
                     const double E = P.emid[b] * pq.z;
                     const double lnEb = P.lnemid[b] + pq.lnz;
                     double dtau = 0.0;
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
                 const float gam = float(P.gam), invcut = float(P.invcut);
 #pragma unroll
                 for (int b = 0; b < NBMAX; ++b) {
-                    if (b >= P.nb) break;
+                    if (b >= P.nb) continue;
                     const double E = P.emid[b] * pq.z;
                     const float lnEb = float(P.lnemid[b]) + lnz;
                     float dtau = 0.0f;
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
                     double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
 #pragma unroll
                     for (int b = 0; b < NBMAX; ++b) {
-                        if (b >= P.nb) break;
+                        if (b >= P.nb) continue;
                         ws += P.w[b];
                         ts += P.w[b] * bt[b];
                         ps += P.w[b] * bp[b];
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
             if (A.I) A.I[ray] = agg_i;
 #pragma unroll
             for (int b = 0; b < NBMAX; ++b) {
-                if (b >= P.nb) break;
+                if (b >= P.nb) continue;
                 if (A.btau) A.btau[ray * P.nb + b] = bt[b];
                 if (A.bp) A.bp[ray * P.nb + b] = bp[b];
                 if (A.bi) A.bi[ray * P.nb + b] = bi[b];

@@ -8,10 +8,17 @@
 // lengths (less tail divergence than a 64-pixel image row).
 #include "kernels.h"
 
+// Waves per SIMD the register allocator targets for the fused kernel (the
+// FP64 state, six RKF45 stages and the per-bin accumulators compete for VGPRs).
+#ifndef HD_TRACE_WAVES_PER_EU
+#define HD_TRACE_WAVES_PER_EU 2
+#endif
+
 namespace hd {
 
 template <int MODE, int PREC, int NBMAX>
-__global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HD_TRACE_WAVES_PER_EU)))
+k_trace_rt(TraceArgs A)
 {
     extern __shared__ double smem[];
     const DevRt& P = A.rt;
@@ -96,7 +103,7 @@ __global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
             const double err = err_ref(y4, y5);
             acc = (err < C.tol) || (h <= h_min);
             if (!acc) {
-                const double factor = 0.8 * pow(C.tol / dmax(err, 1.0e-30), 0.25);
+                const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));   // x^(1/4)
                 h *= dclamp(factor, 0.1, 5.0);
                 if (h < h_min) h = h_min;
                 if (++att_in_step == 50) {
@@ -112,11 +119,11 @@ __global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
             const double err = err_cfg(y, y4, y5, C.tol);
             acc = (err <= 1.0) || (h <= CFG_H_MIN);
             if (acc) {
-                double fac = (err > 0.0) ? 0.9 * pow(err, -0.2) : 5.0;
+                double fac = (err > 0.0) ? 0.9 * exp(-0.2 * log(err)) : 5.0;
                 fac = fmin(5.0, fmax(0.2, fac));
                 h = fmax(h * fac, CFG_H_MIN);
             } else {
-                double fac = (err == err) ? 0.9 * pow(err, -0.2) : 0.1;
+                double fac = (err == err) ? 0.9 * exp(-0.2 * log(err)) : 0.1;
                 fac = fmin(0.9, fmax(0.1, fac));
                 h = fmax(h * fac, CFG_H_MIN);
                 if (attempts >= max_attempts) {
@@ -144,23 +151,29 @@ __global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
         ++npts;
         const double dl = zamo_dl(a, y, yp);
         const double z = zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c;
+        // The scene block is re-read through scalar loads at every point: an
+        // empty asm on the pointer keeps the compiler from hoisting ~100 uniform
+        // doubles out of the loop into vector registers.
+        const DevRt* Pp = A.rtg;
+        asm volatile("" : "+s"(Pp));
+        const DevRt& Pq = *Pp;
         PointQ q;
-        if (point_q(P, y.r, y.th, dl, z, q)) {
+        if (point_q(Pq, y.r, y.th, dl, z, q)) {
             if constexpr (PREC == HADROS_PREC_FP64) {
-                point_rt64<NBMAX>(P, q, lnE, lnS, slope, tau, I);
+                point_rt64<NBMAX>(Pq, q, lnE, lnS, slope, tau, I);
             } else {
                 const float lnz = float(q.lnz);
                 const float lP = (q.g3Adl > 0.0) ? float(log(q.g3Adl)) : 0.0f;
                 const float lN = (q.nbar_dl > 0.0) ? float(log(q.nbar_dl)) : 0.0f;
-                const float gam = float(P.gam), invcut = float(P.invcut);
+                const float gam = float(Pq.gam), invcut = float(Pq.invcut);
 #pragma unroll
                 for (int b = 0; b < NBMAX; ++b) {
-                    if (b >= P.nb) break;
-                    const double E = P.emid[b] * q.z;
-                    const float lnEb = float(P.lnemid[b]) + lnz;
-                    if (P.spec_ok && q.g3Adl > 0.0)
+                    if (b >= Pq.nb) continue;
+                    const double E = Pq.emid[b] * q.z;
+                    const float lnEb = float(Pq.lnemid[b]) + lnz;
+                    if (Pq.spec_ok && q.g3Adl > 0.0)
                         kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
-                    if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
+                    if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= Pq.e_lo && E <= Pq.e_hi)
                         ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb)));
                 }
             }
@@ -198,7 +211,7 @@ __global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
             double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
 #pragma unroll
             for (int b = 0; b < NBMAX; ++b) {
-                if (b >= P.nb) break;
+                if (b >= P.nb) continue;
                 ws += P.w[b];
                 ts += P.w[b] * bt[b];
                 ps += P.w[b] * bp[b];
@@ -221,7 +234,7 @@ __global__ void __launch_bounds__(256) k_trace_rt(TraceArgs A)
         if (O.beta) O.beta[k] = v;
 #pragma unroll
         for (int b = 0; b < NBMAX; ++b) {
-            if (b >= P.nb) break;
+            if (b >= P.nb) continue;
             if (O.btau) O.btau[k * P.nb + b] = bt[b];
             if (O.bp) O.bp[k * P.nb + b] = bp[b];
             if (O.bi) O.bi[k * P.nb + b] = bi[b];
@@ -235,8 +248,9 @@ template <int MODE, bool WRITE>
 __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
 {
     const DevCam& C = A.cam;
-    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= long(C.nx) * C.ny) return;
+    const long kk = long(blockIdx.x) * blockDim.x + threadIdx.x;   // index within the range
+    const long k = A.k0 + kk;
+    if (k >= A.k1) return;
     const int i = int(k / C.ny), j = int(k % C.ny);
     const double a = C.a;
     const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
@@ -255,15 +269,15 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
     double* out = nullptr;
     long cap = 0;
     if constexpr (WRITE) {
-        out = A.points + 7 * A.offset[k];
-        cap = A.offset[k + 1] - A.offset[k];
+        out = A.points + 7 * A.offset[kk];
+        cap = A.offset[kk + 1] - A.offset[kk];
     }
     auto emit = [&](long idx, const Y5& yy, double dl, double z) {
         if constexpr (WRITE) {
             if (idx < cap) {
                 double st, ct, sp, cp;
-                sincos(yy.th, &st, &ct);
-                sincos(yy.ph, &sp, &cp);
+                sincos_m(yy.th, &st, &ct);
+                sincos_m(yy.ph, &sp, &cp);
                 double* o = out + 7 * idx;
                 o[0] = yy.r;
                 o[1] = yy.th;
@@ -299,7 +313,7 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
             const double err = err_ref(y4, y5);
             acc = (err < C.tol) || (h <= h_min);
             if (!acc) {
-                const double factor = 0.8 * pow(C.tol / dmax(err, 1.0e-30), 0.25);
+                const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));   // x^(1/4)
                 h *= dclamp(factor, 0.1, 5.0);
                 if (h < h_min) h = h_min;
                 if (++att_in_step == 50) {
@@ -315,11 +329,11 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
             const double err = err_cfg(y, y4, y5, C.tol);
             acc = (err <= 1.0) || (h <= CFG_H_MIN);
             if (acc) {
-                double fac = (err > 0.0) ? 0.9 * pow(err, -0.2) : 5.0;
+                double fac = (err > 0.0) ? 0.9 * exp(-0.2 * log(err)) : 5.0;
                 fac = fmin(5.0, fmax(0.2, fac));
                 h = fmax(h * fac, CFG_H_MIN);
             } else {
-                double fac = (err == err) ? 0.9 * pow(err, -0.2) : 0.1;
+                double fac = (err == err) ? 0.9 * exp(-0.2 * log(err)) : 0.1;
                 fac = fmin(0.9, fmax(0.1, fac));
                 h = fmax(h * fac, CFG_H_MIN);
                 if (attempts >= max_attempts) break;
@@ -340,8 +354,8 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
         emit(npts++, y, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
     }
     if constexpr (!WRITE) {
-        A.counts[k] = npts;
-        A.captured[k] = captured;
+        A.counts[kk] = npts;
+        A.captured[kk] = captured;
     }
 }
 
@@ -386,7 +400,7 @@ hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStre
 
 hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s)
 {
-    const long n = long(A.cam.nx) * A.cam.ny;
+    const long n = A.k1 - A.k0;
     const int block = 64;
     const int grid = int((n + block - 1) / block);
     if (A.cam.mode == HADROS_MODE_CFG) {

@@ -29,8 +29,10 @@ EXPORTS = [
     "hadros_sigma_check", "hadros_trace_rt", "hadros_tracer_create", "hadros_tracer_set_block",
     "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_fetch", "hadros_tracer_destroy",
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
-    "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_eval_rho",
+    "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
+    "hadros_batch_create", "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run",
+    "hadros_batch_last_ms", "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
 ]
 
 
@@ -85,6 +87,17 @@ def lib():
     L.hadros_replay_fetch.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
     L.hadros_replay_destroy.argtypes = [C.c_void_p]
     L.hadros_trace_cache.argtypes = [C.POINTER(abi.Camera), _lp, _ip, _dp, C.c_int64]
+    L.hadros_trace_cache_range.argtypes = [C.POINTER(abi.Camera), C.c_int64, C.c_int64, _lp, _ip, _dp, C.c_int64]
+    L.hadros_batch_create.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), C.c_int, _dp, _dp, C.c_int,
+                                      C.c_int, C.POINTER(C.c_void_p)]
+    L.hadros_batch_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.hadros_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.hadros_batch_run.argtypes = [C.c_void_p, C.c_void_p]
+    L.hadros_batch_last_ms.argtypes = [C.c_void_p, _dp]
+    L.hadros_batch_fetch.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.ImageOut)]
+    L.hadros_batch_stats.argtypes = [C.c_void_p, C.POINTER(abi.Stats)]
+    L.hadros_batch_info.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.hadros_batch_destroy.argtypes = [C.c_void_p]
     L.hadros_eval_rho.argtypes = [C.POINTER(abi.Medium), _dp, _dp, _dp, C.c_int64]
     L.hadros_eval_sigma.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int64]
     L.hadros_eval_emissivity.argtypes = [C.POINTER(abi.Scene), _dp, _dp, _dp, _dp, C.c_int64]
@@ -319,6 +332,66 @@ def trace_cache(cam):
     beta = (2.0 * (j + 0.5) / cam.ny - 1.0) * th
     return Cache(cam.a_spin, off, pts[:, 0], pts[:, 1], pts[:, 5], pts[:, 6], captured=cap, pixel_i=i, pixel_j=j,
                  alpha=alpha, beta=beta, x=pts[:, 2], y=pts[:, 3], z=pts[:, 4], nx=cam.nx, ny=cam.ny)
+
+
+class Batch:
+    """Persistent multi-camera tracer (hadros_batch_*): one launch for all cameras."""
+
+    def __init__(self, cams, scenes, sig_e, sig_s, precision=abi.PREC_FP64):
+        L = lib()
+        self._se, self._ss = _sig(sig_e, sig_s)
+        n = len(cams)
+        self.cams = (abi.Camera * n)(*cams)
+        self.scenes = (abi.Scene * n)(*scenes)
+        h = C.c_void_p()
+        check(L.hadros_batch_create(self.cams, self.scenes, n, dptr(self._se), dptr(self._ss), self._se.size,
+                                    precision, C.byref(h)))
+        self.h = h
+        self.n = n
+
+    def set_tuning(self, block, steps_per_chunk):
+        check(lib().hadros_batch_set_tuning(self.h, block, steps_per_chunk))
+
+    def autotune(self):
+        b, s = C.c_int(), C.c_int()
+        check(lib().hadros_batch_autotune(self.h, C.byref(b), C.byref(s)))
+        return b.value, s.value
+
+    def info(self):
+        n, b, s, g = C.c_int64(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().hadros_batch_info(self.h, C.byref(n), C.byref(b), C.byref(s), C.byref(g)))
+        return {"n_rays": n.value, "block": b.value, "steps_per_chunk": s.value, "grid": g.value}
+
+    def run(self, stream=None):
+        check(lib().hadros_batch_run(self.h, stream))
+
+    def last_ms(self):
+        ms = C.c_double()
+        check(lib().hadros_batch_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def stats(self):
+        st = abi.Stats()
+        check(lib().hadros_batch_stats(self.h, C.byref(st)))
+        return st
+
+    def fetch(self, cam, nbins):
+        c = self.cams[cam]
+        img = Image(c.nx * c.ny, nbins)
+        o = img.out()
+        check(lib().hadros_batch_fetch(self.h, cam, C.byref(o)))
+        return img
+
+    def close(self):
+        if self.h:
+            lib().hadros_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def eval_rho(medium, r, th):

@@ -74,6 +74,53 @@ def c2_scene():
                      e_min=1.0e5, e_max=1.0e12, n_bins=8)
 
 
+def c3_manifest(seed=0, n_cameras=256):
+    """The batched workload: n cameras drawn from numpy default_rng(seed).
+
+    resolution uniform over {64, 128, 256, 512}^2, spin U[0, 0.998],
+    inclination U[5, 85] deg, profile uniform over the 8 morphologies,
+    log10 rho0 U[8, 12], source uniform over the 5 models, spectral model
+    uniform over the 3, 8 bins on 1e5-1e12 GeV.  Draw order per camera:
+    res, spin, inclination, profile, log rho0, source, spectrum.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(n_cameras):
+        res = int(rng.choice([64, 128, 256, 512]))
+        a = float(rng.uniform(0.0, 0.998))
+        inc = float(rng.uniform(5.0, 85.0))
+        prof = abi.PROFILE_NAMES[int(rng.integers(0, 8))]
+        lrho = float(rng.uniform(8.0, 12.0))
+        src = abi.SOURCE_NAMES[int(rng.integers(0, 5))]
+        spec = abi.SPECTRUM_NAMES[int(rng.integers(0, 3))]
+        out.append({"camera": c, "res": res, "spin": a, "inclination_deg": inc, "profile": prof,
+                    "log10_rho0": lrho, "source": src, "spectrum": spec})
+    return out
+
+
+def manifest_camera(m, mode=abi.MODE_CFG, res=None):
+    r = res or m["res"]
+    return abi.camera(a_spin=m["spin"], r_obs=1000.0, theta_obs_deg=m["inclination_deg"],
+                      fov_deg=fov_for_half_width(20.0, 1000.0), nx=r, ny=r, r_max=1100.0, h=1.0, mode=mode)
+
+
+def manifest_scene(m, slope=2.0):
+    """Scene of one manifest entry: the c2_scene medium conventions (funnel
+    depletion 1 - 1e-4 at 20 deg, envelope 1e-2 rho0, r in [3, 50], floor 1e-3),
+    the c1 inner_ring / c2 funnel_wall source widths, reference defaults for
+    the other sources, gamma 2 and Ecut 1e12 for the spectra."""
+    rho0 = 10.0 ** m["log10_rho0"]
+    src = m["source"]
+    kw = dict(source_r=3.5, source_sigma=1.0, source_theta_deg=15.0)
+    if src == "inner_ring":
+        kw = dict(source_r=6.0, source_sigma=1.0, source_theta_deg=math.degrees(0.1))
+    return abi.scene(enu=1.0e9, m_bh=3.0, rho0=rho0, r0=10.0, sigma_r=3.0, h_over_r=0.3, profile=m["profile"],
+                     radial_power=slope, funnel_depletion=1.0 - 1.0e-4, funnel_theta_deg=20.0,
+                     envelope_rho0=1.0e-2 * rho0, envelope_alpha=2.5, r_min=3.0, r_max=50.0, rho_floor=1.0e-3,
+                     source=src, source_funnel_theta_deg=20.0, spectrum=m["spectrum"], gamma=2.0, ecut=1.0e12,
+                     e_min=1.0e5, e_max=1.0e12, n_bins=8, **kw)
+
+
 WORKLOADS = {
     "c1": (c1_camera, c1_scene),
     "c2": (c2_camera, c2_scene),

@@ -306,6 +306,37 @@ int hadros_replay_destroy(hadros_replay* p);
    offset[nx*ny]*7 doubles.  captured[k] as in BinaryRayHeader. */
 int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captured,
                        double* points, int64_t points_cap);
+/* Same for pixels [k0, k1) only (offset has k1-k0+1 entries, relative to
+   the first pixel of the range), so that large images stream in batches
+   as the reference's per-ray file writes do. */
+int hadros_trace_cache_range(const hadros_camera* cam, int64_t k0, int64_t k1, int64_t* offset,
+                             int32_t* captured, double* points, int64_t points_cap);
+
+/* Batched tracer: many cameras (each with its own spin, inclination,
+   resolution, medium, source and spectrum; one integrator mode and one sigma
+   table for all) traced by one persistent kernel that pulls rays from a
+   global queue and refills finished lanes by wavefront ballot.  Each image
+   equals the plain hadros_trace_rt of its camera bit for bit.  There is no
+   single reference interface: the reference runs one camera per process
+   (apps/compute_kerr_image_cuda.cu:2418-2483 tiles one image; the parameter
+   scans of scripts/run_robustness_scans.py:103-165 launch one process per
+   camera). */
+typedef struct hadros_batch hadros_batch;
+int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, int n_cams,
+                        const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                        int precision, hadros_batch** out);
+/* workgroup size (64..256, multiple of 64) and RKF45 attempts between refills */
+int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk);
+/* times every (block in {64,128,256}) x (steps_per_chunk in {1,4,16,64}) on
+   the first eighth of the rays and keeps the fastest pair */
+int hadros_batch_autotune(hadros_batch* b, int* block, int* steps_per_chunk);
+int hadros_batch_run(hadros_batch* b, void* stream);
+int hadros_batch_last_ms(hadros_batch* b, double* ms);
+/* outputs of camera `cam` (pixel k = i*ny + j of that camera) */
+int hadros_batch_fetch(hadros_batch* b, int cam, hadros_image_out* out);
+int hadros_batch_stats(hadros_batch* b, hadros_stats* stats);
+int hadros_batch_info(hadros_batch* b, int64_t* n_rays, int* block, int* steps_per_chunk, int* grid);
+int hadros_batch_destroy(hadros_batch* b);
 
 /* ---- point-wise device evaluations (parity hooks for each physics term) -- */
 /* TorusProfile::rho (src/torus_profile.cpp:261-264), n points. */

@@ -84,3 +84,15 @@ def test_no_cpu_fallback_without_gpu():
     with pytest.raises(ffi.HadrosError) as ei:
         ffi.trace_rt(workloads.c1_camera(4, 4), workloads.c1_scene(), e, s)
     assert ei.value.code == abi.ERR_DEVICE
+
+
+def test_c3_manifest_is_the_seeded_draw():
+    import json
+
+    from hadros_amd import workloads
+
+    committed = json.load(open(os.path.join(ROOT, "hadros_amd", "manifests", "c3_cameras.json")))
+    assert committed == workloads.c3_manifest()
+    assert len(committed) == 256
+    res = np.array([m["res"] for m in committed])
+    assert set(res) <= {64, 128, 256, 512}
