@@ -62,6 +62,34 @@ def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0):
                       f"FP64 oracle (hadros_oracle, CFG mode), {dt:.1f} s"}
 
 
+def batched_leg(e, s):
+    """The 256-camera batch (manifest hadros_amd/manifests/c3_cameras.json): one
+    persistent launch, autotuned workgroup size and steps per refill."""
+    import numpy as np
+
+    from hadros_amd import abi, ffi, workloads
+
+    man = workloads.c3_manifest()
+    cams = [workloads.manifest_camera(m) for m in man]
+    scenes = [workloads.manifest_scene(m) for m in man]
+    b = ffi.Batch(cams, scenes, e, s, abi.PREC_FP64)
+    blk, spc = b.autotune()
+    b.run()
+    b.last_ms()
+    ms = []
+    for _ in range(2):
+        b.run()
+        ms.append(b.last_ms())
+    st = b.stats()
+    info = b.info()
+    b.close()
+    best = float(np.min(ms))
+    return {"value": info["n_rays"] / (best * 1e-3), "unit": "rays/s", "images_per_s": len(cams) / (best * 1e-3),
+            "cameras": len(cams), "rays": info["n_rays"], "kernel_ms": best, "block": blk, "steps_per_chunk": spc,
+            "grid_blocks": info["grid"], "points": st.total_points, "attempts": st.total_attempts,
+            "algorithmic_tflops": tracer_flops(st.total_attempts, st.total_points, 8) / (best * 1e-3) / 1e12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,7 +98,11 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-camera leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--pmc-target", action="store_true",
+                    help="for rocprofv3 --pmc passes: run the plain tracer twice, then copy a 256 MiB device "
+                         "buffer twice (calibration of FETCH_SIZE / WRITE_SIZE), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the per-ray work has no "
@@ -88,6 +120,18 @@ def main():
     cam = workloads.c2_camera(nx=args.size, ny=args.size)
     sc = workloads.c2_scene()
     npix = cam.nx * cam.ny
+    if args.pmc_target:
+        t = ffi.Tracer(cam, sc, e, s, abi.PREC_FP64, args.block)
+        for _ in range(2):
+            t.run()
+            t.last_ms()
+        t.close()
+        x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        y = torch.ones_like(x)
+        for _ in range(2):
+            x.copy_(y)
+        torch.cuda.synchronize()
+        return
 
     def timed(precision):
         t = ffi.Tracer(cam, sc, e, s, precision, args.block)
@@ -146,6 +190,14 @@ def main():
         "image": {"captured": st.n_captured, "exhausted": st.n_exhausted, "mean_P": float(img.p.mean()),
                   "sum_I": si, "points_per_ray": st.total_points / npix},
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg(e, s)
+    tr = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tr):
+        t = json.load(open(tr))
+        if t.get("workload") == out["config"]["workload"]:
+            out["roofline"]["traffic"] = t["bytes_per_launch"]
+            out["roofline"]["traffic_source"] = t["source"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds)
     print(json.dumps(out), flush=True)
