@@ -285,6 +285,10 @@ void dfree(T*& p)
 
 }  // namespace
 
+// error reporting and device check shared with the other host units
+int hd::set_error(int code, const char* msg) { return fail(code, msg); }
+int hd::device_ready() { return ensure_device(); }
+
 // ================================================================ plans
 struct hadros_tracer {
     TraceArgs args{};

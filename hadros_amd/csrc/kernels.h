@@ -71,6 +71,11 @@ __device__ __forceinline__ void wave_add4(unsigned long long* c, unsigned long l
     }
 }
 
+// host helpers of hadros_abi.hip: set the thread's last error (returns code),
+// check for a usable gfx950 device (HADROS_OK or HADROS_ERR_DEVICE)
+int set_error(int code, const char* msg);
+int device_ready();
+
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStream_t s);
 hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s);

@@ -33,6 +33,8 @@ EXPORTS = [
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
     "hadros_batch_create", "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run",
     "hadros_batch_last_ms", "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
+    "hadros_mlp_create", "hadros_mlp_n_params", "hadros_mlp_get_params", "hadros_mlp_set_params",
+    "hadros_mlp_set_norm", "hadros_mlp_forward", "hadros_mlp_grad", "hadros_mlp_train_step", "hadros_mlp_destroy",
 ]
 
 
@@ -98,6 +100,17 @@ def lib():
     L.hadros_batch_stats.argtypes = [C.c_void_p, C.POINTER(abi.Stats)]
     L.hadros_batch_info.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hadros_batch_destroy.argtypes = [C.c_void_p]
+    _fp = C.POINTER(C.c_float)
+    L.hadros_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                    C.POINTER(C.c_void_p)]
+    L.hadros_mlp_n_params.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.hadros_mlp_get_params.argtypes = [C.c_void_p, _fp, C.c_int64]
+    L.hadros_mlp_set_params.argtypes = [C.c_void_p, _fp, C.c_int64]
+    L.hadros_mlp_set_norm.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp]
+    L.hadros_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.hadros_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, _fp, _fp]
+    L.hadros_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, _fp]
+    L.hadros_mlp_destroy.argtypes = [C.c_void_p]
     L.hadros_eval_rho.argtypes = [C.POINTER(abi.Medium), _dp, _dp, _dp, C.c_int64]
     L.hadros_eval_sigma.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int64]
     L.hadros_eval_emissivity.argtypes = [C.POINTER(abi.Scene), _dp, _dp, _dp, _dp, C.c_int64]
@@ -385,6 +398,73 @@ class Batch:
     def close(self):
         if self.h:
             lib().hadros_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """The BF16-MFMA surrogate network (hadros_mlp_*).  Data arguments are
+    torch CUDA float32 tensors (device memory plumbing only)."""
+
+    def __init__(self, n_in=14, width=256, depth=4, n_out=16, train_batch=8192, infer_batch=1 << 20, seed=0):
+        L = lib()
+        h = C.c_void_p()
+        check(L.hadros_mlp_create(n_in, width, depth, n_out, train_batch, infer_batch, seed, C.byref(h)))
+        self.h = h
+        self.n_in, self.width, self.depth, self.n_out = n_in, width, depth, n_out
+        n = C.c_int64()
+        check(L.hadros_mlp_n_params(self.h, C.byref(n)))
+        self.n_params = n.value
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        check(lib().hadros_mlp_get_params(self.h, fptr(p), p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib().hadros_mlp_set_params(self.h, fptr(p), p.size))
+
+    def set_norm(self, x_mean, x_std, y_mean, y_std):
+        a = [np.ascontiguousarray(v, np.float32) for v in (x_mean, x_std, y_mean, y_std)]
+        self._norm = a
+        check(lib().hadros_mlp_set_norm(self.h, *(fptr(v) for v in a)))
+
+    def forward(self, x, y=None, stream=None):
+        import torch
+
+        assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous() and x.shape[1] == self.n_in
+        if y is None:
+            y = torch.empty((x.shape[0], self.n_out), dtype=torch.float32, device=x.device)
+        check(lib().hadros_mlp_forward(self.h, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(y.data_ptr()),
+                                       stream))
+        return y
+
+    def grad(self, x, t):
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        check(lib().hadros_mlp_grad(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
+                                    fptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, x, t, lr):
+        loss = C.c_float()
+        check(lib().hadros_mlp_train_step(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
+                                          lr, C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().hadros_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):

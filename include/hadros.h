@@ -338,6 +338,36 @@ int hadros_batch_stats(hadros_batch* b, hadros_stats* stats);
 int hadros_batch_info(hadros_batch* b, int64_t* n_rays, int* block, int* steps_per_chunk, int* grid);
 int hadros_batch_destroy(hadros_batch* b);
 
+/* ---- learned surrogate --------------------------------------------------
+   A multilayer perceptron n_in -> width x depth (SiLU) -> n_out with BF16
+   weights and activations on MFMA and FP32 accumulation, trained with Adam
+   by the library's own forward/backward kernels.  It maps per-ray features
+   (alpha, beta, spin, cos inclination, profile one-hot, log10 rho0, slope)
+   to log10 tau and survival per energy bin.  No reference counterpart: the
+   reference has no learned component; its parameter scans re-run the tracer
+   for every point (scripts/run_robustness_scans.py:103-165).
+   x, y are DEVICE pointers to row-major FP32 [n x n_in] / [n x n_out]. */
+typedef struct hadros_mlp hadros_mlp;
+/* n_in <= 16, width a multiple of 16; train_batch / infer_batch bound the
+   rows of one training step / one inference chunk.  He-uniform init from seed. */
+int hadros_mlp_create(int n_in, int width, int depth, int n_out, int train_batch,
+                      int infer_batch, uint64_t seed, hadros_mlp** out);
+int hadros_mlp_n_params(hadros_mlp* m, int64_t* n);
+/* parameters as FP32 in the order W1 (n_in x width, row-major), b1, ..., W_L, b_L */
+int hadros_mlp_get_params(hadros_mlp* m, float* host, int64_t n);
+int hadros_mlp_set_params(hadros_mlp* m, const float* host, int64_t n);
+/* feature and target standardisation (host arrays of n_in / n_out) */
+int hadros_mlp_set_norm(hadros_mlp* m, const float* x_mean, const float* x_std,
+                        const float* y_mean, const float* y_std);
+/* y = net(x) for any n (processed in infer_batch chunks), asynchronous on stream */
+int hadros_mlp_forward(hadros_mlp* m, const float* x, int64_t n, float* y, void* stream);
+/* gradient of the mean squared error on standardised targets w.r.t. every
+   parameter (host FP32, get_params order) and the loss, without updating */
+int hadros_mlp_grad(hadros_mlp* m, const float* x, const float* y, int64_t n, float* grad, float* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on n <= train_batch rows */
+int hadros_mlp_train_step(hadros_mlp* m, const float* x, const float* y, int64_t n, float lr, float* loss);
+int hadros_mlp_destroy(hadros_mlp* m);
+
 /* ---- point-wise device evaluations (parity hooks for each physics term) -- */
 /* TorusProfile::rho (src/torus_profile.cpp:261-264), n points. */
 int hadros_eval_rho(const hadros_medium* m, const double* r, const double* th,
