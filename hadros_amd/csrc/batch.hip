@@ -44,8 +44,9 @@ __device__ __forceinline__ void batch_init_ray(const BatchArgs& A, const long* _
     L.ray = ray;
     L.cam = lo;
     const DevCam& C = A.cams[lo];
-    const long k = ray - pix0[lo];
-    const int i = int(k / C.ny), j = int(k % C.ny);
+    const long k = ray - pix0[lo];   // index in the camera's (sub-)lattice
+    const int i = C.sub_offset + C.sub_stride * int(k / C.sub_ny);
+    const int j = C.sub_offset + C.sub_stride * int(k % C.sub_ny);
     // src/kerr_camera.cpp:110-186 with the observer metric from the host
     const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
     const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;

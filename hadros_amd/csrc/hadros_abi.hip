@@ -264,6 +264,10 @@ int make_cam(const hadros_camera& c, DevCam& d)
     d.g_thth = g[3];
     d.g_phph = g[4];
     d.r_h = 1.0 + std::sqrt(1.0 - c.a_spin * c.a_spin);
+    d.sub_stride = 1;
+    d.sub_offset = 0;
+    d.sub_nx = c.nx;
+    d.sub_ny = c.ny;
     return HADROS_OK;
 }
 
@@ -840,10 +844,12 @@ int hadros_rt_from_cache(const hadros_cache_view* c, const hadros_scene* scene, 
 
 
 // ---------------------------------------------------------------- batched tracer
-int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, int n_cams, const double* se,
-                        const double* ss, int n_sigma, int precision, hadros_batch** out)
+int hadros_batch_create_sub(const hadros_camera* cams, const hadros_scene* scenes, int n_cams, const double* se,
+                            const double* ss, int n_sigma, int precision, int stride, int offset,
+                            hadros_batch** out)
 {
     if (!cams || !scenes || !out || n_cams <= 0) return fail(HADROS_ERR_ARG, "bad batch arguments");
+    if (stride < 1 || offset < 0) return fail(HADROS_ERR_ARG, "sub-lattice stride must be >= 1 and offset >= 0");
     if (precision != HADROS_PREC_FP64 && precision != HADROS_PREC_FP32K) return fail(HADROS_ERR_ARG, "unknown precision");
     for (int c = 1; c < n_cams; ++c)
         if (cams[c].mode != cams[0].mode) return fail(HADROS_ERR_ARG, "all cameras of a batch must share one integrator mode");
@@ -860,9 +866,13 @@ int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, i
             return rc;
         }
         dc[c].pad_rt = c;
+        dc[c].sub_stride = stride;
+        dc[c].sub_offset = offset;
+        dc[c].sub_nx = cams[c].nx > offset ? (cams[c].nx - offset + stride - 1) / stride : 0;
+        dc[c].sub_ny = cams[c].ny > offset ? (cams[c].ny - offset + stride - 1) / stride : 0;
         b->nbc[c] = dr[c].nb;
         b->nb_max = std::max(b->nb_max, dr[c].nb);
-        b->pix0[c + 1] = b->pix0[c] + long(cams[c].nx) * cams[c].ny;
+        b->pix0[c + 1] = b->pix0[c] + long(dc[c].sub_nx) * dc[c].sub_ny;
     }
     b->mode = cams[0].mode;
     b->precision = precision;
@@ -907,6 +917,31 @@ int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, i
     A.out = TraceOut{b->d_tau, b->d_p, b->d_I, nullptr, nullptr, b->d_bt, b->d_bp, b->d_bi,
                      b->d_cap, b->d_np, b->d_flags, b->d_cnt};
     *out = b;
+    return HADROS_OK;
+}
+
+int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, int n_cams, const double* se,
+                        const double* ss, int n_sigma, int precision, hadros_batch** out)
+{
+    return hadros_batch_create_sub(cams, scenes, n_cams, se, ss, n_sigma, precision, 1, 0, out);
+}
+
+int hadros_batch_fetch_all(hadros_batch* b, hadros_image_out* o)
+{
+    if (!b || !b->ran || !o) return fail(HADROS_ERR_ARG, "batch has not run");
+    HIPC(hipEventSynchronize(b->ev1));
+    const size_t n = size_t(b->n_rays), nbo = size_t(b->nb_max);
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst && src && bytes) HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return HADROS_OK;
+    };
+    int rc = 0;
+    if ((rc = cp(o->tau, b->d_tau, n * 8)) || (rc = cp(o->p_surv, b->d_p, n * 8)) ||
+        (rc = cp(o->intensity, b->d_I, n * 8)) || (rc = cp(o->captured, b->d_cap, n * 4)) ||
+        (rc = cp(o->npoints, b->d_np, n * 4)) || (rc = cp(o->flags, b->d_flags, n * 4)) ||
+        (rc = cp(o->bin_tau, b->d_bt, n * nbo * 8)) || (rc = cp(o->bin_p, b->d_bp, n * nbo * 8)) ||
+        (rc = cp(o->bin_i, b->d_bi, n * nbo * 8)))
+        return rc;
     return HADROS_OK;
 }
 
@@ -1020,7 +1055,8 @@ int hadros_batch_fetch(hadros_batch* b, int cam, hadros_image_out* o)
         DevCam dc;
         HIPC(hipMemcpy(&dc, b->d_cams + cam, sizeof(DevCam), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < n; ++k) {
-            const int i = int(k / size_t(dc.ny)), j = int(k % size_t(dc.ny));
+            const int i = dc.sub_offset + dc.sub_stride * int(k / size_t(dc.sub_ny));
+            const int j = dc.sub_offset + dc.sub_stride * int(k % size_t(dc.sub_ny));
             if (o->alpha) o->alpha[k] = (2.0 * (i + 0.5) / dc.nx - 1.0) * dc.tanhalf;
             if (o->beta) o->beta[k] = (2.0 * (j + 0.5) / dc.ny - 1.0) * dc.tanhalf;
         }

@@ -101,7 +101,8 @@ struct DevRt {
 struct DevCam {
     double a, r_obs, th_obs, tanhalf, r_max, h, tol;
     int nx, ny, mode, max_steps;
-    int pad_rt, reserved;   // scene index of this camera in a batch
+    int pad_rt;             // scene index of this camera in a batch
+    int sub_stride, sub_offset, sub_nx, sub_ny;   // traced pixels i = offset + stride p (batch only)
     // observer constants of KerrCamera::initial_state (src/kerr_camera.cpp:110-186)
     double g_tt, g_tph, g_rr, g_thth, g_phph, alpha, omega;
     double r_h;

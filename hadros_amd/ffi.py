@@ -31,7 +31,7 @@ EXPORTS = [
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
-    "hadros_batch_create", "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run",
+    "hadros_batch_create", "hadros_batch_create_sub", "hadros_batch_fetch_all", "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run",
     "hadros_batch_last_ms", "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
     "hadros_mlp_create", "hadros_mlp_n_params", "hadros_mlp_get_params", "hadros_mlp_set_params",
     "hadros_mlp_set_norm", "hadros_mlp_forward", "hadros_mlp_grad", "hadros_mlp_train_step", "hadros_mlp_destroy",
@@ -92,6 +92,9 @@ def lib():
     L.hadros_trace_cache_range.argtypes = [C.POINTER(abi.Camera), C.c_int64, C.c_int64, _lp, _ip, _dp, C.c_int64]
     L.hadros_batch_create.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), C.c_int, _dp, _dp, C.c_int,
                                       C.c_int, C.POINTER(C.c_void_p)]
+    L.hadros_batch_create_sub.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), C.c_int, _dp, _dp, C.c_int,
+                                          C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.hadros_batch_fetch_all.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut)]
     L.hadros_batch_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.hadros_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hadros_batch_run.argtypes = [C.c_void_p, C.c_void_p]
@@ -350,17 +353,26 @@ def trace_cache(cam):
 class Batch:
     """Persistent multi-camera tracer (hadros_batch_*): one launch for all cameras."""
 
-    def __init__(self, cams, scenes, sig_e, sig_s, precision=abi.PREC_FP64):
+    def __init__(self, cams, scenes, sig_e, sig_s, precision=abi.PREC_FP64, stride=1, offset=0):
         L = lib()
         self._se, self._ss = _sig(sig_e, sig_s)
         n = len(cams)
         self.cams = (abi.Camera * n)(*cams)
         self.scenes = (abi.Scene * n)(*scenes)
         h = C.c_void_p()
-        check(L.hadros_batch_create(self.cams, self.scenes, n, dptr(self._se), dptr(self._ss), self._se.size,
-                                    precision, C.byref(h)))
+        check(L.hadros_batch_create_sub(self.cams, self.scenes, n, dptr(self._se), dptr(self._ss), self._se.size,
+                                        precision, stride, offset, C.byref(h)))
         self.h = h
         self.n = n
+        self.stride, self.offset = stride, offset
+
+    def fetch_all(self, nbins):
+        n = self.info()["n_rays"]
+        img = Image(n, nbins)
+        o = img.out()
+        o.alpha = o.beta = None
+        check(lib().hadros_batch_fetch_all(self.h, C.byref(o)))
+        return img
 
     def set_tuning(self, block, steps_per_chunk):
         check(lib().hadros_batch_set_tuning(self.h, block, steps_per_chunk))
@@ -390,7 +402,8 @@ class Batch:
 
     def fetch(self, cam, nbins):
         c = self.cams[cam]
-        img = Image(c.nx * c.ny, nbins)
+        sub = lambda n: max(0, (n - self.offset + self.stride - 1) // self.stride)  # noqa: E731
+        img = Image(sub(c.nx) * sub(c.ny), nbins)
         o = img.out()
         check(lib().hadros_batch_fetch(self.h, cam, C.byref(o)))
         return img

@@ -325,6 +325,15 @@ typedef struct hadros_batch hadros_batch;
 int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, int n_cams,
                         const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
                         int precision, hadros_batch** out);
+/* Same, tracing only the pixel sub-lattice i, j = offset + stride * (0, 1, ...)
+   of every camera (e.g. stride 10, offset 5: 1 % of a 256x256 image, for
+   spot checks); outputs are indexed by the sub-lattice pixel p * sub_ny + q. */
+int hadros_batch_create_sub(const hadros_camera* cams, const hadros_scene* scenes, int n_cams,
+                            const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                            int precision, int stride, int offset, hadros_batch** out);
+/* all rays' outputs in global ray order; bins are [ray x n_bins_max] with
+   n_bins_max the largest bin count of the batch's scenes */
+int hadros_batch_fetch_all(hadros_batch* b, hadros_image_out* out);
 /* workgroup size (64..256, multiple of 64) and RKF45 attempts between refills */
 int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk);
 /* times every (block in {64,128,256}) x (steps_per_chunk in {1,4,16,64}) on
