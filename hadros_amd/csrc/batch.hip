@@ -83,14 +83,19 @@ k_trace_batch(BatchArgs A)
     const float* lnEf = reinterpret_cast<const float*>(smem);
     const float* lnSf = lnEf + n;
     const float* slopef = lnEf + 2 * n;
-    long* pix0 = reinterpret_cast<long*>(smem + 3 * n);   // [n_cams + 1]
+    // camera start table: in LDS for up to BATCH_LDS_CAMS cameras, else read
+    // through the cache hierarchy (one binary search per ray refill)
+    long* pix0_lds = reinterpret_cast<long*>(smem + 3 * n);   // [n_cams + 1]
+    const bool pix_in_lds = A.n_cams <= BATCH_LDS_CAMS;
+    const long* pix0 = pix_in_lds ? pix0_lds : A.pix0;
     if constexpr (PREC == HADROS_PREC_FP64) {
         for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) smem[t] = A.sig[t];
     } else {
         float* sf = reinterpret_cast<float*>(smem);
         for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) sf[t] = A.sigf[t];
     }
-    for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0[t] = A.pix0[t];
+    if (pix_in_lds)
+        for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0_lds[t] = A.pix0[t];
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -301,8 +306,8 @@ static void launch_b(const BatchArgs& A, int nb, int grid, int block, size_t lds
 
 size_t batch_lds_bytes(int n_sigma, int n_cams, int precision)
 {
-    return size_t(3) * n_sigma * sizeof(double) + size_t(n_cams + 1) * sizeof(long) +
-           (precision == HADROS_PREC_FP64 ? 0 : 0);
+    (void)precision;   // the FP32 table is stored in the first half of the FP64 one's space
+    return size_t(3) * n_sigma * sizeof(double) + (n_cams <= BATCH_LDS_CAMS ? size_t(n_cams + 1) * sizeof(long) : 0);
 }
 
 hipError_t batch_resident_blocks(int mode, int precision, int nb, int block, int n_sigma, int n_cams, int* blocks_per_cu)

@@ -945,6 +945,26 @@ int hadros_batch_fetch_all(hadros_batch* b, hadros_image_out* o)
     return HADROS_OK;
 }
 
+int hadros_batch_fetch_all_device(hadros_batch* b, const hadros_image_out* o, void* stream)
+{
+    if (!b || !b->ran || !o) return fail(HADROS_ERR_ARG, "batch has not run");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    HIPC(hipStreamWaitEvent(s, b->ev1, 0));
+    const size_t n = size_t(b->n_rays), nbo = size_t(b->nb_max);
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst && src && bytes) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+        return HADROS_OK;
+    };
+    int rc = 0;
+    if ((rc = cp(o->tau, b->d_tau, n * 8)) || (rc = cp(o->p_surv, b->d_p, n * 8)) ||
+        (rc = cp(o->intensity, b->d_I, n * 8)) || (rc = cp(o->captured, b->d_cap, n * 4)) ||
+        (rc = cp(o->npoints, b->d_np, n * 4)) || (rc = cp(o->flags, b->d_flags, n * 4)) ||
+        (rc = cp(o->bin_tau, b->d_bt, n * nbo * 8)) || (rc = cp(o->bin_p, b->d_bp, n * nbo * 8)) ||
+        (rc = cp(o->bin_i, b->d_bi, n * nbo * 8)))
+        return rc;
+    return HADROS_OK;
+}
+
 int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk)
 {
     if (!b) return fail(HADROS_ERR_ARG, "NULL batch");

@@ -40,6 +40,9 @@ struct ReplayArgs {
     unsigned long long* counters;
 };
 
+// batches of up to this many cameras keep their camera start table in LDS
+constexpr int BATCH_LDS_CAMS = 1024;
+
 struct BatchArgs {
     const DevCam* cams;   // [n_cams], cams[c].pad_rt indexes rts
     const long* pix0;     // [n_cams+1] first global ray of each camera

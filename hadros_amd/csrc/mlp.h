@@ -20,6 +20,26 @@ struct GemmArgs {
     __bf16* Cb;             // BF16 output (activation / back-propagated gradient)
 };
 
+// fused inference (mlp_infer.hip): hidden width 256, n_in <= 16, n_out <= 16
+constexpr int FUSED_MAX_LAYERS = 8;
+struct FusedArgs {
+    const float* x;          // [M x n_in] raw features
+    long M;
+    int n_in, L, n_out, pad;
+    const float* xm;         // feature mean
+    const float* xi;         // feature 1 / std
+    const void* w;           // swizzled bf16 weights (16-byte fragments)
+    long wo[FUSED_MAX_LAYERS];   // per layer: first fragment
+    long bo[FUSED_MAX_LAYERS];   // per layer: bias offset into `bias`
+    const float* bias;
+    const float* ys;         // output scale (target std)
+    const float* ym;         // output shift (target mean)
+    float* y;                // [M x n_out] outputs, or [M x n_out/2] survivals
+};
+
+hipError_t mlp_swizzle(const __bf16* w, int K, int N, int KS, int NT, __bf16* out, hipStream_t s);
+hipError_t mlp_fused(const FusedArgs& a, int mode, hipStream_t s);
+
 hipError_t mlp_gemm(const GemmArgs& g, bool ta, bool tb, int epi, int splits, hipStream_t s);
 hipError_t mlp_sum_parts(const float* p, int parts, long n, float* out, hipStream_t s);
 hipError_t mlp_colsum(const __bf16* x, int M, int N, int rows_per_chunk, float* part, hipStream_t s);

@@ -81,6 +81,14 @@ struct hadros_mlp {
     float* lpart = nullptr;    // loss partials
     // inference buffers (ib rows)
     __bf16 *Xi = nullptr, *Ha = nullptr, *Hb = nullptr;
+    // fused inference: weights in MFMA fragment order, refreshed after every
+    // parameter change (sw_dirty)
+    bool fused_ok = false;
+    int infer_path = 0;        // 0: fused when the shape allows, 1: per-layer GEMMs
+    __bf16* Wsw = nullptr;
+    std::vector<long> swo;     // per layer: first 16-byte fragment
+    size_t n_sw = 0;           // bf16 elements
+    bool sw_dirty = true;
 };
 
 namespace {
@@ -103,6 +111,50 @@ void layout(hadros_mlp* m)
         o += size_t(m->out[l]);
     }
     m->n_int = o;
+    // swizzled copies: K padded to 32, N to 16
+    m->fused_ok = m->width == 256 && m->n_out <= 16 && m->L <= FUSED_MAX_LAYERS;
+    long f = 0;
+    for (int l = 0; l < m->L; ++l) {
+        m->swo.push_back(f);
+        f += long((m->in[l] + 31) / 32) * ((m->out[l] + 15) / 16) * 64;
+    }
+    m->n_sw = size_t(f) * 8;
+}
+
+int refresh_swizzle(hadros_mlp* m, hipStream_t s)
+{
+    if (!m->sw_dirty) return HADROS_OK;
+    for (int l = 0; l < m->L; ++l)
+        MC(mlp_swizzle(m->W16 + m->woff[l], m->in[l], m->out[l], (m->in[l] + 31) / 32, (m->out[l] + 15) / 16,
+                       m->Wsw + size_t(m->swo[l]) * 8, s));
+    m->sw_dirty = false;
+    return HADROS_OK;
+}
+
+// fused inference of n rows: mode 0 de-standardised outputs, mode 1 survivals
+int infer_fused(hadros_mlp* m, const float* x, int64_t n, float* y, int mode, hipStream_t s)
+{
+    int rc = refresh_swizzle(m, s);
+    if (rc) return rc;
+    FusedArgs a{};
+    a.x = x;
+    a.M = long(n);
+    a.n_in = m->n_in;
+    a.L = m->L;
+    a.n_out = m->n_out;
+    a.xm = m->xm;
+    a.xi = m->xi;
+    a.w = m->Wsw;
+    for (int l = 0; l < m->L; ++l) {
+        a.wo[l] = m->swo[l];
+        a.bo[l] = long(m->boff[l]);
+    }
+    a.bias = m->P;
+    a.ys = m->ys;
+    a.ym = m->ym;
+    a.y = y;
+    MC(mlp_fused(a, mode, s));
+    return HADROS_OK;
 }
 
 size_t n_public(const hadros_mlp* m) { return m->n_int - size_t(PAD_IN - m->n_in) * m->out[0]; }
@@ -248,7 +300,7 @@ int hadros_mlp_create(int n_in, int width, int depth, int n_out, int train_batch
         (rc = dal(&m->Xb, tb * PAD_IN)) || (rc = dal(&m->Y, tb * n_out)) || (rc = dal(&m->dY, tb * n_out)) ||
         (rc = dal(&m->Ga, tb * width)) || (rc = dal(&m->Gb, tb * width)) || (rc = dal(&m->part, m->part_n)) ||
         (rc = dal(&m->lpart, (tb * n_out + 255) / 256)) || (rc = dal(&m->Xi, ib * PAD_IN)) ||
-        (rc = dal(&m->Ha, ib * width)) || (rc = dal(&m->Hb, ib * width))) {
+        (rc = dal(&m->Ha, ib * width)) || (rc = dal(&m->Hb, ib * width)) || (rc = dal(&m->Wsw, m->n_sw))) {
         hadros_mlp_destroy(m);
         return rc;
     }
@@ -279,6 +331,7 @@ int hadros_mlp_create(int n_in, int width, int depth, int n_out, int train_batch
     MC(hipMemcpy(m->ys, ones.data(), size_t(n_out) * 4, hipMemcpyHostToDevice));
     MC(hipMemcpy(m->one, ones.data(), size_t(n_out) * 4, hipMemcpyHostToDevice));
     MC(mlp_to_bf16(m->P, long(m->n_int), m->W16, nullptr));
+    m->sw_dirty = true;
     MC(hipDeviceSynchronize());
     *out = m;
     return HADROS_OK;
@@ -307,6 +360,7 @@ int hadros_mlp_set_params(hadros_mlp* m, const float* host, int64_t n)
     map_public(m, [&](size_t q, size_t i) { p[i] = host[q]; });
     MC(hipMemcpy(m->P, p.data(), m->n_int * 4, hipMemcpyHostToDevice));
     MC(mlp_to_bf16(m->P, long(m->n_int), m->W16, nullptr));
+    m->sw_dirty = true;
     MC(hipDeviceSynchronize());
     return HADROS_OK;
 }
@@ -339,6 +393,7 @@ int hadros_mlp_forward(hadros_mlp* m, const float* x, int64_t n, float* y, void*
 {
     if (!m || (n > 0 && (!x || !y)) || n < 0) return set_error(HADROS_ERR_ARG, "bad forward arguments");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (m->fused_ok && m->infer_path == 0) return infer_fused(m, x, n, y, 0, s);
     for (int64_t r0 = 0; r0 < n; r0 += m->ib) {
         const long rows = long(std::min<int64_t>(m->ib, n - r0));
         MC(mlp_pack(x + r0 * m->n_in, rows, m->n_in, PAD_IN, m->xm, m->xi, m->Xi, s));
@@ -370,6 +425,22 @@ int hadros_mlp_forward(hadros_mlp* m, const float* x, int64_t n, float* y, void*
     return HADROS_OK;
 }
 
+int hadros_mlp_survival(hadros_mlp* m, const float* x, int64_t n, float* p, void* stream)
+{
+    if (!m || (n > 0 && (!x || !p)) || n < 0) return set_error(HADROS_ERR_ARG, "bad survival arguments");
+    if (!m->fused_ok || m->n_out % 2)
+        return set_error(HADROS_ERR_ARG, "survival output needs width 256, an even n_out <= 16 and <= 7 hidden layers");
+    return infer_fused(m, x, n, p, 1, reinterpret_cast<hipStream_t>(stream));
+}
+
+int hadros_mlp_set_infer_path(hadros_mlp* m, int path)
+{
+    if (!m || path < 0 || path > 1) return set_error(HADROS_ERR_ARG, "path must be 0 (fused) or 1 (per-layer GEMMs)");
+    if (path == 0 && !m->fused_ok) return set_error(HADROS_ERR_ARG, "this network shape has no fused path");
+    m->infer_path = path;
+    return HADROS_OK;
+}
+
 int hadros_mlp_grad(hadros_mlp* m, const float* x, const float* y, int64_t n, float* grad, float* loss)
 {
     if (!m) return set_error(HADROS_ERR_ARG, "NULL mlp");
@@ -392,6 +463,7 @@ int hadros_mlp_train_step(hadros_mlp* m, const float* x, const float* y, int64_t
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.0f - std::pow(b1, float(m->step)), c2 = 1.0f - std::pow(b2, float(m->step));
     MC(mlp_adam(m->P, m->G, m->Mo, m->Vo, long(m->n_int), lr, b1, b2, eps, c1, c2, m->W16, nullptr));
+    m->sw_dirty = true;
     return HADROS_OK;
 }
 
@@ -403,6 +475,7 @@ int hadros_mlp_destroy(hadros_mlp* m)
     dfr(m->Mo);
     dfr(m->Vo);
     dfr(m->W16);
+    dfr(m->Wsw);
     dfr(m->xm);
     dfr(m->xi);
     dfr(m->ym);

@@ -31,10 +31,12 @@ EXPORTS = [
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
-    "hadros_batch_create", "hadros_batch_create_sub", "hadros_batch_fetch_all", "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run",
-    "hadros_batch_last_ms", "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
+    "hadros_batch_create", "hadros_batch_create_sub", "hadros_batch_fetch_all", "hadros_batch_fetch_all_device",
+    "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run", "hadros_batch_last_ms",
+    "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
     "hadros_mlp_create", "hadros_mlp_n_params", "hadros_mlp_get_params", "hadros_mlp_set_params",
-    "hadros_mlp_set_norm", "hadros_mlp_forward", "hadros_mlp_grad", "hadros_mlp_train_step", "hadros_mlp_destroy",
+    "hadros_mlp_set_norm", "hadros_mlp_forward", "hadros_mlp_survival", "hadros_mlp_set_infer_path",
+    "hadros_mlp_grad", "hadros_mlp_train_step", "hadros_mlp_destroy",
 ]
 
 
@@ -95,6 +97,7 @@ def lib():
     L.hadros_batch_create_sub.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), C.c_int, _dp, _dp, C.c_int,
                                           C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.hadros_batch_fetch_all.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut)]
+    L.hadros_batch_fetch_all_device.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.c_void_p]
     L.hadros_batch_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.hadros_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hadros_batch_run.argtypes = [C.c_void_p, C.c_void_p]
@@ -111,6 +114,8 @@ def lib():
     L.hadros_mlp_set_params.argtypes = [C.c_void_p, _fp, C.c_int64]
     L.hadros_mlp_set_norm.argtypes = [C.c_void_p, _fp, _fp, _fp, _fp]
     L.hadros_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.hadros_mlp_survival.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+    L.hadros_mlp_set_infer_path.argtypes = [C.c_void_p, C.c_int]
     L.hadros_mlp_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, _fp, _fp]
     L.hadros_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, _fp]
     L.hadros_mlp_destroy.argtypes = [C.c_void_p]
@@ -366,13 +371,41 @@ class Batch:
         self.n = n
         self.stride, self.offset = stride, offset
 
-    def fetch_all(self, nbins):
-        n = self.info()["n_rays"]
-        img = Image(n, nbins)
+    ALL_FIELDS = ("tau", "p_surv", "intensity", "captured", "npoints", "flags", "bin_tau", "bin_p", "bin_i")
+
+    def fetch_all(self, nbins, fields=ALL_FIELDS):
+        """Outputs of every ray of the batch (camera-major); only `fields`
+        (hadros_image_out member names) are copied, the others stay zero."""
+        n, nb = self.info()["n_rays"], self.stats().n_bins
+        if nbins is not None and nbins != nb:
+            raise ValueError(f"batch stores {nb} bins per ray, asked for {nbins}")
+        img = Image(n, nb)
         o = img.out()
         o.alpha = o.beta = None
+        for f in self.ALL_FIELDS:
+            if f not in fields:
+                setattr(o, f, None)
         check(lib().hadros_batch_fetch_all(self.h, C.byref(o)))
         return img
+
+    def fetch_all_device(self, **dst):
+        """Device-to-device copy of whole-batch outputs into caller tensors:
+        keyword = hadros_image_out member name, value = contiguous torch CUDA
+        tensor of the matching size and dtype (float64, int32 for the ints)."""
+        import torch
+
+        n, nb = self.info()["n_rays"], self.stats().n_bins
+        o = abi.ImageOut()
+        for k, t in dst.items():
+            if k not in self.ALL_FIELDS:
+                raise ValueError(f"unknown output field {k}")
+            want = n * nb if k.startswith("bin_") else n
+            dt = torch.int32 if k in ("captured", "npoints", "flags") else torch.float64
+            if not (t.is_cuda and t.is_contiguous() and t.dtype == dt and t.numel() == want):
+                raise ValueError(f"{k}: need a contiguous CUDA {dt} tensor of {want} elements")
+            setattr(o, k, C.cast(C.c_void_p(t.data_ptr()), _ip if dt == torch.int32 else _dp))
+        check(lib().hadros_batch_fetch_all_device(self.h, C.byref(o), None))
+        return dst
 
     def set_tuning(self, block, steps_per_chunk):
         check(lib().hadros_batch_set_tuning(self.h, block, steps_per_chunk))
@@ -400,10 +433,23 @@ class Batch:
         check(lib().hadros_batch_stats(self.h, C.byref(st)))
         return st
 
-    def fetch(self, cam, nbins):
+    def cam_bins(self, cam):
+        """Bins stored per ray for camera `cam` (the library's rule: 1 for a
+        monochromatic scene, 0 when the spectrum's range misses the sigma
+        table, else n_bins)."""
+        sp = self.scenes[cam].spectrum
+        if sp.kind == abi.SPECTRUM_KINDS["mono"]:
+            return 1
+        lo, hi = max(sp.e_min_GeV, self._se[0]), min(sp.e_max_GeV, self._se[-1])
+        return max(sp.n_bins, 1) if (lo > 0.0 and hi > lo) else 0
+
+    def fetch(self, cam, nbins=None):
         c = self.cams[cam]
+        nbc = self.cam_bins(cam)
+        if nbins is not None and nbins != nbc:
+            raise ValueError(f"camera {cam} stores {nbc} bins per ray, asked for {nbins}")
         sub = lambda n: max(0, (n - self.offset + self.stride - 1) // self.stride)  # noqa: E731
-        img = Image(sub(c.nx) * sub(c.ny), nbins)
+        img = Image(sub(c.nx) * sub(c.ny), nbc)
         o = img.out()
         check(lib().hadros_batch_fetch(self.h, cam, C.byref(o)))
         return img
@@ -458,9 +504,26 @@ class MLP:
         assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous() and x.shape[1] == self.n_in
         if y is None:
             y = torch.empty((x.shape[0], self.n_out), dtype=torch.float32, device=x.device)
+        assert y.is_cuda and y.dtype == torch.float32 and y.is_contiguous() and y.numel() == x.shape[0] * self.n_out
         check(lib().hadros_mlp_forward(self.h, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(y.data_ptr()),
                                        stream))
         return y
+
+    def survival(self, x, p=None, stream=None):
+        """P = exp(-10^y_b) of the first n_out/2 outputs (log10 tau per bin)."""
+        import torch
+
+        assert x.is_cuda and x.dtype == torch.float32 and x.is_contiguous() and x.shape[1] == self.n_in
+        nb = self.n_out // 2
+        if p is None:
+            p = torch.empty((x.shape[0], nb), dtype=torch.float32, device=x.device)
+        assert p.is_cuda and p.dtype == torch.float32 and p.is_contiguous() and p.numel() == x.shape[0] * nb
+        check(lib().hadros_mlp_survival(self.h, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(p.data_ptr()),
+                                        stream))
+        return p
+
+    def set_infer_path(self, path):
+        check(lib().hadros_mlp_set_infer_path(self.h, path))
 
     def grad(self, x, t):
         g = np.zeros(self.n_params, np.float32)

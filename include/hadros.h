@@ -334,6 +334,12 @@ int hadros_batch_create_sub(const hadros_camera* cams, const hadros_scene* scene
 /* all rays' outputs in global ray order; bins are [ray x n_bins_max] with
    n_bins_max the largest bin count of the batch's scenes */
 int hadros_batch_fetch_all(hadros_batch* b, hadros_image_out* out);
+/* as hadros_batch_fetch_all, but every non-NULL pointer of `out` is DEVICE
+   memory of the current device: asynchronous device-to-device copies on
+   `stream`, ordered after the batch's last run (the library's own product
+   surface for consumers that keep the outputs on the GPU, such as the
+   surrogate scan's spot checks; no reference counterpart). */
+int hadros_batch_fetch_all_device(hadros_batch* b, const hadros_image_out* out, void* stream);
 /* workgroup size (64..256, multiple of 64) and RKF45 attempts between refills */
 int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk);
 /* times every (block in {64,128,256}) x (steps_per_chunk in {1,4,16,64}) on
@@ -368,8 +374,16 @@ int hadros_mlp_set_params(hadros_mlp* m, const float* host, int64_t n);
 /* feature and target standardisation (host arrays of n_in / n_out) */
 int hadros_mlp_set_norm(hadros_mlp* m, const float* x_mean, const float* x_std,
                         const float* y_mean, const float* y_std);
-/* y = net(x) for any n (processed in infer_batch chunks), asynchronous on stream */
+/* y = net(x) for any n, asynchronous on stream.  Width 256 with n_out <= 16
+   runs the fused kernel (all layers of a 128-row tile in one workgroup,
+   activations in LDS); other shapes run one GEMM per layer in infer_batch
+   chunks. */
 int hadros_mlp_forward(hadros_mlp* m, const float* x, int64_t n, float* y, void* stream);
+/* survival form of the fused forward: outputs [0, n_out/2) are log10 tau per
+   energy bin; p[row * n_out/2 + b] = exp(-10^y_b).  Needs the fused shape. */
+int hadros_mlp_survival(hadros_mlp* m, const float* x, int64_t n, float* p, void* stream);
+/* 0: fused forward where the shape allows (default); 1: per-layer GEMMs */
+int hadros_mlp_set_infer_path(hadros_mlp* m, int path);
 /* gradient of the mean squared error on standardised targets w.r.t. every
    parameter (host FP32, get_params order) and the loss, without updating */
 int hadros_mlp_grad(hadros_mlp* m, const float* x, const float* y, int64_t n, float* grad, float* loss);

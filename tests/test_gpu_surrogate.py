@@ -1,0 +1,177 @@
+"""GPU: the surrogate's fused inference kernel, the batch sub-lattice used by
+its spot checks, and the spot-check / fallback scan.
+
+- fused forward (one workgroup per 128 rows, all layers) vs the per-layer
+  GEMM path and vs the FP64 oracle: relative error <= 2e-2 (BF16 activations),
+  on ragged row counts (1, 127, 129, 5000);
+- survival form: P = exp(-10^y) of the log10-tau outputs, |dP| <= 1e-5;
+- sub-lattice batch: every spot pixel equals the full image's pixel bit for
+  bit, and its tan-plane coordinates are the surrogate's features;
+- batches of more than 1024 cameras (camera table outside LDS) equal the
+  plain per-camera tracer bit for bit;
+- the scan: accepted images pass their spot check (|dP| <= tol at every
+  spot pixel), fallback images equal the full trace, and with a trained
+  network the scan traces fewer rays than the all-traced scan.
+"""
+import numpy as np
+import pytest
+
+from hadros_amd import abi, ffi, surrogate as S, workloads
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+E, _, SIG = workloads.synthetic_sigma()
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def net_and_data(n, seed=4):
+    import torch
+
+    net = ffi.MLP(14, 256, 4, 16, train_batch=4096, infer_batch=1024, seed=seed)
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 14)).astype(np.float32)
+    xm, xs = x.mean(0), x.std(0) + 0.1
+    tm, ts = rng.normal(size=16).astype(np.float32), rng.uniform(0.5, 2.0, 16).astype(np.float32)
+    net.set_norm(xm, xs, tm, ts)
+    return net, x, torch.from_numpy(x).cuda(), (xm, xs, tm, ts)
+
+
+@pytest.mark.parametrize("n", [1, 127, 129, 5000])
+def test_fused_forward_matches_layered_and_oracle(n):
+    net, x, X, (xm, xs, tm, ts) = net_and_data(n)
+    y_fused = net.forward(X).cpu().numpy().astype(np.float64)
+    net.set_infer_path(1)
+    y_layer = net.forward(X).cpu().numpy().astype(np.float64)
+    net.set_infer_path(0)
+    layers = mo.unflatten(mo.bf16(net.params()), 14, 256, 4, 16)
+    yr, _ = mo.forward(layers, mo.bf16((x - xm) / xs), round_act=True)
+    yr = yr * ts + tm
+    for y in (y_fused, y_layer):
+        assert np.linalg.norm(y - yr) / np.linalg.norm(yr) < 2e-2
+    assert np.linalg.norm(y_fused - y_layer) / np.linalg.norm(y_layer) < 1e-2
+
+
+def test_fused_follows_parameter_updates():
+    """The fused path's fragment-ordered weights are refreshed after set_params
+    and after every training step."""
+    import torch
+
+    net, x, X, _ = net_and_data(2048)
+    y0 = net.forward(X).clone()
+    p = net.params()
+    net.set_params(p * 0.5)
+    y1 = net.forward(X)
+    net.set_infer_path(1)
+    y1l = net.forward(X)
+    net.set_infer_path(0)
+    assert not torch.equal(y0, y1)
+    assert float((y1 - y1l).norm() / y1l.norm()) < 1e-2
+    T = torch.zeros((2048, 16), device="cuda")
+    net.train_step(X, T, 1e-2)
+    y2 = net.forward(X)
+    net.set_infer_path(1)
+    y2l = net.forward(X)
+    assert float((y2 - y2l).norm() / y2l.norm()) < 1e-2
+
+
+def test_survival_form():
+    import torch
+
+    net, x, X, _ = net_and_data(3000)
+    y = net.forward(X).double()
+    p = net.survival(X).double()
+    ref = torch.exp(-torch.pow(10.0, y[:, :8]))
+    assert float((p - ref).abs().max()) <= 1e-5
+
+
+def test_sub_lattice_equals_full_image():
+    rng = np.random.default_rng(11)
+    prm = S.sample_params(rng, 6)
+    prm["log_rho0"] = prm["log_rho0"] - 8.0   # keep survival non-trivial
+    full = S.make_batch(prm, 40)
+    sub = S.make_batch(prm, 40, 10, 5)
+    try:
+        full.run()
+        sub.run()
+        idx = np.arange(5, 40, 10)
+        pix = (idx[:, None] * 40 + idx[None, :]).ravel()
+        for c in range(6):
+            f = full.fetch(c)
+            s = sub.fetch(c)
+            assert s.p.size == 16
+            for name in ("tau", "p", "I", "captured", "npoints", "flags", "alpha", "beta", "bin_tau", "bin_p",
+                         "bin_i"):
+                assert same(getattr(s, name), getattr(f, name)[pix]), (c, name)
+            a, b = S.pixel_coords(40, 10, 5, prm["fov_scale"][c])
+            assert np.allclose(s.alpha, a, rtol=0, atol=1e-15) and np.allclose(s.beta, b, rtol=0, atol=1e-15)
+        # whole-batch fetches (host and device) are the per-camera ones concatenated
+        import torch
+
+        allp = sub.fetch_all(8, ("bin_p", "tau")).bin_p
+        dev = torch.empty(allp.size, dtype=torch.float64, device="cuda")
+        sub.fetch_all_device(bin_p=dev)
+        assert same(dev.cpu().numpy(), allp.ravel())
+        assert same(allp, np.concatenate([sub.fetch(c).bin_p for c in range(6)]))
+    finally:
+        full.close()
+        sub.close()
+
+
+def test_batch_beyond_lds_camera_table():
+    """1100 cameras (> 1024: camera table read from global memory)."""
+    rng = np.random.default_rng(12)
+    prm = S.sample_params(rng, 1100)
+    prm["log_rho0"] = prm["log_rho0"] - 8.0
+    b = S.make_batch(prm, 6)
+    try:
+        b.run()
+        for c in (0, 1, 517, 1023, 1024, 1099):
+            got = b.fetch(c)
+            cam = S.camera_for(prm["spin"][c], prm["inc"][c], 6, prm["fov_scale"][c])
+            sc = S.scene_for(prm["profile"][c], prm["log_rho0"][c], prm["slope"][c])
+            img, _ = ffi.trace_rt(cam, sc, E, SIG)
+            for name in ("tau", "p", "I", "captured", "npoints", "bin_tau", "bin_p", "bin_i"):
+                assert same(getattr(got, name), getattr(img, name)), (c, name)
+    finally:
+        b.close()
+
+
+def test_scan_spot_check_and_fallback():
+    import torch
+
+    # small training set from the tracer, then a scan of 12 Sobol points at 64 x 64
+    X, Y, _ = S.generate(128, 16, seed=21)
+    sur = S.Surrogate(seed=0, train_batch=4096, infer_batch=1 << 16)
+    hist = sur.fit(X, Y, epochs=20)
+    assert hist[-1] < hist[0]
+    prm = S.sobol_points(12)
+    P, rep = S.scan(sur, prm, res=64, spot_stride=8, spot_offset=4, tol=1e-3, group=4)
+    _, _, kept = S.traced_scan(prm, res=64, keep=set(range(12)))
+    idx = np.arange(4, 64, 8)
+    pix = (idx[:, None] * 64 + idx[None, :]).ravel()
+    failed = set(rep["failed"].tolist())
+    Ph = P.cpu().numpy()
+    for i in range(12):
+        if i in failed:
+            assert same(Ph[i], kept[i]), i                    # fallback = exact trace (float32)
+        else:
+            assert np.abs(Ph[i][pix] - kept[i][pix]).max() <= 1e-3 + 1e-7, i
+    assert rep["traced_rays"] == 12 * 64 + len(failed) * 64 * 64
+    # the envelope medium is opaque throughout these images: the trained network
+    # serves them, so the scan traces far fewer rays than tracing every pixel
+    assert rep["fallback_rate"] < 0.5, rep
+    assert rep["traced_rays"] < 0.5 * 12 * 64 * 64
+    # without training the same network fails its spot checks and every
+    # image is traced in full
+    raw = S.Surrogate(seed=0, train_batch=4096, infer_batch=1 << 16)
+    raw.net.set_norm(*sur.net._norm)
+    P0, rep0 = S.scan(raw, prm, res=64, spot_stride=8, spot_offset=4, tol=1e-3, group=4)
+    assert rep0["fallback_rate"] > rep["fallback_rate"], (rep0, rep)
+    assert same(P0.cpu().numpy(), np.stack([kept[i] for i in range(12)])) if rep0["fallback_rate"] == 1.0 else True
+    del P, P0
+    torch.cuda.empty_cache()
