@@ -10,7 +10,9 @@ parameters (no HBM-resident input); outputs stay in HBM until the timed
 region ends.
 
 Prints ONE JSON line (rank 0).  `value` is rays/s of the plain FP64 tracer;
-the precision form (FP64 geodesics + FP32-Kahan tau/I) is reported beside it.
+beside it: the precision form (FP64 geodesics + FP32-Kahan tau/I), the
+batched 256-camera persistent launch, the surrogate-served 4096-point scan
+against the same scan traced in full, and the FP64 CPU oracle on the host.
 """
 import argparse
 import json
@@ -30,6 +32,9 @@ METRIC = "rays/s on 1 MI355X, plain vs batched+surrogate, at ≤1e-3 rel err vs 
 ALGO = {"F_RHS": 69, "F_COMB": 390, "F_PT": 150, "F_BIN": 20}
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix), AMD datasheet
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
+# surrogate forward per row: 2 (14 x 256 + 3 x 256 x 256 + 256 x 16) FLOP
+MLP_FLOP_PER_ROW = 2 * (14 * 256 + 3 * 256 * 256 + 256 * 16)
 
 
 def tracer_flops(attempts, points, nbins):
@@ -90,6 +95,63 @@ def batched_leg(e, s):
             "algorithmic_tflops": tracer_flops(st.total_attempts, st.total_points, 8) / (best * 1e-3) / 1e12}
 
 
+def surrogate_leg():
+    """Train the surrogate on 4096 sampled cameras x 32 x 32 traced rays, then
+    serve the 4096-point Sobol scan (256 x 256 x 8 bins per point, 1 %
+    spot-checked, full-trace fallback above |dP| 1e-3) and trace the same scan
+    in full for comparison."""
+    import numpy as np
+    import torch
+
+    from hadros_amd import surrogate as S
+
+    t0 = time.perf_counter()
+    X, Y, gen_ms = S.generate(4096, 32, seed=1)
+    Xt, Yt, _ = S.generate(200, 32, seed=2)
+    t_gen = time.perf_counter() - t0
+    sur = S.Surrogate(seed=0)
+    hist = sur.fit(X, Y, epochs=20)
+    ev = S.evaluate(sur, Xt, Yt)
+    # fused network kernel alone: 4M rows, survival form, HIP events on the launch stream
+    xb = torch.from_numpy(np.ascontiguousarray(np.tile(Xt, (21, 1))[:1 << 22])).cuda()
+    pb = torch.empty((xb.shape[0], S.N_BINS), dtype=torch.float32, device="cuda")
+    sur.net.survival(xb, pb)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(5):
+        sur.net.survival(xb, pb)
+    ev1.record()
+    torch.cuda.synchronize()
+    mlp_ms = ev0.elapsed_time(ev1) / 5
+    mlp_tflops = xb.shape[0] * MLP_FLOP_PER_ROW / (mlp_ms * 1e-3) / 1e12
+    del xb, pb
+    prm = S.sobol_points(4096)
+    S.scan(sur, S.subset(prm, np.arange(64)))     # warm-up of the scan path
+    P, rep = S.scan(sur, prm)
+    failed = set(rep.pop("failed").tolist())
+    keep = set(int(i) for i in np.random.default_rng(5).choice(4096, 32, replace=False))
+    tw, tms, kept = S.traced_scan(prm, keep=keep)
+    errs = [float(np.abs(P[i].cpu().numpy() - ref).max()) for i, ref in kept.items() if i not in failed]
+    fr = [float(np.mean(np.abs(P[i].cpu().numpy() - ref) > 1e-3)) for i, ref in kept.items() if i not in failed]
+    del P
+    torch.cuda.empty_cache()
+    px = 4096 * 256 * 256
+    return {"value": px / rep["seconds"]["total"], "unit": "pixels/s (256x256x8-bin images)",
+            "images_per_s": rep["images_per_s"], "scan_seconds": rep["seconds"],
+            "all_traced": {"value": px / tw, "seconds": tw, "kernel_ms": tms},
+            "speedup_vs_all_traced": tw / rep["seconds"]["total"],
+            "fallback_rate": rep["fallback_rate"], "spot_rays_per_image": rep["spot_rays_per_image"],
+            "traced_rays": rep["traced_rays"],
+            "served_vs_traced": {"images": len(errs), "max_abs_dP": max(errs) if errs else None,
+                                 "frac_pixel_bins_over_1e-3": float(np.mean(fr)) if fr else None},
+            "training": {"rays": int(X.shape[0]), "trace_ms": gen_ms, "data_seconds": t_gen,
+                         "steps": sur.train_steps, "seconds": sur.train_seconds, "loss_first_last": [hist[0], hist[-1]]},
+            "test_error": ev,
+            "mlp_kernel": {"ms_per_4M_rows": mlp_ms, "achieved": mlp_tflops, "peak": PEAK_BF16_TFLOPS,
+                           "unit": "TFLOP/s", "frac": mlp_tflops / PEAK_BF16_TFLOPS, "bound": "mfma",
+                           "flop_per_row": MLP_FLOP_PER_ROW}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +161,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-camera leg")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training + scan leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-target", action="store_true",
                     help="for rocprofv3 --pmc passes: run the plain tracer twice, then copy a 256 MiB device "
@@ -192,6 +255,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_leg(e, s)
+    if not args.no_surrogate:
+        out["surrogate_scan"] = surrogate_leg()
     tr = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tr):
         t = json.load(open(tr))

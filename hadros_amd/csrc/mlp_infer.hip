@@ -38,7 +38,69 @@ constexpr int FXK = 32;          // first-layer K (n_in zero-padded)
 constexpr int FHP = FW + 8;      // LDS row pitch of the activations (bf16)
 constexpr int FXP = FXK + 8;     // LDS row pitch of the staged inputs (bf16)
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware reciprocal (1 ulp): the result is rounded to BF16
+__device__ __forceinline__ float silu_f(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// one hidden layer's products for this wavefront: acc[f][rt] (features
+// 16 (4 wave + f) + 4 (lane / 16) + e, rows 16 rt + lane % 16) starts at the
+// bias and accumulates KS K-steps; the next step's weight fragments are
+// loaded while the current step's MFMAs issue
+template <int KS, int PITCH>
+__device__ __forceinline__ void layer_products(const __bf16* __restrict__ src, const bf16x8* __restrict__ W,
+                                               const float* __restrict__ b, int wave, int lane, f32x4 (&acc)[4][8])
+{
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int n0 = 16 * (4 * wave + f) + 4 * (lane >> 4);
+        const f32x4 bv = f32x4{b[n0], b[n0 + 1], b[n0 + 2], b[n0 + 3]};
+#pragma unroll
+        for (int rt = 0; rt < 8; ++rt) acc[f][rt] = bv;
+    }
+    bf16x8 wf[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) wf[f] = W[((4 * wave + f) * KS) * 64 + lane];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        bf16x8 wn[4];
+        if (ks + 1 < KS) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) wn[f] = W[((4 * wave + f) * KS + ks + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int rt = 0; rt < 8; ++rt) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(src + rt * 16 * PITCH + 32 * ks);
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+                acc[f][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[f], a, acc[f][rt], 0, 0, 0);
+        }
+        if (ks + 1 < KS) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) wf[f] = wn[f];
+        }
+    }
+}
+
+// SiLU of the accumulators -> BF16 activations in LDS (after every
+// wavefront has finished reading the layer's input)
+__device__ __forceinline__ void layer_store(__bf16* Hs, int wave, int lane, const f32x4 (&acc)[4][8])
+{
+    __syncthreads();
+    const int lr = lane & 15;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        const int n0 = 16 * (4 * wave + f) + 4 * (lane >> 4);
+#pragma unroll
+        for (int rt = 0; rt < 8; ++rt) {
+            bf16x4 h;
+            h[0] = (__bf16)silu_f(acc[f][rt][0]);
+            h[1] = (__bf16)silu_f(acc[f][rt][1]);
+            h[2] = (__bf16)silu_f(acc[f][rt][2]);
+            h[3] = (__bf16)silu_f(acc[f][rt][3]);
+            *reinterpret_cast<bf16x4*>(&Hs[(16 * rt + lr) * FHP + n0]) = h;
+        }
+    }
+    __syncthreads();
+}
 
 __global__ void k_swizzle(const __bf16* __restrict__ w, int K, int N, int KS, int NT, bf16x8* __restrict__ out)
 {
@@ -84,50 +146,21 @@ __global__ void __launch_bounds__(256, 2) k_mlp_fused(FusedArgs A)
     __syncthreads();
 
     const int lr = lane & 15, lk = 8 * (lane >> 4);
-    for (int l = 0; l < A.L - 1; ++l) {
+    const bf16x8* __restrict__ Wall = reinterpret_cast<const bf16x8*>(A.w);
+    {
         f32x4 acc[4][8];
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-#pragma unroll
-            for (int rt = 0; rt < 8; ++rt) acc[f][rt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        const int KS = (l == 0) ? 1 : FW / 32;
-        const bf16x8* __restrict__ W = reinterpret_cast<const bf16x8*>(A.w) + A.wo[l];
-        const __bf16* src = (l == 0) ? Xs + lr * FXP + lk : Hs + lr * FHP + lk;
-        const int pitch16 = 16 * ((l == 0) ? FXP : FHP);
-        for (int ks = 0; ks < KS; ++ks) {
-            bf16x8 wf[4];
-#pragma unroll
-            for (int f = 0; f < 4; ++f) wf[f] = W[((4 * wave + f) * KS + ks) * 64 + lane];
-#pragma unroll
-            for (int rt = 0; rt < 8; ++rt) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(src + rt * pitch16 + 32 * ks);
-#pragma unroll
-                for (int f = 0; f < 4; ++f)
-                    acc[f][rt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[f], a, acc[f][rt], 0, 0, 0);
-            }
-        }
-        __syncthreads();   // every wavefront has read this layer's input
-        const float* __restrict__ b = A.bias + A.bo[l];
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int n0 = 16 * (4 * wave + f) + 4 * (lane >> 4);
-            const float b0 = b[n0], b1 = b[n0 + 1], b2 = b[n0 + 2], b3 = b[n0 + 3];
-#pragma unroll
-            for (int rt = 0; rt < 8; ++rt) {
-                bf16x4 h;
-                h[0] = (__bf16)silu_f(acc[f][rt][0] + b0);
-                h[1] = (__bf16)silu_f(acc[f][rt][1] + b1);
-                h[2] = (__bf16)silu_f(acc[f][rt][2] + b2);
-                h[3] = (__bf16)silu_f(acc[f][rt][3] + b3);
-                *reinterpret_cast<bf16x4*>(&Hs[(16 * rt + lr) * FHP + n0]) = h;
-            }
-        }
-        __syncthreads();
+        layer_products<1, FXP>(Xs + lr * FXP + lk, Wall + A.wo[0], A.bias + A.bo[0], wave, lane, acc);
+        layer_store(Hs, wave, lane, acc);
+    }
+    for (int l = 1; l < A.L - 1; ++l) {
+        f32x4 acc[4][8];
+        layer_products<FW / 32, FHP>(Hs + lr * FHP + lk, Wall + A.wo[l], A.bias + A.bo[l], wave, lane, acc);
+        layer_store(Hs, wave, lane, acc);
     }
 
     // output layer: K = 256, one 16-feature tile; wavefront w takes rows [32 w, 32 w + 32)
     const int l = A.L - 1;
-    const bf16x8* __restrict__ W = reinterpret_cast<const bf16x8*>(A.w) + A.wo[l];
+    const bf16x8* __restrict__ W = Wall + A.wo[l];
     f32x4 o[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
 #pragma unroll
     for (int ks = 0; ks < FW / 32; ++ks) {

@@ -241,7 +241,7 @@ def subset(params, idx):
     return {k: v[idx] for k, v in params.items()}
 
 
-def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=64, tuning=None,
+def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=256, tuning=None,
          fallback_chunk=512):
     """Render every parameter point at res x res x 8 bins with the network,
     spot-check the sub-lattice i, j = offset + stride k of each image (26 x 26
