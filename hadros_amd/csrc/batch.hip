@@ -222,7 +222,7 @@ k_trace_batch(BatchArgs A)
                                 if (P.spec_ok && q.g3Adl > 0.0)
                                     kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
                                 if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                                    ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb)));
+                                    ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d))));
                             }
                         }
                     }

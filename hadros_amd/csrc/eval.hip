@@ -11,7 +11,8 @@ __global__ void k_eval_rho(DevMedium m, const double* r, const double* th, doubl
     if (k < n) out[k] = rho(m, r[k], th[k]);
 }
 
-__global__ void k_eval_sigma(const double* sig, int ns, double e_lo, double e_hi, const double* e, double* out,
+__global__ void k_eval_sigma(const double* sig, int ns, double e_lo, double e_hi, double lo0, double inv_d,
+                             const double* e, double* out,
                              long n)
 {
     const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -21,7 +22,7 @@ __global__ void k_eval_sigma(const double* sig, int ns, double e_lo, double e_hi
         out[k] = __builtin_nan("");
         return;
     }
-    out[k] = exp(sigma_ln(sig, sig + ns, sig + 2 * ns, ns, log(x)));
+    out[k] = exp(sigma_ln(sig, sig + ns, sig + 2 * ns, ns, log(x), lo0, inv_d));
 }
 
 // emissivity_uhe = (norm * spatial) * S(E) (src/radiative_transfer.cpp:369-395)
@@ -80,10 +81,10 @@ hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th
     hipLaunchKernelGGL(k_eval_rho, dim3(blocks(n)), dim3(256), 0, 0, m, r, th, out, n);
     return hipGetLastError();
 }
-hipError_t launch_eval_sigma(const double* sig, int ns, double e_lo, double e_hi, const double* e, double* out,
-                             long n)
+hipError_t launch_eval_sigma(const double* sig, int ns, double e_lo, double e_hi, double lo0, double inv_d,
+                             const double* e, double* out, long n)
 {
-    hipLaunchKernelGGL(k_eval_sigma, dim3(blocks(n)), dim3(256), 0, 0, sig, ns, e_lo, e_hi, e, out, n);
+    hipLaunchKernelGGL(k_eval_sigma, dim3(blocks(n)), dim3(256), 0, 0, sig, ns, e_lo, e_hi, lo0, inv_d, e, out, n);
     return hipGetLastError();
 }
 hipError_t launch_eval_emis(const DevRt& P, const double* r, const double* th, const double* e, double* out, long n)

@@ -129,6 +129,19 @@ int check_sigma(const double* e, const double* s, int n)
     return HADROS_OK;
 }
 
+// index guess of the device lookup for tables uniform in ln E (every node
+// within a quarter cell of the line through the end nodes); inv_d = 0 selects
+// the binary search
+void sigma_guess(const double* se, int n, double& lo0, double& inv_d)
+{
+    const double l0 = std::log(se[0]), l1 = std::log(se[n - 1]);
+    const double d = (l1 - l0) / double(n - 1);
+    bool uniform = d > 0.0;
+    for (int k = 0; uniform && k < n; ++k) uniform = std::fabs(std::log(se[k]) - (l0 + d * double(k))) <= 0.25 * d;
+    lo0 = l0;
+    inv_d = uniform ? 1.0 / d : 0.0;
+}
+
 // DevRt of one scene: integrate_kerr_ray_spectral's bins (src/radiative_transfer.cpp:687-761),
 // spectral factor, r_g, horizon of the transfer, sigma range.
 int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, double a_bh, DevRt& P)
@@ -158,6 +171,7 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
     P.n_sigma = n;
     P.e_lo = se[0];
     P.e_hi = se[n - 1];
+    sigma_guess(se, n, P.sig_lo, P.sig_inv_d);
     P.rg_cm = G_CGS * s.m_bh_msun * MSUN_G / (C_CGS * C_CGS);
     const double a = dclamp(a_bh, 0.0, 0.999999);
     P.r_h_rt = 1.0 + std::sqrt(1.0 - a * a);
@@ -1278,9 +1292,11 @@ int hadros_eval_sigma(const double* se, const double* ss, int n_sigma, const dou
     std::vector<float> sf;
     sigma_tables(se, ss, n_sigma, sd, sf);
     const double lo = se[0], hi = se[n_sigma - 1];
+    double lo0 = 0.0, inv_d = 0.0;
+    sigma_guess(se, n_sigma, lo0, inv_d);
     return run_eval({{sd.data(), sd.size()}, {e, size_t(n)}}, out, size_t(n),
                     [&](std::vector<double*>& d, double* o) {
-                        return launch_eval_sigma(d[0], n_sigma, lo, hi, d[1], o, long(n));
+                        return launch_eval_sigma(d[0], n_sigma, lo, hi, lo0, inv_d, d[1], o, long(n));
                     });
 }
 

@@ -88,8 +88,8 @@ hipError_t batch_resident_blocks(int mode, int precision, int nb, int block, int
 
 // point-wise hooks (eval.hip)
 hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th, double* out, long n);
-hipError_t launch_eval_sigma(const double* sig, int n_sigma, double e_lo, double e_hi, const double* e, double* out,
-                             long n);
+hipError_t launch_eval_sigma(const double* sig, int n_sigma, double e_lo, double e_hi, double lo0, double inv_d,
+                             const double* e, double* out, long n);
 hipError_t launch_eval_emis(const DevRt& P, const double* r, const double* th, const double* e, double* out,
                             long n);
 hipError_t launch_eval_rhs(double a, int mode, const double* y, double* dy, long n);

@@ -95,6 +95,8 @@ struct DevRt {
     double rg_cm;         // r_g in cm (include/constants.hpp:17-20)
     double r_h_rt;        // 1 + sqrt(1 - clamp(a,0,0.999999)^2) (src/radiative_transfer.cpp:485-489)
     double e_lo, e_hi;    // sigma table range
+    double sig_lo;        // ln E of the first table node
+    double sig_inv_d;     // (n - 1) / (ln E_last - ln E_first) when the nodes are uniform in ln E, else 0
     double emid[HADROS_MAX_BINS], lnemid[HADROS_MAX_BINS], de[HADROS_MAX_BINS], w[HADROS_MAX_BINS];
 };
 
@@ -465,9 +467,21 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
 // LDS-staged table: ln E_k, ln sigma_k, slope_k = d ln sigma / d ln E on
 // [k-1, k] (slope_0 unused).  lower_bound over ln E as std::lower_bound
 // over E (src/sigma_table.cpp:100-124).
-__device__ __forceinline__ double sigma_ln(const double* __restrict__ lnE, const double* __restrict__ lnS,
-                                           const double* __restrict__ slope, int n, double lx)
+// lower_bound of lx in the ascending table lnE[0..n): the first k with
+// lnE[k] >= lx (std::lower_bound of src/sigma_table.cpp:55-58,108-110).  A
+// table uniform in ln E (inv_d > 0) starts from the arithmetic index guess
+// and walks it to the exact bound against the table (one or two LDS reads);
+// other tables use a binary search.
+template <typename T>
+__device__ __forceinline__ int sigma_lower_bound(const T* __restrict__ lnE, int n, T lx, T lo0, T inv_d)
 {
+    if (inv_d > T(0)) {
+        const T t = (lx - lo0) * inv_d;
+        int g = !(t > T(0)) ? 0 : (t >= T(n - 1) ? n - 1 : int(ceil(t)));
+        while (g > 0 && lnE[g - 1] >= lx) --g;
+        while (g < n && lnE[g] < lx) ++g;
+        return g;
+    }
     int lo = 0, cnt = n;
     while (cnt > 0) {
         const int half = cnt >> 1;
@@ -478,24 +492,26 @@ __device__ __forceinline__ double sigma_ln(const double* __restrict__ lnE, const
             cnt = half;
         }
     }
+    return lo;
+}
+
+// ln sigma at ln E = lx: log-log interpolation of the table (slope[k] is the
+// ln sigma slope of interval [k-1, k]); clamped to the end nodes.
+__device__ __forceinline__ double sigma_ln(const double* __restrict__ lnE, const double* __restrict__ lnS,
+                                           const double* __restrict__ slope, int n, double lx, double lo0 = 0.0,
+                                           double inv_d = 0.0)
+{
+    const int lo = sigma_lower_bound(lnE, n, lx, lo0, inv_d);
     if (lo == 0) return lnS[0];
     if (lo >= n) return lnS[n - 1];
     return lnS[lo - 1] + (lx - lnE[lo - 1]) * slope[lo];
 }
 
 __device__ __forceinline__ float sigma_ln_f(const float* __restrict__ lnE, const float* __restrict__ lnS,
-                                            const float* __restrict__ slope, int n, float lx)
+                                            const float* __restrict__ slope, int n, float lx, float lo0 = 0.0f,
+                                            float inv_d = 0.0f)
 {
-    int lo = 0, cnt = n;
-    while (cnt > 0) {
-        const int half = cnt >> 1;
-        if (lnE[lo + half] < lx) {
-            lo += half + 1;
-            cnt -= half + 1;
-        } else {
-            cnt = half;
-        }
-    }
+    const int lo = sigma_lower_bound(lnE, n, lx, lo0, inv_d);
     if (lo == 0) return lnS[0];
     if (lo >= n) return lnS[n - 1];
     return lnS[lo - 1] + (lx - lnE[lo - 1]) * slope[lo];
@@ -539,7 +555,7 @@ __device__ __forceinline__ void point_rt64(const DevRt& P, const PointQ& q, cons
         const double lnEb = P.lnemid[b] + q.lnz;
         if (P.spec_ok && q.g3Adl != 0.0) I[b] += q.g3Adl * exp(-P.gam * lnEb - E * P.invcut - tau[b]);
         if (q.rho > 0.0 && E >= P.e_lo && E <= P.e_hi) {
-            const double ls = sigma_ln(lnE, lnS, slope, P.n_sigma, lnEb);
+            const double ls = sigma_ln(lnE, lnS, slope, P.n_sigma, lnEb, P.sig_lo, P.sig_inv_d);
             tau[b] += q.nbar_dl * exp(ls);
         }
     }

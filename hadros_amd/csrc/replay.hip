@@ -6,34 +6,90 @@
 // ray sequentially (tau before a point weights that point's emission); here a
 // wavefront owns a ray and processes 64 consecutive points per iteration:
 // coalesced SoA loads of (r, theta, dl, z), per-bin dtau and emission terms,
-// a wavefront exclusive prefix sum of dtau gives tau before every point, and
-// a wavefront reduction adds the 64 emission terms.  HBM traffic is the
-// algorithmic 32 B per point.
+// a wavefront prefix sum of dtau gives tau before every point, and a second
+// prefix sum adds the 64 emission terms (its lane 63).  The prefix sums are
+// DPP lane moves (row shifts 1, 2, 4, 8, then the row-15 and row-31
+// broadcasts), run for all bins in lockstep so the bins' dependency chains
+// overlap.  HBM traffic is the algorithmic 32 B per point.
 #include "kernels.h"
 
 namespace hd {
 
-template <typename T>
-__device__ __forceinline__ T wave_excl_scan(T x, T& total)
+// DPP move of one 32-bit word; out-of-row and masked-off lanes keep `old`
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_word(int old, int v)
 {
-    const int lane = threadIdx.x & 63;
-    T incl = x;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const T y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    total = __shfl(incl, 63, 64);
-    return incl - x;
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T x)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v)   // identity +0.0
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(dpp_word<CTRL, ROW_MASK>(0, int(u & 0xffffffffull)));
+    const unsigned hi = unsigned(dpp_word<CTRL, ROW_MASK>(0, int(u >> 32)));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v, float identity)
+{
+    return __int_as_float(dpp_word<CTRL, ROW_MASK>(__float_as_int(identity), __float_as_int(v)));
+}
+
+// inclusive prefix sums over the 64 lanes of N independent values per lane
+template <int N>
+__device__ __forceinline__ void wave_scan_add(double (&x)[N])
+{
+#define HD_SCAN_STEP(CTRL, RM)                                            \
+    _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_move<CTRL, RM>(x[b]);
+    HD_SCAN_STEP(0x111, 0xf)   // row_shr:1
+    HD_SCAN_STEP(0x112, 0xf)   // row_shr:2
+    HD_SCAN_STEP(0x114, 0xf)   // row_shr:4
+    HD_SCAN_STEP(0x118, 0xf)   // row_shr:8
+    HD_SCAN_STEP(0x142, 0xa)   // row_bcast:15 into rows 1, 3
+    HD_SCAN_STEP(0x143, 0xc)   // row_bcast:31 into rows 2, 3
+#undef HD_SCAN_STEP
+}
+
+template <int N>
+__device__ __forceinline__ void wave_scan_add(float (&x)[N])
+{
+#define HD_SCAN_STEP(CTRL, RM)                                            \
+    _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_move<CTRL, RM>(x[b], 0.0f);
+    HD_SCAN_STEP(0x111, 0xf)
+    HD_SCAN_STEP(0x112, 0xf)
+    HD_SCAN_STEP(0x114, 0xf)
+    HD_SCAN_STEP(0x118, 0xf)
+    HD_SCAN_STEP(0x142, 0xa)
+    HD_SCAN_STEP(0x143, 0xc)
+#undef HD_SCAN_STEP
+}
+
+// inclusive prefix maxima (identity -inf)
+template <int N>
+__device__ __forceinline__ void wave_scan_max(float (&x)[N])
+{
+#define HD_SCAN_STEP(CTRL, RM)                                            \
+    _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] = fmaxf(x[b], dpp_move<CTRL, RM>(x[b], -INFINITY));
+    HD_SCAN_STEP(0x111, 0xf)
+    HD_SCAN_STEP(0x112, 0xf)
+    HD_SCAN_STEP(0x114, 0xf)
+    HD_SCAN_STEP(0x118, 0xf)
+    HD_SCAN_STEP(0x142, 0xa)
+    HD_SCAN_STEP(0x143, 0xc)
+#undef HD_SCAN_STEP
+}
+
+__device__ __forceinline__ double lane63(double v)
+{
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(u & 0xffffffffull), 63));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(u >> 32), 63));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 
 template <int PREC, int NBMAX>
 __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
@@ -84,47 +140,76 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
                 pq.rho = 0.0;
             }
             if constexpr (PREC == HADROS_PREC_FP64) {
+                double inc[NBMAX], em[NBMAX];
 #pragma unroll
                 for (int b = 0; b < NBMAX; ++b) {
+                    inc[b] = 0.0;
                     if (b >= P.nb) continue;
                     const double E = P.emid[b] * pq.z;
                     const double lnEb = P.lnemid[b] + pq.lnz;
-                    double dtau = 0.0;
                     if (ok && pq.rho > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                        dtau = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb));
-                    double tot;
-                    const double before = tau[b] + wave_excl_scan(dtau, tot);
-                    double em = 0.0;
-                    if (ok && P.spec_ok && pq.g3Adl != 0.0) em = pq.g3Adl * exp(-P.gam * lnEb - E * P.invcut - before);
-                    I[b] += wave_sum(em);
-                    tau[b] += tot;
+                        inc[b] = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d));
+                    em[b] = inc[b];   // dtau of this point, until the scan
+                }
+                wave_scan_add(inc);
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    const double dtau = em[b];
+                    em[b] = 0.0;
+                    if (b >= P.nb) continue;
+                    const double E = P.emid[b] * pq.z;
+                    const double lnEb = P.lnemid[b] + pq.lnz;
+                    const double before = tau[b] + (inc[b] - dtau);
+                    if (ok && P.spec_ok && pq.g3Adl != 0.0) em[b] = pq.g3Adl * exp(-P.gam * lnEb - E * P.invcut - before);
+                }
+                wave_scan_add(em);
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    if (b >= P.nb) continue;
+                    I[b] += lane63(em[b]);
+                    tau[b] += lane63(inc[b]);
                 }
             } else {
                 const float lnz = float(pq.lnz);
                 const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
                 const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
                 const float gam = float(P.gam), invcut = float(P.invcut);
+                float inc[NBMAX], dt[NBMAX], L[NBMAX], m[NBMAX];
 #pragma unroll
                 for (int b = 0; b < NBMAX; ++b) {
+                    inc[b] = 0.0f;
                     if (b >= P.nb) continue;
                     const double E = P.emid[b] * pq.z;
                     const float lnEb = float(P.lnemid[b]) + lnz;
-                    float dtau = 0.0f;
                     if (ok && pq.rho > 0.0 && pq.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                        dtau = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb));
-                    float tot;
-                    const float before = ftau[b].s + wave_excl_scan(dtau, tot);
-                    // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
-                    float L = -INFINITY;
-                    if (ok && P.spec_ok && pq.g3Adl > 0.0) L = lP - gam * lnEb - float(E) * invcut - before;
-                    float m = L;
+                        inc[b] = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d)));
+                    dt[b] = inc[b];
+                }
+                wave_scan_add(inc);
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-                    if (m > -INFINITY) {
-                        const float s = wave_sum(__expf(L - m));
-                        fI[b].add_log(m + __logf(s));
+                for (int b = 0; b < NBMAX; ++b) {
+                    L[b] = -INFINITY;
+                    if (b < P.nb) {
+                        const double E = P.emid[b] * pq.z;
+                        const float lnEb = float(P.lnemid[b]) + lnz;
+                        const float before = ftau[b].s + (inc[b] - dt[b]);
+                        // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
+                        if (ok && P.spec_ok && pq.g3Adl > 0.0) L[b] = lP - gam * lnEb - float(E) * invcut - before;
                     }
-                    ftau[b].add(tot);
+                    m[b] = L[b];
+                }
+                wave_scan_max(m);
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    m[b] = lane63(m[b]);
+                    dt[b] = (m[b] > -INFINITY) ? __expf(L[b] - m[b]) : 0.0f;
+                }
+                wave_scan_add(dt);
+#pragma unroll
+                for (int b = 0; b < NBMAX; ++b) {
+                    if (b >= P.nb) continue;
+                    if (m[b] > -INFINITY) fI[b].add_log(m[b] + __logf(lane63(dt[b])));
+                    ftau[b].add(lane63(inc[b]));
                 }
             }
         }

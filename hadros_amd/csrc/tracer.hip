@@ -174,7 +174,7 @@ k_trace_rt(TraceArgs A)
                     if (Pq.spec_ok && q.g3Adl > 0.0)
                         kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
                     if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= Pq.e_lo && E <= Pq.e_hi)
-                        ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb)));
+                        ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(Pq.sig_lo), float(Pq.sig_inv_d))));
                 }
             }
         }

@@ -88,6 +88,21 @@ def test_eval_sigma():
     assert np.isnan(out).all()
 
 
+def test_eval_sigma_lookup_paths():
+    """The index-guess lookup (tables uniform in ln E) and the binary search
+    (other tables) both bracket exactly like the oracle's lower_bound: every
+    node, every midpoint, and values one ulp either side of a node."""
+    rng = np.random.default_rng(3)
+    uneven = np.sort(10 ** rng.uniform(4, 12, 40))
+    uneven_s = 5.53e-36 * uneven ** 0.363 * (1 + 0.3 * rng.uniform(size=40))
+    for se, ss in ((SIG_E, SIG_S), (uneven, uneven_s)):
+        mids = np.sqrt(se[1:] * se[:-1])
+        e = np.concatenate([se, mids, np.nextafter(se[1:], 0), np.nextafter(se[:-1], np.inf)])
+        g = ffi.eval_sigma(se, ss, e)
+        ref = np.array([po.lib().or_sigma(po.dptr(se), po.dptr(ss), se.size, x) for x in e])
+        assert rel(g, ref).max() < 1e-13
+
+
 def _compare_images(img, ref, tol_p=1e-6, tol_i=1e-6, min_match=0.99):
     assert np.array_equal(img.captured, ref.captured)
     match = np.mean(img.npoints == ref.npoints)
