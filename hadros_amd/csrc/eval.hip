@@ -33,7 +33,7 @@ __global__ void k_eval_emis(DevRt P, const double* r, const double* th, const do
     const double E = e[k];
     double j = 0.0;
     if (E > 0.0 && P.spec_ok) {
-        const double A = source_factor(P.src, P.med, r[k], th[k]);
+        const double A = source_factor(P.src, P.med, r[k], th[k], rho(P.med, r[k], th[k]));
         j = A * exp(-P.gam * log(E) - E * P.invcut);
     }
     out[k] = j;

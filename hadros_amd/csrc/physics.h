@@ -425,7 +425,9 @@ __device__ __forceinline__ double bipolar(double th, double th0, double w)
 
 // norm x spatial factor of emissivity_uhe (src/radiative_transfer.cpp:209-395):
 // bin-independent, so evaluated once per path point.
-__device__ __forceinline__ double source_factor(const DevSource& so, const DevMedium& m, double r, double th)
+// rho_here: the density at (r, th), which the caller has already evaluated
+__device__ __forceinline__ double source_factor(const DevSource& so, const DevMedium& m, double r, double th,
+                                                double rho_here)
 {
     if (so.kind == HADROS_SOURCE_INNER_RING) {
         if (!so.ring_ok) return 0.0;
@@ -441,7 +443,7 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
     } else if (so.kind == HADROS_SOURCE_JET_BASE) {
         W = exp(-sq(r / so.rc)) * bipolar(th, 0.0, so.tw);
     } else if (so.kind == HADROS_SOURCE_DENSITY_WEIGHTED) {
-        const double rh = rho(m, r, th);
+        const double rh = rho_here;
         const double rho_norm = dmax(so.rho_ref > 0.0 ? so.rho_ref : so.rho_norm_c, 1.0e-300);
         const double dw = pow(dmax(rh / rho_norm, 0.0), so.q);
         const double rw = pow(dmax(r / dmax(so.rc, 1.0e-300), 1.0e-300), -so.s);
@@ -449,13 +451,22 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
     } else if (so.kind == HADROS_SOURCE_SHOCK_LAYER) {
         const double dr = dmax(so.gdr, 1.0e-4);
         const double dth = dmax(so.gdth, 1.0e-5);
-        const double rp = rho(m, r + dr, th);
-        const double rm = rho(m, dmax(r - dr, 1.0e-6), th);
-        const double tp = rho(m, r, dclamp(th + dth, 0.0, PI));
-        const double tm = rho(m, r, dclamp(th - dth, 0.0, PI));
+        // rho at (r +- dr, th) and (r, th +- dth): one density evaluation in a
+        // rolled loop (four inlined copies would set the kernels' register use)
+        double sr = 0.0, st = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < 4; ++k) {
+            const double rr = (k == 0) ? r + dr : (k == 1) ? dmax(r - dr, 1.0e-6) : r;
+            const double tt = (k == 2) ? dclamp(th + dth, 0.0, PI) : (k == 3) ? dclamp(th - dth, 0.0, PI) : th;
+            const double v = rho(m, rr, tt);
+            if (k == 0) sr = v;
+            else if (k == 1) sr = sr - v;
+            else if (k == 2) st = v;
+            else st = st - v;
+        }
         const double rho_norm = dmax(so.rho_norm_c, 1.0e-300);
-        const double d_r = (rp - rm) / (2.0 * dr);
-        const double d_t = (tp - tm) / (2.0 * dth * dmax(r, 1.0e-6));
+        const double d_r = sr / (2.0 * dr);
+        const double d_t = st / (2.0 * dth * dmax(r, 1.0e-6));
         const double grad = sqrt(d_r * d_r + d_t * d_t) / rho_norm;
         const double taper = exp(-sq(r / dmax(so.rc, 1.0e-300)));
         W = dclamp(grad * taper, dmax(so.cmin, 0.0), dmax(so.cmax, dmax(so.cmin, 0.0)));
@@ -532,7 +543,7 @@ __device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, dou
     const double g = 1.0 / zc;
     const double dl_cm = dl * P.rg_cm;
     q.rho = rho(P.med, r, th);
-    const double A = source_factor(P.src, P.med, r, th);
+    const double A = source_factor(P.src, P.med, r, th, q.rho);
     q.z = zc;
     q.lnz = log(zc);
     q.g3Adl = g * g * g * A * dl_cm;

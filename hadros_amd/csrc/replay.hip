@@ -91,11 +91,44 @@ __device__ __forceinline__ double lane63(double v)
 
 __device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 
-template <int PREC, int NBMAX>
-__global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
+#ifndef HD_REPLAY_WAVES_PER_EU
+#define HD_REPLAY_WAVES_PER_EU 4
+#endif
+
+constexpr int REPLAY_BLOCK = 256;
+constexpr int REPLAY_WAVES = REPLAY_BLOCK / 64;
+
+// Per-wavefront, per-bin running sums of the current ray, kept in LDS so that
+// the bin loop stays rolled (few registers, more resident wavefronts).  Lane
+// 63 holds each chunk's inclusive totals and alone updates them; LDS
+// operations of one wavefront complete in order, so the next bin/chunk reads
+// see the update.
+struct ReplayAcc64 {
+    double tau[HADROS_MAX_BINS], I[HADROS_MAX_BINS];
+};
+struct ReplayAcc32 {
+    Kahan tau[HADROS_MAX_BINS];
+    ScaledKahan I[HADROS_MAX_BINS];
+};
+
+template <int PREC>
+__global__ void __launch_bounds__(REPLAY_BLOCK) __attribute__((amdgpu_waves_per_eu(HD_REPLAY_WAVES_PER_EU)))
+k_replay(ReplayArgs A)
 {
     extern __shared__ double smem[];
+    __shared__ ReplayAcc64 acc64[REPLAY_WAVES];
+    __shared__ ReplayAcc32 acc32[REPLAY_WAVES];
+    // per-bin constants, indexed by the rolled bin loop
+    __shared__ double bin_e[HADROS_MAX_BINS], bin_lne[HADROS_MAX_BINS], bin_w[HADROS_MAX_BINS],
+        bin_de[HADROS_MAX_BINS];
     const DevRt& P = A.rt;
+    if (threadIdx.x < HADROS_MAX_BINS) {
+        const int b = threadIdx.x;
+        bin_e[b] = P.emid[b];
+        bin_lne[b] = P.lnemid[b];
+        bin_w[b] = P.w[b];
+        bin_de[b] = P.de[b];
+    }
     const int n = P.n_sigma;
     const double* lnE = smem;
     const double* lnS = smem + n;
@@ -112,21 +145,21 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
-    const int waves_per_block = blockDim.x >> 6;
-    const long nwaves = long(gridDim.x) * waves_per_block;
+    const int wv = threadIdx.x >> 6;
+    ReplayAcc64& a64 = acc64[wv];
+    ReplayAcc32& a32 = acc32[wv];
+    const int nb = P.nb;
+    const long nwaves = long(gridDim.x) * REPLAY_WAVES;
     unsigned long long pts_done = 0;
-    for (long ray = long(blockIdx.x) * waves_per_block + (threadIdx.x >> 6); ray < A.n_rays; ray += nwaves) {
+    for (long ray = long(blockIdx.x) * REPLAY_WAVES + wv; ray < A.n_rays; ray += nwaves) {
         const long b0 = A.offset[ray], b1 = A.offset[ray + 1];
-        double tau[NBMAX], I[NBMAX];
-        Kahan ftau[NBMAX];
-        ScaledKahan fI[NBMAX];
-#pragma unroll
-        for (int b = 0; b < NBMAX; ++b) {
-            tau[b] = 0.0;
-            I[b] = 0.0;
-            ftau[b] = {0.0f, 0.0f};
-            fI[b] = {{0.0f, 0.0f}, 0.0f, 0};
+        if (lane < nb) {
+            a64.tau[lane] = 0.0;
+            a64.I[lane] = 0.0;
+            a32.tau[lane] = {0.0f, 0.0f};
+            a32.I[lane] = {{0.0f, 0.0f}, 0.0f, 0};
         }
+        __builtin_amdgcn_wave_barrier();
         for (long base = b0; base < b1; base += 64) {
             const long q = base + lane;
             PointQ pq;
@@ -140,152 +173,128 @@ __global__ void __launch_bounds__(256) k_replay(ReplayArgs A)
                 pq.rho = 0.0;
             }
             if constexpr (PREC == HADROS_PREC_FP64) {
-                double inc[NBMAX], em[NBMAX];
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    inc[b] = 0.0;
-                    if (b >= P.nb) continue;
-                    const double E = P.emid[b] * pq.z;
-                    const double lnEb = P.lnemid[b] + pq.lnz;
+#pragma unroll 1
+                for (int b = 0; b < nb; ++b) {
+                    const double E = bin_e[b] * pq.z;
+                    const double lnEb = bin_lne[b] + pq.lnz;
+                    double dt = 0.0;
                     if (ok && pq.rho > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                        inc[b] = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d));
-                    em[b] = inc[b];   // dtau of this point, until the scan
-                }
-                wave_scan_add(inc);
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    const double dtau = em[b];
-                    em[b] = 0.0;
-                    if (b >= P.nb) continue;
-                    const double E = P.emid[b] * pq.z;
-                    const double lnEb = P.lnemid[b] + pq.lnz;
-                    const double before = tau[b] + (inc[b] - dtau);
-                    if (ok && P.spec_ok && pq.g3Adl != 0.0) em[b] = pq.g3Adl * exp(-P.gam * lnEb - E * P.invcut - before);
-                }
-                wave_scan_add(em);
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    if (b >= P.nb) continue;
-                    I[b] += lane63(em[b]);
-                    tau[b] += lane63(inc[b]);
+                        dt = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d));
+                    double inc[1] = {dt};
+                    wave_scan_add(inc);
+                    const double tau_b = a64.tau[b];
+                    const double before = tau_b + (inc[0] - dt);
+                    double em[1] = {0.0};
+                    if (ok && P.spec_ok && pq.g3Adl != 0.0)
+                        em[0] = pq.g3Adl * exp(-P.gam * lnEb - E * P.invcut - before);
+                    wave_scan_add(em);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 63) {
+                        a64.tau[b] = tau_b + inc[0];
+                        a64.I[b] += em[0];
+                    }
+                    __builtin_amdgcn_wave_barrier();
                 }
             } else {
                 const float lnz = float(pq.lnz);
                 const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
                 const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
                 const float gam = float(P.gam), invcut = float(P.invcut);
-                float inc[NBMAX], dt[NBMAX], L[NBMAX], m[NBMAX];
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    inc[b] = 0.0f;
-                    if (b >= P.nb) continue;
-                    const double E = P.emid[b] * pq.z;
-                    const float lnEb = float(P.lnemid[b]) + lnz;
+#pragma unroll 1
+                for (int b = 0; b < nb; ++b) {
+                    const double E = bin_e[b] * pq.z;
+                    const float lnEb = float(bin_lne[b]) + lnz;
+                    float dt = 0.0f;
                     if (ok && pq.rho > 0.0 && pq.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                        inc[b] = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d)));
-                    dt[b] = inc[b];
-                }
-                wave_scan_add(inc);
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    L[b] = -INFINITY;
-                    if (b < P.nb) {
-                        const double E = P.emid[b] * pq.z;
-                        const float lnEb = float(P.lnemid[b]) + lnz;
-                        const float before = ftau[b].s + (inc[b] - dt[b]);
-                        // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
-                        if (ok && P.spec_ok && pq.g3Adl > 0.0) L[b] = lP - gam * lnEb - float(E) * invcut - before;
+                        dt = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d)));
+                    float inc[1] = {dt};
+                    wave_scan_add(inc);
+                    const float before = a32.tau[b].s + (inc[0] - dt);
+                    // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
+                    float L = -INFINITY;
+                    if (ok && P.spec_ok && pq.g3Adl > 0.0) L = lP - gam * lnEb - float(E) * invcut - before;
+                    float m[1] = {L};
+                    wave_scan_max(m);
+                    const float mx = lane63(m[0]);
+                    float w[1] = {(mx > -INFINITY) ? __expf(L - mx) : 0.0f};
+                    wave_scan_add(w);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane == 63) {
+                        if (mx > -INFINITY) a32.I[b].add_log(mx + __logf(w[0]));
+                        a32.tau[b].add(inc[0]);
                     }
-                    m[b] = L[b];
-                }
-                wave_scan_max(m);
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    m[b] = lane63(m[b]);
-                    dt[b] = (m[b] > -INFINITY) ? __expf(L[b] - m[b]) : 0.0f;
-                }
-                wave_scan_add(dt);
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    if (b >= P.nb) continue;
-                    if (m[b] > -INFINITY) fI[b].add_log(m[b] + __logf(lane63(dt[b])));
-                    ftau[b].add(lane63(inc[b]));
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
         }
         pts_done += (unsigned long long)(b1 - b0);
         if (lane == 0) {
-            double bt[NBMAX], bp[NBMAX], bi[NBMAX];
-#pragma unroll
-            for (int b = 0; b < NBMAX; ++b) {
+            double t0 = 0.0, p0 = 1.0, i0 = 0.0;
+            double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
+#pragma unroll 1
+            for (int b = 0; b < nb; ++b) {
+                double bt, bi;
                 if (PREC == HADROS_PREC_FP64) {
-                    bt[b] = tau[b];
-                    bi[b] = I[b];
+                    bt = a64.tau[b];
+                    bi = a64.I[b];
                 } else {
-                    bt[b] = double(ftau[b].s) - double(ftau[b].c);
-                    bi[b] = fI[b].set ? (double(fI[b].k.s) - double(fI[b].k.c)) * exp(double(fI[b].off)) : 0.0;
+                    const Kahan kt = a32.tau[b];
+                    const ScaledKahan ki = a32.I[b];
+                    bt = double(kt.s) - double(kt.c);
+                    bi = ki.set ? (double(ki.k.s) - double(ki.k.c)) * exp(double(ki.off)) : 0.0;
                 }
-                bp[b] = exp(-bt[b]);
+                const double bp = exp(-bt);
+                if (b == 0) {
+                    t0 = bt;
+                    p0 = bp;
+                    i0 = bi;
+                }
+                ws += bin_w[b];
+                ts += bin_w[b] * bt;
+                ps += bin_w[b] * bp;
+                is += bi * bin_de[b];
+                if (A.btau) A.btau[ray * nb + b] = bt;
+                if (A.bp) A.bp[ray * nb + b] = bp;
+                if (A.bi) A.bi[ray * nb + b] = bi;
             }
             double agg_t = 0.0, agg_p = 1.0, agg_i = 0.0;
             if (b1 > b0) {
                 if (P.mono) {
-                    if (P.nb > 0) {
-                        agg_t = bt[0];
-                        agg_p = bp[0];
-                        agg_i = bi[0];
+                    if (nb > 0) {
+                        agg_t = t0;
+                        agg_p = p0;
+                        agg_i = i0;
                     }
-                } else {
-                    double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
-#pragma unroll
-                    for (int b = 0; b < NBMAX; ++b) {
-                        if (b >= P.nb) continue;
-                        ws += P.w[b];
-                        ts += P.w[b] * bt[b];
-                        ps += P.w[b] * bp[b];
-                        is += bi[b] * P.de[b];
-                    }
-                    if (ws > 0.0) {
-                        agg_t = ts / ws;
-                        agg_p = ps / ws;
-                        agg_i = is;
-                    }
+                } else if (ws > 0.0) {
+                    agg_t = ts / ws;
+                    agg_p = ps / ws;
+                    agg_i = is;
                 }
             }
             if (A.tau) A.tau[ray] = agg_t;
             if (A.p) A.p[ray] = agg_p;
             if (A.I) A.I[ray] = agg_i;
-#pragma unroll
-            for (int b = 0; b < NBMAX; ++b) {
-                if (b >= P.nb) continue;
-                if (A.btau) A.btau[ray * P.nb + b] = bt[b];
-                if (A.bp) A.bp[ray * P.nb + b] = bp[b];
-                if (A.bi) A.bi[ray * P.nb + b] = bi[b];
-            }
         }
+        __builtin_amdgcn_wave_barrier();
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);
 }
 
 template <int PREC>
-static void launch_p(const ReplayArgs& A, int grid, int block, size_t lds, hipStream_t s)
+static void launch_p(const ReplayArgs& A, int grid, size_t lds, hipStream_t s)
 {
-    if (A.rt.nb <= 1) hipLaunchKernelGGL((k_replay<PREC, 1>), dim3(grid), dim3(block), lds, s, A);
-    else if (A.rt.nb <= 8) hipLaunchKernelGGL((k_replay<PREC, 8>), dim3(grid), dim3(block), lds, s, A);
-    else hipLaunchKernelGGL((k_replay<PREC, 16>), dim3(grid), dim3(block), lds, s, A);
+    hipLaunchKernelGGL((k_replay<PREC>), dim3(grid), dim3(REPLAY_BLOCK), lds, s, A);
 }
 
 hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s)
 {
-    const int block = 256;
-    const long waves = A.n_rays;
-    long grid = (waves + 3) / 4;
-    const long max_grid = 256L * 64;   // persistent-ish: waves loop over rays
+    long grid = (long(A.n_rays) + REPLAY_WAVES - 1) / REPLAY_WAVES;
+    const long max_grid = 256L * 64;   // persistent-ish: wavefronts loop over rays
     if (grid > max_grid) grid = max_grid;
     if (grid < 1) grid = 1;
     const size_t lds = size_t(3) * A.rt.n_sigma * (precision == HADROS_PREC_FP64 ? sizeof(double) : sizeof(float));
-    if (precision == HADROS_PREC_FP64) launch_p<HADROS_PREC_FP64>(A, int(grid), block, lds, s);
-    else launch_p<HADROS_PREC_FP32K>(A, int(grid), block, lds, s);
+    if (precision == HADROS_PREC_FP64) launch_p<HADROS_PREC_FP64>(A, int(grid), lds, s);
+    else launch_p<HADROS_PREC_FP32K>(A, int(grid), lds, s);
     return hipGetLastError();
 }
 
