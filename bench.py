@@ -41,10 +41,30 @@ def tracer_flops(attempts, points, nbins):
     return attempts * (6 * ALGO["F_RHS"] + ALGO["F_COMB"]) + points * (ALGO["F_PT"] + nbins * ALGO["F_BIN"])
 
 
-def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0):
+def image_errors(img, ref):
+    """Per-image error of a GPU image against the FP64 oracle's: rel(tau)
+    where tau_ref > 0, |dP|, rel(P) where P_ref > 1e-300, rel(sum I)."""
+    import numpy as np
+
+    m = ref.tau > 0
+    rt = np.abs(img.tau[m] - ref.tau[m]) / ref.tau[m] if m.any() else np.zeros(1)
+    dp = np.abs(img.p - ref.p)
+    mp = ref.p > 1e-300
+    rp = dp[mp] / ref.p[mp] if mp.any() else np.zeros(1)
+    si, sr = float(img.I.sum()), float(ref.I.sum())
+    return {"max_rel_tau": float(rt.max()), "p99_rel_tau": float(np.percentile(rt, 99)),
+            "frac_rel_tau_over_1e-6": float(np.mean(rt > 1e-6)), "max_abs_dP": float(dp.max()),
+            "max_rel_P": float(rp.max()), "rel_sum_I": abs(si - sr) / abs(sr) if sr != 0 else abs(si),
+            "mean_P": float(ref.p.mean()), "sum_I": sr,
+            "captured_mismatch": int(np.sum(img.captured != ref.captured))}
+
+
+def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0, gpu_images=None):
     """Time the FP64 CPU oracle (same algorithm, OpenMP schedule(dynamic)) on a
     band of image rows through the image centre, growing the band until it
-    takes >= min_seconds."""
+    takes >= min_seconds.  With `gpu_images` ({label: (scene, {precision:
+    Image})}) the oracle also traces each full image once and the GPU images'
+    errors against it are reported."""
     from oracle import py_oracle
     from hadros_amd import workloads
 
@@ -69,9 +89,16 @@ def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0):
         for _ in range(reps):
             py_oracle.trace_image(cam, sc, e, s, nthreads=threads, k0=k0, k1=k1)
         dt = time.perf_counter() - t0
-    return {"value": reps * (k1 - k0) / dt, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} x {k1 - k0} rays = image rows [{i0},{i0 + rows}) of the {nx}x{nx} camera, "
-                      f"FP64 oracle (hadros_oracle, CFG mode), {dt:.1f} s"}
+    out = {"value": reps * (k1 - k0) / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} x {k1 - k0} rays = image rows [{i0},{i0 + rows}) of the {nx}x{nx} camera, "
+                     f"FP64 oracle (hadros_oracle, CFG mode), {dt:.1f} s"}
+    if gpu_images:
+        errs = {}
+        for label, (scene, imgs) in gpu_images.items():
+            ref, _ = py_oracle.trace_image(cam, scene, e, s, nthreads=threads)
+            errs[label] = {prec: image_errors(img, ref) for prec, img in imgs.items()}
+        out["errors_vs_oracle"] = errs
+    return out
 
 
 def batched_leg(e, s):
@@ -265,6 +292,14 @@ def main():
     achieved = flops / (kavg * 1e-3) / 1e12
 
     wall32, kern32, img32, st32 = timed(abi.PREC_FP32K)
+    # the same camera through a medium 1e-8 as dense (rho0 100 g/cm^3): the
+    # 512x512 workload's medium is opaque at every pixel, this one is not
+    thin = workloads.c2_scene()
+    thin.medium.rho0 *= 1e-8
+    thin.medium.envelope_rho0 *= 1e-8
+    thin_imgs = {}
+    for name, prec in (("fp64", abi.PREC_FP64), ("fp32k", abi.PREC_FP32K)):
+        thin_imgs[name], _ = ffi.trace_rt(cam, thin, e, s, prec)
     m = (img.tau > 0) & (img.tau < 700)
     err_tau = float(np.max(np.abs(img32.tau[m] - img.tau[m]) / img.tau[m])) if m.any() else 0.0
     err_p = float(np.max(np.abs(img32.p - img.p)))
@@ -310,7 +345,9 @@ def main():
             out["roofline"]["traffic"] = t["bytes_per_launch"]
             out["roofline"]["traffic_source"] = t["source"]
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds,
+                                           {"workload": (sc, {"fp64": img, "fp32k": img32}),
+                                            "rho0_x1e-8": (thin, thin_imgs)})
     print(json.dumps(out), flush=True)
 
 

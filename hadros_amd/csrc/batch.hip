@@ -210,20 +210,7 @@ k_trace_batch(BatchArgs A)
                         if constexpr (PREC == HADROS_PREC_FP64) {
                             point_rt64<NBMAX>(P, q, lnE, lnS, slope, tau, I);
                         } else {
-                            const float lnz = float(q.lnz);
-                            const float lP = (q.g3Adl > 0.0) ? float(log(q.g3Adl)) : 0.0f;
-                            const float lN = (q.nbar_dl > 0.0) ? float(log(q.nbar_dl)) : 0.0f;
-                            const float gam = float(P.gam), invcut = float(P.invcut);
-#pragma unroll
-                            for (int b = 0; b < NBMAX; ++b) {
-                                if (b >= P.nb) continue;
-                                const double E = P.emid[b] * q.z;
-                                const float lnEb = float(P.lnemid[b]) + lnz;
-                                if (P.spec_ok && q.g3Adl > 0.0)
-                                    kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
-                                if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                                    ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d))));
-                            }
+                            point_rt32<NBMAX>(P, q, lnEf, lnSf, slopef, n, ktau, kI);
                         }
                     }
                 }

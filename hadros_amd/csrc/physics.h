@@ -606,4 +606,27 @@ struct ScaledKahan {
     }
 };
 
+
+// FP32-Kahan transfer of one path point (the precision form): dtau and the
+// emission term in FP32 from FP64 per-point quantities; I kept in log form
+// with a per-bin power-of-e offset.
+template <int NBMAX>
+__device__ __forceinline__ void point_rt32(const DevRt& P, const PointQ& q, const float* lnEf, const float* lnSf,
+                                           const float* slopef, int n, Kahan* ktau, ScaledKahan* kI)
+{
+    const float lnz = float(q.lnz);
+    const float lP = (q.g3Adl > 0.0) ? float(log(q.g3Adl)) : 0.0f;
+    const float lN = (q.nbar_dl > 0.0) ? float(log(q.nbar_dl)) : 0.0f;
+    const float gam = float(P.gam), invcut = float(P.invcut);
+#pragma unroll
+    for (int b = 0; b < NBMAX; ++b) {
+        if (b >= P.nb) continue;
+        const double E = P.emid[b] * q.z;
+        const float lnEb = float(P.lnemid[b]) + lnz;
+        if (P.spec_ok && q.g3Adl > 0.0) kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
+        if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
+            ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d))));
+    }
+}
+
 }  // namespace hd

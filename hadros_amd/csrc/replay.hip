@@ -92,7 +92,7 @@ __device__ __forceinline__ double lane63(double v)
 __device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 
 #ifndef HD_REPLAY_WAVES_PER_EU
-#define HD_REPLAY_WAVES_PER_EU 4
+#define HD_REPLAY_WAVES_PER_EU 3
 #endif
 
 constexpr int REPLAY_BLOCK = 256;

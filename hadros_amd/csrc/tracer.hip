@@ -162,20 +162,7 @@ k_trace_rt(TraceArgs A)
             if constexpr (PREC == HADROS_PREC_FP64) {
                 point_rt64<NBMAX>(Pq, q, lnE, lnS, slope, tau, I);
             } else {
-                const float lnz = float(q.lnz);
-                const float lP = (q.g3Adl > 0.0) ? float(log(q.g3Adl)) : 0.0f;
-                const float lN = (q.nbar_dl > 0.0) ? float(log(q.nbar_dl)) : 0.0f;
-                const float gam = float(Pq.gam), invcut = float(Pq.invcut);
-#pragma unroll
-                for (int b = 0; b < NBMAX; ++b) {
-                    if (b >= Pq.nb) continue;
-                    const double E = Pq.emid[b] * q.z;
-                    const float lnEb = float(Pq.lnemid[b]) + lnz;
-                    if (Pq.spec_ok && q.g3Adl > 0.0)
-                        kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
-                    if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= Pq.e_lo && E <= Pq.e_hi)
-                        ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(Pq.sig_lo), float(Pq.sig_inv_d))));
-                }
+                point_rt32<NBMAX>(Pq, q, lnEf, lnSf, slopef, n, ktau, kI);
             }
         }
     }

@@ -14,7 +14,7 @@
  * Pinning (what checks each part; details in tests/):
  *   metric / inverse metric / lapse / omega / horizon:
  *       known answers printed by the reference's apps/test_kerr_metric.cpp
- *       (tests/golden/ref_points.json) + closed-form identities g.g^-1 = 1.
+ *       (tests/golden/known_answers.json, pointwise.npz) + g.g^-1 = 1.
  *   rhs (REF, finite differences): reference harness vectors (bit-exact) and
  *       the analytic derivative (agreement to FD truncation error).
  *   rhs (CFG, analytic): central differences of the Hamiltonian, H = 0 and
