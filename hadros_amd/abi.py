@@ -197,3 +197,20 @@ def source_params(so):
 
 def spectrum_params(sp):
     return [sp.gamma, sp.ecut_GeV, sp.e_min_GeV, sp.e_max_GeV, float(sp.n_bins)]
+
+
+def opacity_args(argv):
+    """argv 1-19 of apps/extract_opacity_surface.cpp:72-93 (strings, numbers or
+    defaults for missing entries) -> (Medium, enu_GeV, m_bh_msun, sigma_path,
+    tau_surface, ntheta, nr)."""
+    dflt = ["1e5", "3", "data/sigma/sigma_nuN_CC_GBW.dat", "gaussian", "1e-2", "10", "5", "0.25", "2", "0",
+            "15", "0", "2.5", "4", "60", "1e-99", "1", "181", "1200"]
+    a = [str(x) for x in argv] + dflt[len(argv):]
+    m = Medium()
+    m.kind = PROFILE_KINDS[a[3]]
+    m.rho0, m.r0, m.sigma_r, m.h_over_r = float(a[4]), float(a[5]), float(a[6]), float(a[7])
+    m.radial_power, m.funnel_depletion = float(a[8]), float(a[9])
+    m.funnel_theta_rad = float(a[10]) * math.pi / 180.0
+    m.envelope_rho0, m.envelope_alpha = float(a[11]), float(a[12])
+    m.r_min, m.r_max, m.rho_floor = float(a[13]), float(a[14]), float(a[15])
+    return m, float(a[0]), float(a[1]), a[2], float(a[16]), int(a[17]), int(a[18])

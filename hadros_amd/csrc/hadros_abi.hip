@@ -1234,6 +1234,68 @@ int hadros_trace_cache(const hadros_camera* cam, int64_t* offset, int32_t* captu
 
 }  // extern "C"
 
+// ---------------------------------------------------------------- opacity surfaces
+extern "C" int hadros_opacity_surface(const hadros_medium* m, const double* se, const double* ss, int n_sigma,
+                                      double enu_GeV, double m_bh_msun, double tau_surface, int nr, int ntheta,
+                                      double* theta, double* r_tau, double* tau_total, double* classification,
+                                      int* crossing)
+{
+    if (!m || !theta || !r_tau || !tau_total || !classification || !crossing)
+        return fail(HADROS_ERR_ARG, "NULL argument");
+    if (ntheta < 2 || nr < 4 || m->r_max <= m->r_min)
+        return fail(HADROS_ERR_ARG, "Invalid opacity-surface grid parameters.");
+    if (m->kind < 0 || m->kind > 7) return fail(HADROS_ERR_KIND, "unknown density profile kind");
+    int rc = check_sigma(se, ss, n_sigma);
+    if (rc) return rc;
+    if (enu_GeV < se[0] || enu_GeV > se[n_sigma - 1]) return fail(HADROS_ERR_RANGE, "Energy outside sigma table range.");
+    if ((rc = ensure_device())) return rc;
+    std::vector<double> sd;
+    std::vector<float> sf;
+    sigma_tables(se, ss, n_sigma, sd, sf);
+    OpacityArgs A{};
+    A.m = make_medium(*m);
+    A.n_sigma = n_sigma;
+    A.nr = nr;
+    A.ntheta = ntheta;
+    sigma_guess(se, n_sigma, A.sig_lo, A.sig_inv_d);
+    A.ln_enu = std::log(enu_GeV);
+    A.rg_cm = G_CGS * m_bh_msun * MSUN_G / (C_CGS * C_CGS);   // include/constants.hpp:17-20
+    A.r_min = m->r_min;
+    A.dr = (m->r_max - m->r_min) / static_cast<double>(nr - 1);
+    A.tau_s = tau_surface;
+    double* d_sig = nullptr;
+    double* d_out = nullptr;   // theta, r_tau, tau_total, cls
+    int* d_cross = nullptr;
+    auto cleanup = [&]() {
+        dfree(d_sig);
+        dfree(d_out);
+        dfree(d_cross);
+    };
+    const size_t nt = size_t(ntheta);
+    if ((rc = dalloc(&d_sig, sd.size())) || (rc = dalloc(&d_out, 4 * nt)) || (rc = dalloc(&d_cross, nt))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpy(d_sig, sd.data(), sd.size() * 8, hipMemcpyHostToDevice);
+    A.sig = d_sig;
+    A.theta = d_out;
+    A.r_tau = d_out + nt;
+    A.tau_total = d_out + 2 * nt;
+    A.cls = d_out + 3 * nt;
+    A.crossing = d_cross;
+    if (e == hipSuccess) e = launch_opacity_surface(A, nullptr);
+    std::vector<double> h(4 * nt);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), d_out, 4 * nt * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(crossing, d_cross, nt * sizeof(int), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(HADROS_ERR_DEVICE, hipGetErrorString(e));
+    std::memcpy(theta, h.data(), nt * 8);
+    std::memcpy(r_tau, h.data() + nt, nt * 8);
+    std::memcpy(tau_total, h.data() + 2 * nt, nt * 8);
+    std::memcpy(classification, h.data() + 3 * nt, nt * 8);
+    return HADROS_OK;
+}
+
 // ---------------------------------------------------------------- point-wise hooks
 namespace {
 template <class F>

@@ -40,6 +40,16 @@ struct ReplayArgs {
     unsigned long long* counters;
 };
 
+struct OpacityArgs {
+    DevMedium m;
+    const double* sig;        // 3*n_sigma: ln E, ln sigma, slope
+    int n_sigma, nr, ntheta, pad;
+    double sig_lo, sig_inv_d; // index guess of the sigma lookup
+    double ln_enu, rg_cm, r_min, dr, tau_s;
+    double *theta, *r_tau, *tau_total, *cls;
+    int* crossing;
+};
+
 // batches of up to this many cameras keep their camera start table in LDS
 constexpr int BATCH_LDS_CAMS = 1024;
 
@@ -83,6 +93,7 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStream_t s);
 hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s);
 hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s);
+hipError_t launch_opacity_surface(const OpacityArgs& A, hipStream_t s);
 hipError_t launch_trace_batch(const BatchArgs& A, int mode, int precision, int nb, int grid, int block, hipStream_t s);
 hipError_t batch_resident_blocks(int mode, int precision, int nb, int block, int n_sigma, int n_cams, int* blocks_per_cu);
 

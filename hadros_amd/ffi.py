@@ -36,7 +36,7 @@ EXPORTS = [
     "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
     "hadros_mlp_create", "hadros_mlp_n_params", "hadros_mlp_get_params", "hadros_mlp_set_params",
     "hadros_mlp_set_norm", "hadros_mlp_forward", "hadros_mlp_survival", "hadros_mlp_set_infer_path",
-    "hadros_mlp_grad", "hadros_mlp_train_step", "hadros_mlp_destroy",
+    "hadros_mlp_grad", "hadros_mlp_train_step", "hadros_mlp_destroy", "hadros_opacity_surface",
 ]
 
 
@@ -120,6 +120,8 @@ def lib():
     L.hadros_mlp_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, _fp]
     L.hadros_mlp_destroy.argtypes = [C.c_void_p]
     L.hadros_eval_rho.argtypes = [C.POINTER(abi.Medium), _dp, _dp, _dp, C.c_int64]
+    L.hadros_opacity_surface.argtypes = [C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip]
     L.hadros_eval_sigma.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int64]
     L.hadros_eval_emissivity.argtypes = [C.POINTER(abi.Scene), _dp, _dp, _dp, _dp, C.c_int64]
     L.hadros_eval_rhs.argtypes = [C.c_double, C.c_int, _dp, _dp, C.c_int64]
@@ -548,6 +550,19 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+def opacity_surface(medium, sig_e, sig_s, enu, m_bh, tau_surface, nr, ntheta):
+    """hadros_opacity_surface: dict of theta, r_tau, tau_total, cls (ntheta
+    doubles each) and crossing (int32)."""
+    se, ss = _sig(sig_e, sig_s)
+    o = {k: np.zeros(ntheta) for k in ("theta", "r_tau", "tau_total", "cls")}
+    cr = np.zeros(ntheta, np.int32)
+    check(lib().hadros_opacity_surface(C.byref(medium), dptr(se), dptr(ss), se.size, enu, m_bh, tau_surface, nr,
+                                       ntheta, dptr(o["theta"]), dptr(o["r_tau"]), dptr(o["tau_total"]),
+                                       dptr(o["cls"]), iptr(cr)))
+    o["crossing"] = cr
+    return o
 
 
 def eval_rho(medium, r, th):

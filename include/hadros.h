@@ -392,6 +392,21 @@ int hadros_mlp_train_step(hadros_mlp* m, const float* x, const float* y, int64_t
 int hadros_mlp_destroy(hadros_mlp* m);
 
 /* ---- point-wise device evaluations (parity hooks for each physics term) -- */
+/* Opacity surface of a density model for one neutrino energy: replaces the
+   loop of apps/extract_opacity_surface.cpp:127-206.  For theta_k =
+   pi k / (ntheta - 1), k < ntheta: tau(r) = integral of rho/m_u sigma(E) dl
+   from r to m->r_max on the uniform grid of nr nodes over [m->r_min,
+   m->r_max] (trapezoidal density, :165-174); tau_total = tau(r_min); r_tau =
+   the innermost radius where tau falls through tau_surface, linear in tau
+   between nodes, or -1 (:176-190); classification 0/1/2/3 for tau_total
+   < 0.1 / < 1 / < 3 / else (:55-68); crossing 1 if r_tau was found.  Host
+   arrays of ntheta.  HADROS_ERR_ARG for ntheta < 2, nr < 4 or r_max <= r_min
+   (:95-98), HADROS_ERR_RANGE for E outside the table (:101-104). */
+int hadros_opacity_surface(const hadros_medium* m, const double* sigma_e_gev, const double* sigma_cm2,
+                           int n_sigma, double enu_GeV, double m_bh_msun, double tau_surface, int nr,
+                           int ntheta, double* theta, double* r_tau, double* tau_total,
+                           double* classification, int* crossing);
+
 /* TorusProfile::rho (src/torus_profile.cpp:261-264), n points. */
 int hadros_eval_rho(const hadros_medium* m, const double* r, const double* th,
                     double* rho, int64_t n);

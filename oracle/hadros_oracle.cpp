@@ -952,3 +952,50 @@ int or_rt_from_cache(const hadros_cache_view* c, const hadros_scene* s, const do
 }
 
 }  // extern "C"
+
+// apps/extract_opacity_surface.cpp:127-206 (grid :127-129, tau :165-174,
+// crossing :176-190, classify_tau :55-68)
+int or_opacity_surface(const hadros_medium* m, const double* sig_e, const double* sig_s, int n_sig,
+                       double enu_GeV, double m_bh_msun, double tau_surface, int nr, int ntheta,
+                       double* theta, double* r_tau, double* tau_total, double* cls, int* crossing)
+{
+    if (ntheta < 2 || nr < 4 || m->r_max <= m->r_min) return 1;
+    if (enu_GeV < sig_e[0] || enu_GeV > sig_e[n_sig - 1]) return 2;
+    const Medium med(m);
+    const double sigma_cm2 = Sigma(sig_e, sig_s, n_sig).eval(enu_GeV);
+    const double rg_cm = G_CGS * m_bh_msun * MSUN_G / (C_CGS * C_CGS);
+    const double dr = (m->r_max - m->r_min) / static_cast<double>(nr - 1);
+    std::vector<double> rho(static_cast<size_t>(nr)), tau(static_cast<size_t>(nr));
+    for (int it = 0; it < ntheta; ++it) {
+        const double th = PI * static_cast<double>(it) / static_cast<double>(ntheta - 1);
+        for (int ir = 0; ir < nr; ++ir) rho[size_t(ir)] = med.rho(m->r_min + dr * static_cast<double>(ir), th);
+        tau[size_t(nr - 1)] = 0.0;
+        for (int ir = nr - 2; ir >= 0; --ir) {
+            const double rho_mid = 0.5 * (rho[size_t(ir)] + rho[size_t(ir + 1)]);
+            const double nb = rho_mid / M_U_G;
+            const double dtau = nb * sigma_cm2 * dr * rg_cm;
+            tau[size_t(ir)] = tau[size_t(ir + 1)] + dtau;
+        }
+        const double tt = tau[0];
+        double rt = -1.0;
+        int cf = 0;
+        if (tt >= tau_surface) {
+            for (int ir = 0; ir < nr - 1; ++ir) {
+                const double t0 = tau[size_t(ir)], t1 = tau[size_t(ir + 1)];
+                if (t0 >= tau_surface && t1 <= tau_surface) {
+                    const double r0 = m->r_min + dr * static_cast<double>(ir);
+                    const double frac = (tau_surface - t0) / std::max(t1 - t0, -1.0e-300);
+                    rt = r0 + std::clamp(frac, 0.0, 1.0) * dr;
+                    cf = 1;
+                    break;
+                }
+            }
+        }
+        theta[it] = th;
+        r_tau[it] = rt;
+        tau_total[it] = tt;
+        cls[it] = tt < 0.1 ? 0.0 : tt < 1.0 ? 1.0 : tt < 3.0 ? 2.0 : 3.0;
+        crossing[it] = cf;
+    }
+    return 0;
+}

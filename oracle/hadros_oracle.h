@@ -111,6 +111,17 @@ int or_rt_from_cache(const hadros_cache_view* c, const hadros_scene* s,
                      const double* sig_e, const double* sig_s, int n_sig,
                      int nthreads, hadros_image_out* out);
 
+/* apps/extract_opacity_surface.cpp:127-206: per polar angle theta_k =
+   pi k / (ntheta - 1), the radial optical depth from r to R_max of a nu of
+   energy E on the uniform grid r_i = R_min + i dr (nr points, trapezoidal
+   density, n = rho / m_u, sigma(E) of the table), its total, the radius where
+   it crosses tau_surface (linear in tau between samples, -1 if none), the
+   classification 0/1/2/3 (tau < 0.1 / < 1 / < 3 / else) and the crossing flag.
+   Returns 0, or 1 for an invalid grid / 2 for E outside the table. */
+int or_opacity_surface(const hadros_medium* m, const double* sig_e, const double* sig_s, int n_sig,
+                       double enu_GeV, double m_bh_msun, double tau_surface, int nr, int ntheta,
+                       double* theta, double* r_tau, double* tau_total, double* cls, int* crossing);
+
 #ifdef __cplusplus
 }
 #endif

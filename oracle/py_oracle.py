@@ -60,6 +60,8 @@ def lib():
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.or_sigma.argtypes = [_dp, _dp, C.c_int, C.c_double]
         L.or_rho.argtypes = [C.POINTER(abi.Medium), C.c_double, C.c_double]
+        L.or_opacity_surface.argtypes = [C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.or_spectral_weight.argtypes = [C.POINTER(abi.Scene), C.c_double]
         L.or_source_weight.argtypes = [C.POINTER(abi.Scene), C.c_double, C.c_double]
         L.or_emissivity.argtypes = [C.POINTER(abi.Scene), C.c_double, C.c_double, C.c_double]
@@ -208,3 +210,19 @@ def integrate_ray(scene, sig_e, sig_s, a_bh, pts):
     L.or_integrate_ray(C.byref(scene), dptr(sig_e), dptr(sig_s), sig_e.size, a_bh, dptr(pts), pts.shape[0],
                        dptr(agg), dptr(bt), dptr(bp), dptr(bi))
     return agg, bt, bp, bi
+
+
+def opacity_surface(medium, sig_e, sig_s, enu, m_bh, tau_surface, nr, ntheta):
+    """Oracle opacity surface: dict of theta, r_tau, tau_total, cls, crossing."""
+    L = lib()
+    sig_e = np.ascontiguousarray(sig_e, np.float64)
+    sig_s = np.ascontiguousarray(sig_s, np.float64)
+    o = {k: np.zeros(ntheta) for k in ("theta", "r_tau", "tau_total", "cls")}
+    cr = np.zeros(ntheta, np.int32)
+    rc = L.or_opacity_surface(C.byref(medium), dptr(sig_e), dptr(sig_s), sig_e.size, enu, m_bh, tau_surface, nr,
+                              ntheta, dptr(o["theta"]), dptr(o["r_tau"]), dptr(o["tau_total"]), dptr(o["cls"]),
+                              cr.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc:
+        raise RuntimeError(f"or_opacity_surface failed: {rc}")
+    o["crossing"] = cr
+    return o

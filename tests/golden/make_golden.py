@@ -19,6 +19,10 @@ fixtures are data only: inputs and the reference's outputs.
                        compute_kerr_image_from_cache on that cache for several
                        (profile, source, spectrum) argv sets, with the header
   sigma_GBW.npz        the reference's GBW DIS table (data file)
+  opacity_surfaces.npz output files of the reference's extract_opacity_surface
+                       for several argv sets (columns + header)
+
+`python tests/golden/make_golden.py --only opacity` rewrites only the last.
 """
 import ctypes as C
 import json
@@ -264,7 +268,45 @@ def integrate_fixture(R, rays, sigma_path):
     return d
 
 
+# argv 1-19 of apps/extract_opacity_surface.cpp:72-93 (SIGMA = the GBW table)
+OPACITY_CASES = {
+    "defaults": ["1e5", "3", "SIGMA", "gaussian", "1e-2", "10", "5", "0.25", "2", "0", "15", "0", "2.5", "4", "60",
+                 "1e-99", "1", "181", "1200"],
+    "gaussian_dense": ["1e9", "3", "SIGMA", "gaussian", "1e4", "10", "3", "0.3", "2", "0", "15", "0", "2.5", "4",
+                       "60", "1e-3", "1", "91", "600"],
+    "funnel_envelope": ["1e7", "3", "SIGMA", "powerlaw_funnel_envelope", "1e3", "30", "3", "0.3", "1.5", "0.9999",
+                        "20", "10", "2.5", "3", "50", "1e-3", "1", "91", "800"],
+    "collapsar": ["1e6", "10", "SIGMA", "collapsar_ndaf_like", "1e5", "10", "5", "0.25", "2", "0.5", "15", "1",
+                  "2.5", "4", "60", "1e-6", "3", "61", "500"],
+    "powerlaw_shallow": ["1e8", "3", "SIGMA", "powerlaw_funnel", "30", "10", "5", "0.4", "1", "0.99", "25", "0",
+                         "2.5", "6", "80", "0", "0.1", "73", "400"],
+}
+
+
+def opacity_fixture(tmp):
+    d = {}
+    for name, args in OPACITY_CASES.items():
+        out = os.path.join(tmp, name + ".dat")
+        argv = [GBW if x == "SIGMA" else x for x in args] + [out]
+        run([os.path.join(REF_BIN, "extract_opacity_surface")] + argv, tmp)
+        hdr, rows = [], []
+        with open(out) as f:
+            for ln in f:
+                if ln.startswith("#"):
+                    hdr.append(ln.rstrip("\n"))
+                elif ln.strip():
+                    rows.append([float(x) for x in ln.split()])
+        d[name + "_argv"] = np.array(args)
+        d[name + "_header"] = np.array(hdr)
+        d[name + "_rows"] = np.array(rows)
+    return d
+
+
 def main():
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "opacity":
+        with tempfile.TemporaryDirectory() as tmp:
+            np.savez_compressed(os.path.join(HERE, "opacity_surfaces.npz"), **opacity_fixture(tmp))
+        return
     if not po.have_ref():
         sys.exit("oracle/_ref not built: run `make -C oracle ref` (needs /root/reference)")
     R = po.ref()
@@ -291,6 +333,7 @@ def main():
         imd = image_fixture(tmp, cache_path, sig_path)
         imd.update(integrate_fixture(R, rays, sig_path))
         np.savez_compressed(os.path.join(HERE, "image_rows.npz"), **imd)
+        np.savez_compressed(os.path.join(HERE, "opacity_surfaces.npz"), **opacity_fixture(tmp))
     for fn in sorted(os.listdir(HERE)):
         print(fn, os.path.getsize(os.path.join(HERE, fn)))
 

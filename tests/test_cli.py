@@ -90,3 +90,20 @@ def test_cli_without_gpu_fails_loudly(work):
         pytest.skip("GPU present")
     r = run("compute_kerr_geodesics", ["0.9", "60", "80", "25", "2", "2", "120", "0.1", "out.bin"], work)
     assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
+def test_opacity_surface_cli_errors(work):
+    """apps/extract_opacity_surface.cpp:95-104: invalid grid and energy outside
+    the table exit 1 with the reference's messages before any GPU use; an
+    unknown profile exits 1 too."""
+    base = ["1e5", "3", "sigma.dat", "gaussian", "1e-2", "10", "5", "0.25", "2", "0", "15", "0", "2.5", "4", "60",
+            "1e-99", "1"]
+    for grid in (["1", "1200"], ["181", "3"]):
+        r = run("extract_opacity_surface", base + grid + ["out.dat"], work)
+        assert r.returncode == 1 and "Invalid opacity-surface grid parameters." in r.stderr
+    r = run("extract_opacity_surface", base[:4] + base[4:14] + ["3"] + base[15:] + ["181", "100", "out.dat"], work)
+    assert r.returncode == 1 and "Invalid opacity-surface grid parameters." in r.stderr   # R_max <= R_min
+    r = run("extract_opacity_surface", ["1e13"] + base[1:] + ["181", "100", "out.dat"], work)
+    assert r.returncode == 1 and "Energy outside sigma table range." in r.stderr
+    r = run("extract_opacity_surface", base[:3] + ["donut"] + base[4:] + ["181", "100", "out.dat"], work)
+    assert r.returncode == 1 and "donut" in r.stderr

@@ -366,3 +366,57 @@ def test_empty_spectral_range_gives_defaults():
     img, st = po.trace_image(cam, sc, e, s)
     assert st.n_bins == -1 or st.n_bins == 0
     assert np.all(img.tau == 0) and np.all(img.p == 1) and np.all(img.I == 0)
+
+
+# ------------------------------------------------------------------ opacity surfaces
+@pytest.fixture(scope="module")
+def opac():
+    return np.load(os.path.join(GOLD, "opacity_surfaces.npz"))
+
+
+def opacity_cases(d):
+    return sorted(k[:-5] for k in d.files if k.endswith("_argv"))
+
+
+def test_opacity_surface_matches_reference_files(opac, gbw):
+    """The oracle against the output files of the reference's own
+    extract_opacity_surface (printed %.10e): every column of every case."""
+    e, s = gbw
+    for name in opacity_cases(opac):
+        m, enu, mbh, _, ts, nth, nr = abi.opacity_args(opac[name + "_argv"])
+        o = po.opacity_surface(m, e, s, enu, mbh, ts, nr, nth)
+        rows = opac[name + "_rows"]
+        assert rows.shape == (nth, 6)
+        np.testing.assert_allclose(o["theta"], rows[:, 0], rtol=1e-10, atol=0)
+        np.testing.assert_allclose(o["theta"] * 180.0 / math.pi, rows[:, 1], rtol=1e-10, atol=1e-300)
+        np.testing.assert_allclose(o["r_tau"], rows[:, 2], rtol=1e-10, atol=0)
+        np.testing.assert_allclose(o["tau_total"], rows[:, 3], rtol=1e-10, atol=0)
+        assert np.array_equal(o["cls"], rows[:, 4]) and np.array_equal(o["crossing"], rows[:, 5].astype(int))
+        hdr = dict(ln[2:].split(" ", 1) for ln in opac[name + "_header"] if " " in ln[2:])
+        assert abs(float(hdr["sigma_cm2"]) / po.lib().or_sigma(po.dptr(e), po.dptr(s), e.size, enu) - 1) < 1e-5
+
+
+def test_opacity_surface_closed_form(gbw):
+    """A floor-only medium has constant density: tau(r) = n sigma (R_max - r) r_g
+    exactly linear, so tau_total and the tau = 1 radius are closed forms.
+    The reference's interpolation divides by std::max(t1 - t0, -1e-300), which
+    is -1e-300 whenever tau falls between the nodes (t1 < t0), so the clamped
+    fraction is 1 and r_tau is the grid node just outside the crossing
+    (apps/extract_opacity_surface.cpp:183-186); the oracle keeps that."""
+    e, s = gbw
+    m, enu, mbh, _, _, _, _ = abi.opacity_args(["1e9", "3", "x", "gaussian", "0", "10", "5", "0.25", "2", "0",
+                                                "15", "0", "2.5", "4", "60", "2.0"])
+    sig = po.lib().or_sigma(po.dptr(e), po.dptr(s), e.size, 1e9)
+    rg = 6.67430e-8 * 3.0 * 1.98847e33 / 2.99792458e10 ** 2
+    k = 2.0 / 1.66053906660e-24 * sig * rg          # dtau / dr
+    o = po.opacity_surface(m, e, s, 1e9, 3.0, 1.0, 4001, 5)
+    np.testing.assert_allclose(o["tau_total"], k * 56.0, rtol=1e-12)
+    assert (o["crossing"] == 1).all()
+    r_star = 60.0 - 1.0 / k
+    dr = 56.0 / 4000
+    node_out = 4.0 + dr * math.ceil((r_star - 4.0) / dr)
+    assert node_out - dr < r_star <= node_out
+    np.testing.assert_allclose(o["r_tau"], node_out, rtol=1e-12)
+    # a surface value above the total: no crossing, r_tau = -1, classification of the total
+    o = po.opacity_surface(m, e, s, 1e9, 3.0, 2.0 * k * 56.0, 101, 3)
+    assert (o["crossing"] == 0).all() and (o["r_tau"] == -1.0).all()
