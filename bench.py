@@ -59,7 +59,7 @@ def image_errors(img, ref):
             "captured_mismatch": int(np.sum(img.captured != ref.captured))}
 
 
-def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0, gpu_images=None):
+def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0, gpu_images=None, opacity=False):
     """Time the FP64 CPU oracle (same algorithm, OpenMP schedule(dynamic)) on a
     band of image rows through the image centre, growing the band until it
     takes >= min_seconds.  With `gpu_images` ({label: (scene, {precision:
@@ -92,6 +92,14 @@ def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0, gpu_images=None):
     out = {"value": reps * (k1 - k0) / dt, "unit": "rays/s", "cores": threads, "kind": "port",
            "sample": f"{reps} x {k1 - k0} rays = image rows [{i0},{i0 + rows}) of the {nx}x{nx} camera, "
                      f"FP64 oracle (hadros_oracle, CFG mode), {dt:.1f} s"}
+    if opacity:
+        m = opacity_medium()
+        out["opacity_surface"] = {}
+        for nth, nr in OPACITY_GRIDS:
+            t0 = time.perf_counter()
+            py_oracle.opacity_surface(m, e, s, 1.0e7, 3.0, 1.0, nr, nth)
+            dt = time.perf_counter() - t0
+            out["opacity_surface"][f"{nth}x{nr}"] = {"seconds": dt, "nodes_per_s": nth * nr / dt, "cores": 1}
     if gpu_images:
         errs = {}
         for label, (scene, imgs) in gpu_images.items():
@@ -159,6 +167,36 @@ def replay_leg(cam, sc, e, s, steps=5):
         out[name] = {"kernel_ms": kms, "rays_per_s": rays / (kms * 1e-3), "achieved": alg / (kms * 1e-3) / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      "bound": "hbm", "algorithmic_bytes": alg}
+    return out
+
+
+OPACITY_GRIDS = ((181, 1200), (2048, 16384))   # (ntheta, nr): the reference default and a fine grid
+
+
+def opacity_medium():
+    """The 512x512 workload's medium, 1e-8 as dense, on [R_min, R_max] = [3, 50]."""
+    from hadros_amd import workloads
+
+    m = workloads.c2_scene().medium
+    m.rho0 *= 1e-8
+    m.envelope_rho0 *= 1e-8
+    return m
+
+
+def opacity_leg(e, s):
+    """Opacity surfaces (apps/extract_opacity_surface.cpp) at E = 1e7 GeV, tau_surface 1:
+    host-to-host call time of hadros_opacity_surface (allocation, one launch,
+    copies) per grid."""
+    from hadros_amd import ffi
+
+    m = opacity_medium()
+    out = {}
+    for nth, nr in OPACITY_GRIDS:
+        ffi.opacity_surface(m, e, s, 1.0e7, 3.0, 1.0, nr, nth)
+        t0 = time.perf_counter()
+        o = ffi.opacity_surface(m, e, s, 1.0e7, 3.0, 1.0, nr, nth)
+        dt = time.perf_counter() - t0
+        out[f"{nth}x{nr}"] = {"seconds": dt, "nodes_per_s": nth * nr / dt, "crossings": int(o["crossing"].sum())}
     return out
 
 
@@ -230,6 +268,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 256-camera leg")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate training + scan leg")
     ap.add_argument("--no-replay", action="store_true", help="skip the cache-replay leg")
+    ap.add_argument("--no-opacity", action="store_true", help="skip the opacity-surface leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-target", nargs="?", const="tracer", choices=["tracer", "replay"],
                     help="for rocprofv3 --pmc passes: run the plain tracer (or the cache replay) twice, then copy "
@@ -336,6 +375,8 @@ def main():
         out["batched"] = batched_leg(e, s)
     if not args.no_replay:
         out["replay"] = replay_leg(cam, sc, e, s)
+    if not args.no_opacity:
+        out["opacity_surface"] = opacity_leg(e, s)
     if not args.no_surrogate:
         out["surrogate_scan"] = surrogate_leg()
     tr = os.path.join(ROOT, "profiles", "traffic.json")
@@ -347,7 +388,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds,
                                            {"workload": (sc, {"fp64": img, "fp32k": img32}),
-                                            "rho0_x1e-8": (thin, thin_imgs)})
+                                            "rho0_x1e-8": (thin, thin_imgs)}, opacity=not args.no_opacity)
     print(json.dumps(out), flush=True)
 
 

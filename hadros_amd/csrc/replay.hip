@@ -12,84 +12,9 @@
 // broadcasts), run for all bins in lockstep so the bins' dependency chains
 // overlap.  HBM traffic is the algorithmic 32 B per point.
 #include "kernels.h"
+#include "wave.h"
 
 namespace hd {
-
-// DPP move of one 32-bit word; out-of-row and masked-off lanes keep `old`
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ int dpp_word(int old, int v)
-{
-    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
-}
-
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_move(double v)   // identity +0.0
-{
-    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
-    const unsigned lo = unsigned(dpp_word<CTRL, ROW_MASK>(0, int(u & 0xffffffffull)));
-    const unsigned hi = unsigned(dpp_word<CTRL, ROW_MASK>(0, int(u >> 32)));
-    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_move(float v, float identity)
-{
-    return __int_as_float(dpp_word<CTRL, ROW_MASK>(__float_as_int(identity), __float_as_int(v)));
-}
-
-// inclusive prefix sums over the 64 lanes of N independent values per lane
-template <int N>
-__device__ __forceinline__ void wave_scan_add(double (&x)[N])
-{
-#define HD_SCAN_STEP(CTRL, RM)                                            \
-    _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_move<CTRL, RM>(x[b]);
-    HD_SCAN_STEP(0x111, 0xf)   // row_shr:1
-    HD_SCAN_STEP(0x112, 0xf)   // row_shr:2
-    HD_SCAN_STEP(0x114, 0xf)   // row_shr:4
-    HD_SCAN_STEP(0x118, 0xf)   // row_shr:8
-    HD_SCAN_STEP(0x142, 0xa)   // row_bcast:15 into rows 1, 3
-    HD_SCAN_STEP(0x143, 0xc)   // row_bcast:31 into rows 2, 3
-#undef HD_SCAN_STEP
-}
-
-template <int N>
-__device__ __forceinline__ void wave_scan_add(float (&x)[N])
-{
-#define HD_SCAN_STEP(CTRL, RM)                                            \
-    _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_move<CTRL, RM>(x[b], 0.0f);
-    HD_SCAN_STEP(0x111, 0xf)
-    HD_SCAN_STEP(0x112, 0xf)
-    HD_SCAN_STEP(0x114, 0xf)
-    HD_SCAN_STEP(0x118, 0xf)
-    HD_SCAN_STEP(0x142, 0xa)
-    HD_SCAN_STEP(0x143, 0xc)
-#undef HD_SCAN_STEP
-}
-
-// inclusive prefix maxima (identity -inf)
-template <int N>
-__device__ __forceinline__ void wave_scan_max(float (&x)[N])
-{
-#define HD_SCAN_STEP(CTRL, RM)                                            \
-    _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] = fmaxf(x[b], dpp_move<CTRL, RM>(x[b], -INFINITY));
-    HD_SCAN_STEP(0x111, 0xf)
-    HD_SCAN_STEP(0x112, 0xf)
-    HD_SCAN_STEP(0x114, 0xf)
-    HD_SCAN_STEP(0x118, 0xf)
-    HD_SCAN_STEP(0x142, 0xa)
-    HD_SCAN_STEP(0x143, 0xc)
-#undef HD_SCAN_STEP
-}
-
-__device__ __forceinline__ double lane63(double v)
-{
-    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
-    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(u & 0xffffffffull), 63));
-    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(u >> 32), 63));
-    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-
-__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
 
 #ifndef HD_REPLAY_WAVES_PER_EU
 #define HD_REPLAY_WAVES_PER_EU 3
