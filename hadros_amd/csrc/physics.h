@@ -222,10 +222,13 @@ __device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double p
     const double del = r2 - 2.0 * r + a2;
     const double ra = r2 + a2;
     const double W = ra * pt + a * pph;
-    const double inv_s = 1.0 / s;
+    // one reciprocal for 1/s, 1/Sigma, 1/Delta
+    const double sd = sig * del;
+    const double inv_all = 1.0 / (s * sd);
+    const double inv_s = sd * inv_all;
+    const double inv_sig = s * del * inv_all;
     const double V = pph * inv_s + a * s * pt;
-    const double inv_sig = 1.0 / sig;
-    const double Wd = W / del;
+    const double Wd = W * (s * sig * inv_all);
     const double N = del * y.pr * y.pr + y.pth * y.pth - W * Wd + V * V;
     const double ddel = 2.0 * r - 2.0;
     const double dNr = ddel * y.pr * y.pr - 4.0 * r * pt * Wd + Wd * Wd * ddel;
@@ -319,21 +322,45 @@ __device__ __forceinline__ double err_ref(const Y5& a, const Y5& b)
 }
 
 // CFG mixed error: max |y5-y4| / (tol (1 + max(|y|,|y5|))); NaN forces rejection.
+// The largest ratio is selected by cross-multiplication (scales are positive),
+// so only the winning ratio is divided out.
 __device__ __forceinline__ double err_cfg(const Y5& y, const Y5& y4, const Y5& y5, double tol)
 {
-    double e = 0.0;
+    double nb = 0.0, db = 1.0;
+    bool bad = false;
 #define HD_E(F)                                                                          \
     {                                                                                    \
         const double sc = tol * (1.0 + fmax(fabs(y.F), fabs(y5.F)));                    \
-        const double q = fabs(y5.F - y4.F) / sc;                                         \
-        if (!(q <= e)) e = q;                                                            \
+        const double d = fabs(y5.F - y4.F);                                              \
+        bad = bad || !(d == d) || !(sc == sc);                                           \
+        if (d * db > nb * sc) {                                                          \
+            nb = d;                                                                      \
+            db = sc;                                                                     \
+        }                                                                                \
     }
     HD_E(r); HD_E(th); HD_E(ph); HD_E(pr); HD_E(pth);
 #undef HD_E
-    return e;
+    return bad ? __builtin_nan("") : nb / db;
 }
 
 constexpr double CFG_H_MIN = 1.0e-9;
+
+// err^(-1/5) for the CFG step controller.  The controller clamps 0.9 err^-0.2
+// to [0.1, 5], which only depends on err inside [1e-5, 1e6]: clamp there,
+// seed in FP32 and refine with two division-free Newton steps for y^-5 = x,
+// y <- y (6 - x y^5) / 5 (quadratic: 1e-6 -> 1e-11 -> rounding).  About
+// 25 instructions instead of an FP64 log and exp.
+__device__ __forceinline__ double inv_fifth_root(double x)
+{
+    const double xc = dclamp(x, 1.0e-5, 1.0e6);
+    double y = double(exp2f(-0.2f * log2f(float(xc))));
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double y2 = y * y;
+        y = y * (6.0 - xc * (y2 * y2 * y)) * 0.2;
+    }
+    return y;
+}
 
 // ------------------------------------------------------------------ camera
 // src/kerr_camera.cpp:14-29

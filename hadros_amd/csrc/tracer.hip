@@ -119,11 +119,11 @@ k_trace_rt(TraceArgs A)
             const double err = err_cfg(y, y4, y5, C.tol);
             acc = (err <= 1.0) || (h <= CFG_H_MIN);
             if (acc) {
-                double fac = (err > 0.0) ? 0.9 * exp(-0.2 * log(err)) : 5.0;
+                double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
                 fac = fmin(5.0, fmax(0.2, fac));
                 h = fmax(h * fac, CFG_H_MIN);
             } else {
-                double fac = (err == err) ? 0.9 * exp(-0.2 * log(err)) : 0.1;
+                double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
                 fac = fmin(0.9, fmax(0.1, fac));
                 h = fmax(h * fac, CFG_H_MIN);
                 if (attempts >= max_attempts) {
@@ -316,11 +316,11 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
             const double err = err_cfg(y, y4, y5, C.tol);
             acc = (err <= 1.0) || (h <= CFG_H_MIN);
             if (acc) {
-                double fac = (err > 0.0) ? 0.9 * exp(-0.2 * log(err)) : 5.0;
+                double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
                 fac = fmin(5.0, fmax(0.2, fac));
                 h = fmax(h * fac, CFG_H_MIN);
             } else {
-                double fac = (err == err) ? 0.9 * exp(-0.2 * log(err)) : 0.1;
+                double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
                 fac = fmin(0.9, fmax(0.1, fac));
                 h = fmax(h * fac, CFG_H_MIN);
                 if (attempts >= max_attempts) break;
