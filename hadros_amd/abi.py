@@ -27,6 +27,7 @@ MODE_REF = 0
 MODE_CFG = 1
 PREC_FP64 = 0
 PREC_FP32K = 1
+PATH_AUTO, PATH_FUSED, PATH_SPLIT = 0, 1, 2
 
 MAX_BINS = 16
 MAX_SIGMA = 1024

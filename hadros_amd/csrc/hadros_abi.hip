@@ -313,6 +313,15 @@ struct hadros_tracer {
     int precision = 0;
     int block = 0;
     long npix = 0;
+    // split form (geodesic kernel + transfer over the chunk pool)
+    int path = HADROS_PATH_AUTO;
+    bool split = false;        // effective choice of the last run
+    PointsArgs pargs{};
+    ReplayArgs rargs{};
+    long capacity = 0;         // chunks
+    double* d_pool = nullptr;
+    int *d_next = nullptr, *d_nst = nullptr;
+    unsigned long long* d_ctr = nullptr;
     double *d_sig = nullptr;
     float* d_sigf = nullptr;
     double *d_tau = nullptr, *d_p = nullptr, *d_I = nullptr, *d_alpha = nullptr, *d_beta = nullptr;
@@ -584,9 +593,117 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
     A.rtg = t->d_rt;
     A.out = TraceOut{t->d_tau, t->d_p, t->d_I, t->d_alpha, t->d_beta, t->d_bt, t->d_bp, t->d_bi,
                      t->d_cap, t->d_np, t->d_flags, t->d_cnt};
+    PointsArgs& Q = t->pargs;
+    Q.cam = A.cam;
+    Q.n_rays = t->npix;
+    Q.out = A.out;
+    ReplayArgs& R = t->rargs;
+    R.rt = A.rt;
+    R.sig = t->d_sig;
+    R.sigf = t->d_sigf;
+    R.n_rays = int(t->npix);
+    R.tau = t->d_tau;
+    R.p = t->d_p;
+    R.I = t->d_I;
+    R.btau = t->d_bt;
+    R.bp = t->d_bp;
+    R.bi = t->d_bi;
     *out = t;
     return HADROS_OK;
 }
+
+namespace {
+
+// pool for the split form: a first chunk per pixel plus `extra` chunks
+int tracer_pool(hadros_tracer* t, long extra)
+{
+    dfree(t->d_pool);
+    dfree(t->d_next);
+    t->capacity = 0;
+    const long cap = t->npix + extra;
+    int rc = 0;
+    if ((rc = dalloc(&t->d_pool, size_t(cap) * CHUNK_DOUBLES)) || (rc = dalloc(&t->d_next, size_t(cap)))) return rc;
+    if (!t->d_nst && (rc = dalloc(&t->d_nst, size_t(t->npix)))) return rc;
+    if (!t->d_ctr && (rc = dalloc(&t->d_ctr, 1))) return rc;
+    t->capacity = cap;
+    t->pargs.capacity = cap;
+    t->pargs.pool = t->d_pool;
+    t->pargs.next = t->d_next;
+    t->pargs.n_st = t->d_nst;
+    t->pargs.chunk_ctr = reinterpret_cast<unsigned*>(t->d_ctr);
+    t->rargs.pool = t->d_pool;
+    t->rargs.next = t->d_next;
+    t->rargs.n_st = t->d_nst;
+    return HADROS_OK;
+}
+
+// largest chunk pool of the split form; bigger images use the fused kernel
+constexpr double SPLIT_POOL_MAX_BYTES = 8.0e9;
+
+bool want_split(const hadros_tracer* t)
+{
+    if (t->path == HADROS_PATH_FUSED) return false;
+    if (t->path == HADROS_PATH_SPLIT) return true;
+    return t->args.cam.mode == HADROS_MODE_CFG &&
+           3.0 * double(t->npix) * CHUNK_DOUBLES * sizeof(double) <= SPLIT_POOL_MAX_BYTES;
+}
+
+int tracer_launch(hadros_tracer* t, hipStream_t s)
+{
+    HIPC(launch_zero_u64(t->d_cnt, 4, s));
+    t->split = want_split(t);
+    if (t->split && !t->d_pool) {
+        const int rc = tracer_pool(t, 2 * t->npix);
+        if (rc) return rc;
+    }
+    if (t->split) HIPC(launch_zero_u64(t->d_ctr, 1, s));
+    HIPC(hipEventRecord(t->ev0, s));
+    if (t->split) {
+        HIPC(launch_trace_points(t->pargs, t->block, s));
+        HIPC(launch_replay(t->rargs, t->precision, s));
+    } else {
+        HIPC(launch_trace_rt(t->args, t->precision, t->block, s));
+    }
+    HIPC(hipEventRecord(t->ev1, s));
+    t->ran = true;
+    return HADROS_OK;
+}
+
+// after a split run: if rays ran out of chunks, grow the pool to what they
+// asked for (or give up on the split form) and run again
+int tracer_settle(hadros_tracer* t)
+{
+    HIPC(hipEventSynchronize(t->ev1));
+    // the counter holds every extra chunk the rays needed, so one retry suffices;
+    // the loop bound only guards against a pool that cannot be allocated
+    for (int attempt = 0; attempt < 3 && t->split; ++attempt) {
+        unsigned long long req = 0;
+        HIPC(hipMemcpy(&req, t->d_ctr, sizeof(req), hipMemcpyDeviceToHost));
+        const long need = t->npix + long(unsigned(req));
+        if (need <= t->capacity) return HADROS_OK;
+        const long grown = long(double(need) * 1.25);
+        const double bytes = double(grown) * CHUNK_DOUBLES * sizeof(double);
+        int rc = 0;
+        if (t->path == HADROS_PATH_SPLIT || bytes <= SPLIT_POOL_MAX_BYTES) {
+            if ((rc = tracer_pool(t, grown - t->npix))) return rc;
+        } else {
+            dfree(t->d_pool);
+            dfree(t->d_next);
+            t->capacity = 0;
+            t->path = HADROS_PATH_FUSED;
+        }
+        if ((rc = tracer_launch(t, nullptr))) return rc;
+        HIPC(hipEventSynchronize(t->ev1));
+    }
+    if (t->split) {
+        unsigned long long req = 0;
+        HIPC(hipMemcpy(&req, t->d_ctr, sizeof(req), hipMemcpyDeviceToHost));
+        if (t->npix + long(unsigned(req)) > t->capacity) return fail(HADROS_ERR_MAX_POINTS, "split tracer: chunk pool too small");
+    }
+    return HADROS_OK;
+}
+
+}  // namespace
 
 int hadros_tracer_set_block(hadros_tracer* t, int block)
 {
@@ -602,18 +719,23 @@ int hadros_tracer_run(hadros_tracer* t, void* stream)
     if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // counters cleared by a fill kernel (no memset nodes if the caller captures the stream)
-    HIPC(launch_zero_u64(t->d_cnt, 4, s));
-    HIPC(hipEventRecord(t->ev0, s));
-    HIPC(launch_trace_rt(t->args, t->precision, t->block, s));
-    HIPC(hipEventRecord(t->ev1, s));
-    t->ran = true;
+    return tracer_launch(t, s);
+}
+
+int hadros_tracer_set_path(hadros_tracer* t, int path)
+{
+    if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
+    if (path != HADROS_PATH_AUTO && path != HADROS_PATH_FUSED && path != HADROS_PATH_SPLIT)
+        return fail(HADROS_ERR_ARG, "path must be HADROS_PATH_AUTO, _FUSED or _SPLIT");
+    t->path = path;
     return HADROS_OK;
 }
 
 int hadros_tracer_last_ms(hadros_tracer* t, double* ms)
 {
     if (!t || !ms || !t->ran) return fail(HADROS_ERR_ARG, "tracer has not run");
-    HIPC(hipEventSynchronize(t->ev1));
+    const int rc = tracer_settle(t);
+    if (rc) return rc;
     float f = 0.0f;
     HIPC(hipEventElapsedTime(&f, t->ev0, t->ev1));
     *ms = f;
@@ -623,7 +745,8 @@ int hadros_tracer_last_ms(hadros_tracer* t, double* ms)
 int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* o, hadros_stats* st)
 {
     if (!t || !t->ran) return fail(HADROS_ERR_ARG, "tracer has not run");
-    HIPC(hipEventSynchronize(t->ev1));
+    const int rs = tracer_settle(t);
+    if (rs) return rs;
     const size_t np = size_t(t->npix);
     const size_t nb = size_t(std::max(t->args.rt.nb, 0));
     auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -674,6 +797,10 @@ int hadros_tracer_destroy(hadros_tracer* t)
     dfree(t->d_np);
     dfree(t->d_flags);
     dfree(t->d_cnt);
+    dfree(t->d_pool);
+    dfree(t->d_next);
+    dfree(t->d_nst);
+    dfree(t->d_ctr);
     dfree(t->d_rt);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);

@@ -20,6 +20,17 @@ struct TraceArgs {
     TraceOut out;
 };
 
+struct PointsArgs {
+    DevCam cam;
+    long n_rays;            // nx * ny: chunk k < n_rays is pixel k's first chunk
+    long capacity;          // chunks in the pool
+    double* pool;           // capacity * CHUNK_DOUBLES
+    int* next;              // [capacity]
+    int* n_st;              // [n_rays] points stored per pixel
+    unsigned* chunk_ctr;    // extra chunks requested
+    TraceOut out;           // captured, npoints, flags, alpha, beta, counters
+};
+
 struct CacheArgs {
     DevCam cam;
     long k0, k1;          // pixel range [k0, k1), pixel k = i*ny + j
@@ -29,13 +40,21 @@ struct CacheArgs {
     int* captured;        // [npix] (count pass)
 };
 
+// chunk pool of the split tracer: chunk c holds up to 64 points as
+// r[64], theta[64], dl[64], z[64] at pool + c * CHUNK_DOUBLES
+constexpr int CHUNK_DOUBLES = 256;
+enum { REPLAY_SRC_RANGES = 0, REPLAY_SRC_CHUNKS = 1 };
+
 struct ReplayArgs {
     DevRt rt;
     const double* sig;
     const float* sigf;
     int n_rays;
-    const long* offset;
+    const long* offset;                 // ranges source: [n_rays+1]
     const double *r, *th, *dl, *z;
+    const double* pool;                 // chunks source (non-null selects it)
+    const int* next;                    // [capacity] next chunk of the same ray
+    const int* n_st;                    // [n_rays] points stored per ray
     double *tau, *p, *I, *btau, *bp, *bi;
     unsigned long long* counters;
 };
@@ -91,6 +110,7 @@ int device_ready();
 
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 hipError_t launch_trace_rt(const TraceArgs& A, int precision, int block, hipStream_t s);
+hipError_t launch_trace_points(const PointsArgs& A, int block, hipStream_t s);
 hipError_t launch_trace_cache(const CacheArgs& A, bool write, hipStream_t s);
 hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s);
 hipError_t launch_opacity_surface(const OpacityArgs& A, hipStream_t s);

@@ -9,8 +9,12 @@
 // a wavefront prefix sum of dtau gives tau before every point, and a second
 // prefix sum adds the 64 emission terms (its lane 63).  The prefix sums are
 // DPP lane moves (row shifts 1, 2, 4, 8, then the row-15 and row-31
-// broadcasts), run for all bins in lockstep so the bins' dependency chains
-// overlap.  HBM traffic is the algorithmic 32 B per point.
+// broadcasts).  HBM traffic is the algorithmic 32 B per point.
+//
+// Two point sources: a cache's contiguous SoA ranges (image-from-cache), or
+// the chunk pool written by the split tracer (k_trace_points): 64-point
+// chunks of (r, theta, dl, z), the first at the ray's own index, the rest
+// linked through next[].
 #include "kernels.h"
 #include "wave.h"
 
@@ -36,7 +40,7 @@ struct ReplayAcc32 {
     ScaledKahan I[HADROS_MAX_BINS];
 };
 
-template <int PREC>
+template <int PREC, int SRC>
 __global__ void __launch_bounds__(REPLAY_BLOCK) __attribute__((amdgpu_waves_per_eu(HD_REPLAY_WAVES_PER_EU)))
 k_replay(ReplayArgs A)
 {
@@ -77,7 +81,15 @@ k_replay(ReplayArgs A)
     const long nwaves = long(gridDim.x) * REPLAY_WAVES;
     unsigned long long pts_done = 0;
     for (long ray = long(blockIdx.x) * REPLAY_WAVES + wv; ray < A.n_rays; ray += nwaves) {
-        const long b0 = A.offset[ray], b1 = A.offset[ray + 1];
+        // points of this ray: [b0, b1) of the SoA arrays, or n_st points in linked chunks
+        long b0 = 0, b1 = 0;
+        int chunk = int(ray);
+        if constexpr (SRC == REPLAY_SRC_RANGES) {
+            b0 = A.offset[ray];
+            b1 = A.offset[ray + 1];
+        } else {
+            b1 = A.n_st[ray];
+        }
         if (lane < nb) {
             a64.tau[lane] = 0.0;
             a64.I[lane] = 0.0;
@@ -86,10 +98,16 @@ k_replay(ReplayArgs A)
         }
         __builtin_amdgcn_wave_barrier();
         for (long base = b0; base < b1; base += 64) {
-            const long q = base + lane;
             PointQ pq;
             bool ok = false;
-            if (q < b1) ok = point_q(P, A.r[q], A.th[q], A.dl[q], A.z[q], pq);
+            if constexpr (SRC == REPLAY_SRC_RANGES) {
+                const long q = base + lane;
+                if (q < b1) ok = point_q(P, A.r[q], A.th[q], A.dl[q], A.z[q], pq);
+            } else {
+                const double* c = A.pool + long(chunk) * CHUNK_DOUBLES;
+                if (base + lane < b1) ok = point_q(P, c[lane], c[64 + lane], c[128 + lane], c[192 + lane], pq);
+                if (base + 64 < b1) chunk = A.next[chunk];
+            }
             if (!ok) {
                 pq.z = 1.0;
                 pq.lnz = 0.0;
@@ -205,10 +223,10 @@ k_replay(ReplayArgs A)
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);
 }
 
-template <int PREC>
+template <int PREC, int SRC>
 static void launch_p(const ReplayArgs& A, int grid, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_replay<PREC>), dim3(grid), dim3(REPLAY_BLOCK), lds, s, A);
+    hipLaunchKernelGGL((k_replay<PREC, SRC>), dim3(grid), dim3(REPLAY_BLOCK), lds, s, A);
 }
 
 hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s)
@@ -218,8 +236,14 @@ hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s)
     if (grid > max_grid) grid = max_grid;
     if (grid < 1) grid = 1;
     const size_t lds = size_t(3) * A.rt.n_sigma * (precision == HADROS_PREC_FP64 ? sizeof(double) : sizeof(float));
-    if (precision == HADROS_PREC_FP64) launch_p<HADROS_PREC_FP64>(A, int(grid), lds, s);
-    else launch_p<HADROS_PREC_FP32K>(A, int(grid), lds, s);
+    const bool chunks = A.pool != nullptr;
+    if (precision == HADROS_PREC_FP64) {
+        if (chunks) launch_p<HADROS_PREC_FP64, REPLAY_SRC_CHUNKS>(A, int(grid), lds, s);
+        else launch_p<HADROS_PREC_FP64, REPLAY_SRC_RANGES>(A, int(grid), lds, s);
+    } else {
+        if (chunks) launch_p<HADROS_PREC_FP32K, REPLAY_SRC_CHUNKS>(A, int(grid), lds, s);
+        else launch_p<HADROS_PREC_FP32K, REPLAY_SRC_RANGES>(A, int(grid), lds, s);
+    }
     return hipGetLastError();
 }
 

@@ -351,6 +351,178 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
 }
 
+// ---- split form, geodesic half: the same trace as k_trace_rt, with every
+// emitted point's (r, theta, dl, redshift) stored in the chunk pool for
+// k_replay instead of being integrated here.  Without the per-point transfer
+// in the loop the kernel needs about half the registers of k_trace_rt, so
+// twice the wavefronts stay resident.  Pixel k's first chunk is chunk k;
+// further chunks come from a global counter (beyond n_rays) and are linked
+// through next[].  A ray whose chunk lies beyond the pool stops storing
+// (the host sees the counter, grows the pool and runs again).
+#ifndef HD_POINTS_WAVES_PER_EU
+#define HD_POINTS_WAVES_PER_EU 4
+#endif
+
+template <int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HD_POINTS_WAVES_PER_EU)))
+k_trace_points(PointsArgs A)
+{
+    const DevCam& C = A.cam;
+    const int lane = threadIdx.x & 63;
+    const long gw = (long(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int tiles_y = (C.ny + 7) >> 3;
+    const int ti = int(gw / tiles_y), tj = int(gw % tiles_y);
+    const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
+    const bool valid = (i < C.nx) && (j < C.ny) && (ti * 8 < C.nx);
+    const double a = C.a;
+    const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
+    const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;
+    const double norm = sqrt(1.0 + u * u + v * v);
+    const double n_r = -1.0 / norm, n_th = v / norm, n_ph = u / norm;
+    const double pt_c = 1.0 / C.alpha;
+    const double pr_c = n_r / sqrt(C.g_rr);
+    const double pth_c = n_th / sqrt(C.g_thth);
+    const double pph_c = n_ph / sqrt(C.g_phph) + C.omega * pt_c;
+    const double pt = C.g_tt * pt_c + C.g_tph * pph_c;
+    const double pph = C.g_tph * pt_c + C.g_phph * pph_c;
+    Y5 y{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
+    const double e_obs_c = dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);
+    const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
+    const long k = long(i) * C.ny + j;
+
+    long cur = valid ? k : -1;   // chunk being filled
+    int fill = 0, stored = 0, emitted = 0, requested = 0;
+    bool full = false;
+    auto store = [&](double r, double th, double dl, double z) {
+        ++emitted;
+        if (full) return;
+        if (fill == 64) {
+            ++requested;
+            const long c = A.n_rays + long(atomicAdd(A.chunk_ctr, 1u));
+            if (c >= A.capacity) {
+                full = true;
+                return;
+            }
+            A.next[cur] = int(c);
+            cur = c;
+            fill = 0;
+        }
+        double* o = A.pool + cur * CHUNK_DOUBLES;
+        o[fill] = r;
+        o[64 + fill] = th;
+        o[128 + fill] = dl;
+        o[192 + fill] = z;
+        ++fill;
+        ++stored;
+    };
+
+    int step = 0, captured = 0, exhausted = 0, npts = 0;
+    long attempts = 0;
+    bool done = !valid;
+    // step 0: termination checks, then the observer point (dl = 0: nothing to store)
+    if (!done) {
+        if ((MODE == HADROS_MODE_CFG) ? (y.r < r_stop) : (y.r <= r_stop)) {
+            captured = 1;
+            done = true;
+        } else {
+            npts = 1;
+        }
+    }
+    Y5 yp = y;
+    double h = C.h;
+    const double h_min = C.h * 1.0e-5;
+    int att_in_step = 0;
+    const long max_attempts = 8L * C.max_steps + 64;
+
+    while (!done) {
+        Y5 y4, y5;
+        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        ++attempts;
+        bool acc;
+        if constexpr (MODE == HADROS_MODE_REF) {
+            const double err = err_ref(y4, y5);
+            acc = (err < C.tol) || (h <= h_min);
+            if (!acc) {
+                const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));   // x^(1/4)
+                h *= dclamp(factor, 0.1, 5.0);
+                if (h < h_min) h = h_min;
+                if (++att_in_step == 50) {
+                    y5 = rk4_ref(a, y, pt, pph, C.h);
+                    acc = true;
+                }
+            }
+            if (acc) {
+                h = C.h;
+                att_in_step = 0;
+            }
+        } else {
+            const double err = err_cfg(y, y4, y5, C.tol);
+            acc = (err <= 1.0) || (h <= CFG_H_MIN);
+            if (acc) {
+                double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
+                fac = fmin(5.0, fmax(0.2, fac));
+                h = fmax(h * fac, CFG_H_MIN);
+            } else {
+                double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
+                fac = fmin(0.9, fmax(0.1, fac));
+                h = fmax(h * fac, CFG_H_MIN);
+                if (attempts >= max_attempts) {
+                    exhausted = 1;
+                    break;
+                }
+            }
+        }
+        if (!acc) continue;
+        yp = y;
+        y = y5;
+        ++step;
+        if (step >= C.max_steps) {
+            exhausted = 1;
+            break;
+        }
+        if ((MODE == HADROS_MODE_CFG) ? (y.r < r_stop) : (y.r <= r_stop)) {
+            captured = 1;
+            break;
+        }
+        if ((MODE == HADROS_MODE_CFG) ? (y.r > C.r_max) : (y.r >= C.r_max)) {
+            if (step > 10) break;
+        }
+        // ---- emit path point `step` (src/kerr_camera.cpp:236-273)
+        ++npts;
+        store(y.r, y.th, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+    }
+
+    const unsigned long long c_pts = valid ? (unsigned long long)npts : 0ull;
+    const unsigned long long c_att = valid ? (unsigned long long)attempts : 0ull;
+    const unsigned long long c_cap = (valid && captured) ? 1ull : 0ull;
+    const unsigned long long c_ex = (valid && exhausted) ? 1ull : 0ull;
+    if (full) {
+        // count every chunk this ray needed, so the counter tells the host the pool size to retry with
+        const int need = (emitted + 63) / 64 - 1;
+        if (need > requested) atomicAdd(A.chunk_ctr, unsigned(need - requested));
+    }
+    if (valid) {
+        A.n_st[k] = stored;
+        const TraceOut& O = A.out;
+        if (O.captured) O.captured[k] = captured;
+        if (O.npoints) O.npoints[k] = npts;
+        if (O.flags) O.flags[k] = exhausted ? HADROS_FLAG_EXHAUSTED : 0;
+        if (O.alpha) O.alpha[k] = u;
+        if (O.beta) O.beta[k] = v;
+    }
+    wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
+}
+
+hipError_t launch_trace_points(const PointsArgs& A, int block, hipStream_t s)
+{
+    const int tiles = ((A.cam.nx + 7) / 8) * ((A.cam.ny + 7) / 8);
+    const long threads = long(tiles) * 64;
+    const int grid = int((threads + block - 1) / block);
+    if (A.cam.mode == HADROS_MODE_CFG) hipLaunchKernelGGL(k_trace_points<HADROS_MODE_CFG>, dim3(grid), dim3(block), 0, s, A);
+    else hipLaunchKernelGGL(k_trace_points<HADROS_MODE_REF>, dim3(grid), dim3(block), 0, s, A);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);

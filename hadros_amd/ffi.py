@@ -26,7 +26,7 @@ EXPORTS = [
     "hadros_last_error", "hadros_abi_version", "hadros_device_info", "hadros_camera_defaults",
     "hadros_scene_defaults", "hadros_parse_profile", "hadros_parse_source", "hadros_parse_spectrum",
     "hadros_profile_name", "hadros_source_name", "hadros_spectrum_name", "hadros_sigma_load",
-    "hadros_sigma_check", "hadros_trace_rt", "hadros_tracer_create", "hadros_tracer_set_block",
+    "hadros_sigma_check", "hadros_trace_rt", "hadros_tracer_create", "hadros_tracer_set_block", "hadros_tracer_set_path",
     "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_fetch", "hadros_tracer_destroy",
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
@@ -78,6 +78,7 @@ def lib():
     L.hadros_tracer_create.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
                                        C.POINTER(C.c_void_p)]
     L.hadros_tracer_set_block.argtypes = [C.c_void_p, C.c_int]
+    L.hadros_tracer_set_path.argtypes = [C.c_void_p, C.c_int]
     L.hadros_tracer_run.argtypes = [C.c_void_p, C.c_void_p]
     L.hadros_tracer_last_ms.argtypes = [C.c_void_p, _dp]
     L.hadros_tracer_fetch.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
@@ -205,7 +206,7 @@ def _sig(sig_e, sig_s):
 class Tracer:
     """Re-usable plan of hadros_trace_rt (device buffers allocated once)."""
 
-    def __init__(self, cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0):
+    def __init__(self, cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0, path=abi.PATH_AUTO):
         L = lib()
         self._se, self._ss = _sig(sig_e, sig_s)
         self.cam = cam
@@ -215,6 +216,8 @@ class Tracer:
         self.h = h
         if block:
             check(L.hadros_tracer_set_block(self.h, block))
+        if path != abi.PATH_AUTO:
+            check(L.hadros_tracer_set_path(self.h, path))
 
     def run(self, stream=None):
         check(lib().hadros_tracer_run(self.h, stream))
@@ -244,8 +247,8 @@ class Tracer:
             pass
 
 
-def trace_rt(cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0):
-    t = Tracer(cam, scene, sig_e, sig_s, precision, block)
+def trace_rt(cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0, path=abi.PATH_AUTO):
+    t = Tracer(cam, scene, sig_e, sig_s, precision, block, path)
     try:
         t.run()
         return t.fetch()

@@ -270,6 +270,16 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene,
                          int precision, hadros_tracer** out);
 /* block_size 0 = autotuned default; must be a multiple of 64, <= 1024. */
 int hadros_tracer_set_block(hadros_tracer* t, int block_size);
+/* How a plan runs (no reference counterpart; results agree to rounding):
+   HADROS_PATH_FUSED: one kernel integrates the transfer at every path point
+   (the form the batch kernel shares bit for bit);
+   HADROS_PATH_SPLIT: a geodesic kernel stores the path points in a device
+   chunk pool, then the cache-replay kernel integrates them (fewer registers
+   per kernel, more resident wavefronts);
+   HADROS_PATH_AUTO (default): split in CFG mode while the pool stays under
+   8 GB, fused otherwise.  The split form's pool grows on demand. */
+enum { HADROS_PATH_AUTO = 0, HADROS_PATH_FUSED = 1, HADROS_PATH_SPLIT = 2 };
+int hadros_tracer_set_path(hadros_tracer* t, int path);
 int hadros_tracer_run(hadros_tracer* t, void* stream);
 /* Device time (ms) of the last run's kernel, measured with HIP events on the
    launch stream; waits for the run to finish. */

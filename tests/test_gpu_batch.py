@@ -43,7 +43,7 @@ def test_batch_equals_plain_bitwise(precision):
         st = b.stats()
         tot = 0
         for c, (cam, sc) in enumerate(zip(cams, scenes)):
-            img, pst = ffi.trace_rt(cam, sc, E, S, precision)
+            img, pst = ffi.trace_rt(cam, sc, E, S, precision, path=abi.PATH_FUSED)
             got = b.fetch(c, pst.n_bins)
             for f in ("tau", "p", "I", "captured", "npoints", "flags", "bin_tau", "bin_p", "bin_i"):
                 assert same(getattr(got, f), getattr(img, f)), (block, spc, c, f)

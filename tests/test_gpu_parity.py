@@ -117,10 +117,14 @@ def _compare_images(img, ref, tol_p=1e-6, tol_i=1e-6, min_match=0.99):
     return dp.max(), abs(si - sr) / max(abs(sr), 1e-300)
 
 
+PATHS = [("fused", abi.PATH_FUSED), ("split", abi.PATH_SPLIT)]
+
+
+@pytest.mark.parametrize("path", [p for _, p in PATHS], ids=[n for n, _ in PATHS])
 @pytest.mark.parametrize("name,scene", parity_scenes())
-def test_trace_cfg_fp64(name, scene):
+def test_trace_cfg_fp64(name, scene, path):
     cam = cfg_camera(nx=20, ny=20, a=0.95, theta_deg=70.0)
-    img, st = ffi.trace_rt(cam, scene, SIG_E, SIG_S, abi.PREC_FP64)
+    img, st = ffi.trace_rt(cam, scene, SIG_E, SIG_S, abi.PREC_FP64, path=path)
     ref, rst = po.trace_image(cam, scene, SIG_E, SIG_S)
     print(name, _compare_images(img, ref), st.total_points, rst.total_points, st.kernel_ms)
     assert st.n_bins == rst.n_bins
@@ -141,18 +145,20 @@ def test_trace_workloads_fp64():
         assert st.total_points == rst.total_points or abs(st.total_points - rst.total_points) < 0.01 * rst.total_points
 
 
+@pytest.mark.parametrize("path", [p for _, p in PATHS], ids=[n for n, _ in PATHS])
 @pytest.mark.parametrize("name,scene", parity_scenes()[:4])
-def test_trace_ref_fp64(name, scene):
+def test_trace_ref_fp64(name, scene, path):
     cam = small_ref_camera(nx=6, ny=6)
-    img, st = ffi.trace_rt(cam, scene, SIG_E, SIG_S, abi.PREC_FP64)
+    img, st = ffi.trace_rt(cam, scene, SIG_E, SIG_S, abi.PREC_FP64, path=path)
     ref, rst = po.trace_image(cam, scene, SIG_E, SIG_S)
     print(name, _compare_images(img, ref, min_match=0.9), st.total_points, rst.total_points)
 
 
+@pytest.mark.parametrize("path", [p for _, p in PATHS], ids=[n for n, _ in PATHS])
 @pytest.mark.parametrize("name,scene", parity_scenes())
-def test_trace_fp32k(name, scene):
+def test_trace_fp32k(name, scene, path):
     cam = cfg_camera(nx=20, ny=20, a=0.95, theta_deg=70.0)
-    img, st = ffi.trace_rt(cam, scene, SIG_E, SIG_S, abi.PREC_FP32K)
+    img, st = ffi.trace_rt(cam, scene, SIG_E, SIG_S, abi.PREC_FP32K, path=path)
     ref, rst = po.trace_image(cam, scene, SIG_E, SIG_S)
     assert np.array_equal(img.captured, ref.captured)
     m = (ref.tau > 0) & (ref.tau < 700)
@@ -184,6 +190,28 @@ def test_replay_matches_oracle():
             mi = np.abs(ref.I) > 0
             if mi.any():
                 assert rel(img.I[mi], ref.I[mi]).max() < (1e-11 if prec == abi.PREC_FP64 else 1e-4), name
+
+
+def test_split_pool_grows_and_matches_fused():
+    """The split form starts with a pool of 3 chunks per pixel; a camera whose
+    rays need more (long REF-mode rays) triggers the grow-and-rerun path.
+    Split and fused forms integrate the same points: agreement to rounding."""
+    cam = small_ref_camera(nx=6, ny=6)
+    sc = parity_scenes()[1][1]
+    a, sa = ffi.trace_rt(cam, sc, SIG_E, SIG_S, abi.PREC_FP64, path=abi.PATH_FUSED)
+    b, sb = ffi.trace_rt(cam, sc, SIG_E, SIG_S, abi.PREC_FP64, path=abi.PATH_SPLIT)
+    assert sb.total_points / (cam.nx * cam.ny) > 3 * 64     # needed the pool to grow
+    assert np.array_equal(a.captured, b.captured) and np.array_equal(a.npoints, b.npoints)
+    assert sa.total_attempts == sb.total_attempts
+    m = a.tau > 0
+    assert rel(b.tau[m], a.tau[m]).max() < 1e-12 and np.abs(b.p - a.p).max() < 1e-12
+    assert abs(b.I.sum() - a.I.sum()) <= 1e-12 * abs(a.I.sum())
+    t = ffi.Tracer(cam, sc, SIG_E, SIG_S, abi.PREC_FP64, path=abi.PATH_SPLIT)
+    for _ in range(2):                                       # pool kept from the first run
+        t.run()
+        c, _ = t.fetch()
+        assert np.array_equal(c.p, b.p) and np.array_equal(c.tau, b.tau)
+    t.close()
 
 
 def test_trace_cache_matches_oracle_points():

@@ -134,7 +134,7 @@ def test_batch_beyond_lds_camera_table():
             got = b.fetch(c)
             cam = S.camera_for(prm["spin"][c], prm["inc"][c], 6, prm["fov_scale"][c])
             sc = S.scene_for(prm["profile"][c], prm["log_rho0"][c], prm["slope"][c])
-            img, _ = ffi.trace_rt(cam, sc, E, SIG)
+            img, _ = ffi.trace_rt(cam, sc, E, SIG, path=abi.PATH_FUSED)
             for name in ("tau", "p", "I", "captured", "npoints", "bin_tau", "bin_p", "bin_i"):
                 assert same(getattr(got, name), getattr(img, name)), (c, name)
     finally:
