@@ -567,7 +567,7 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
         return rc;
     }
     t->precision = precision;
-    t->block = 128;
+    t->block = 0;   // automatic: 128 for the fused kernel, 64 for the split geodesic kernel
     t->npix = long(cam->nx) * cam->ny;
     std::vector<double> sd;
     std::vector<float> sf;
@@ -659,10 +659,10 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
     if (t->split) HIPC(launch_zero_u64(t->d_ctr, 1, s));
     HIPC(hipEventRecord(t->ev0, s));
     if (t->split) {
-        HIPC(launch_trace_points(t->pargs, t->block, s));
+        HIPC(launch_trace_points(t->pargs, t->block ? t->block : 64, s));
         HIPC(launch_replay(t->rargs, t->precision, s));
     } else {
-        HIPC(launch_trace_rt(t->args, t->precision, t->block, s));
+        HIPC(launch_trace_rt(t->args, t->precision, t->block ? t->block : 128, s));
     }
     HIPC(hipEventRecord(t->ev1, s));
     t->ran = true;
@@ -708,8 +708,8 @@ int tracer_settle(hadros_tracer* t)
 int hadros_tracer_set_block(hadros_tracer* t, int block)
 {
     if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
-    if (block == 0) block = 128;
-    if (block < 64 || block > 256 || block % 64) return fail(HADROS_ERR_ARG, "block size must be 64, 128, 192 or 256");
+    if (block != 0 && (block < 64 || block > 256 || block % 64))
+        return fail(HADROS_ERR_ARG, "block size must be 0 (automatic), 64, 128, 192 or 256");
     t->block = block;
     return HADROS_OK;
 }
@@ -775,7 +775,7 @@ int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* o, hadros_stats* st)
         st->n_captured = int32_t(c[2]);
         st->n_exhausted = int32_t(c[3]);
         st->n_bins = t->args.rt.nb;
-        st->block_size = t->block;
+        st->block_size = t->block ? t->block : (t->split ? 64 : 128);
     }
     return HADROS_OK;
 }

@@ -354,13 +354,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 // ---- split form, geodesic half: the same trace as k_trace_rt, with every
 // emitted point's (r, theta, dl, redshift) stored in the chunk pool for
 // k_replay instead of being integrated here.  Without the per-point transfer
-// in the loop the kernel needs about half the registers of k_trace_rt, so
-// twice the wavefronts stay resident.  Pixel k's first chunk is chunk k;
+// in the loop the kernel needs 139 VGPRs instead of k_trace_rt's 256 (with
+// spills), so 3 wavefronts per SIMD stay resident instead of 2.  Pixel k's first chunk is chunk k;
 // further chunks come from a global counter (beyond n_rays) and are linked
 // through next[].  A ray whose chunk lies beyond the pool stops storing
 // (the host sees the counter, grows the pool and runs again).
 #ifndef HD_POINTS_WAVES_PER_EU
-#define HD_POINTS_WAVES_PER_EU 4
+#define HD_POINTS_WAVES_PER_EU 3
 #endif
 
 template <int MODE>

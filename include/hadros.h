@@ -268,7 +268,8 @@ typedef struct hadros_tracer hadros_tracer;
 int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene,
                          const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
                          int precision, hadros_tracer** out);
-/* block_size 0 = autotuned default; must be a multiple of 64, <= 1024. */
+/* workgroup size of the tracer kernels: 0 = the default of the running form
+   (128 fused, 64 split), else 64, 128, 192 or 256. */
 int hadros_tracer_set_block(hadros_tracer* t, int block_size);
 /* How a plan runs (no reference counterpart; results agree to rounding):
    HADROS_PATH_FUSED: one kernel integrates the transfer at every path point
