@@ -40,8 +40,9 @@ struct CacheArgs {
     int* captured;        // [npix] (count pass)
 };
 
-// chunk pool of the split tracer: chunk c holds up to 64 points as
-// r[64], theta[64], dl[64], z[64] at pool + c * CHUNK_DOUBLES
+// chunk pool of the split tracer: chunk c holds up to 64 points as 32-byte
+// records (r, theta, dl, z) at pool + c * CHUNK_DOUBLES + 4 p, so each lane's
+// stores fill whole 128-byte lines every 4 points
 constexpr int CHUNK_DOUBLES = 256;
 enum { REPLAY_SRC_RANGES = 0, REPLAY_SRC_CHUNKS = 1 };
 

@@ -13,8 +13,8 @@
 //
 // Two point sources: a cache's contiguous SoA ranges (image-from-cache), or
 // the chunk pool written by the split tracer (k_trace_points): 64-point
-// chunks of (r, theta, dl, z), the first at the ray's own index, the rest
-// linked through next[].
+// chunks of 32-byte (r, theta, dl, z) records, the first at the ray's own
+// index, the rest linked through next[].
 #include "kernels.h"
 #include "wave.h"
 
@@ -104,8 +104,11 @@ k_replay(ReplayArgs A)
                 const long q = base + lane;
                 if (q < b1) ok = point_q(P, A.r[q], A.th[q], A.dl[q], A.z[q], pq);
             } else {
-                const double* c = A.pool + long(chunk) * CHUNK_DOUBLES;
-                if (base + lane < b1) ok = point_q(P, c[lane], c[64 + lane], c[128 + lane], c[192 + lane], pq);
+                if (base + lane < b1) {
+                    const double2* c = reinterpret_cast<const double2*>(A.pool + long(chunk) * CHUNK_DOUBLES + 4 * lane);
+                    const double2 rt = c[0], dz = c[1];
+                    ok = point_q(P, rt.x, rt.y, dz.x, dz.y, pq);
+                }
                 if (base + 64 < b1) chunk = A.next[chunk];
             }
             if (!ok) {

@@ -407,11 +407,10 @@ k_trace_points(PointsArgs A)
             cur = c;
             fill = 0;
         }
-        double* o = A.pool + cur * CHUNK_DOUBLES;
-        o[fill] = r;
-        o[64 + fill] = th;
-        o[128 + fill] = dl;
-        o[192 + fill] = z;
+        // one 32-byte point record: (r, theta) and (dl, z) as two 16-byte stores
+        double2* o = reinterpret_cast<double2*>(A.pool + cur * CHUNK_DOUBLES + 4 * fill);
+        o[0] = make_double2(r, th);
+        o[1] = make_double2(dl, z);
         ++fill;
         ++stored;
     };
