@@ -89,6 +89,11 @@ DevMedium make_medium(const hadros_medium& m)
     d.rmin = m.r_min;
     d.rmax = dmax(m.r_max, m.r_min);
     d.floor_ = dmax(m.rho_floor, 0.0);
+    d.inv_r0 = 1.0 / d.r0;
+    d.inv_sr = 1.0 / d.sr;
+    d.inv_H = 1.0 / d.H;
+    d.inv_rmax = 1.0 / d.rmax;
+    d.inv_ft = 1.0 / d.ft;
     return d;
 }
 
@@ -129,9 +134,10 @@ int check_sigma(const double* e, const double* s, int n)
     return HADROS_OK;
 }
 
-// index guess of the device lookup for tables uniform in ln E (every node
-// within a quarter cell of the line through the end nodes); inv_d = 0 selects
-// the binary search
+// Index guess of the device sigma lookup: a table whose every ln E node lies
+// within d/4 of the uniform grid lo0 + k d gets inv_d = 1/d (the guess
+// ceil((ln E - lo0)/d) is then at most one node from the lower bound);
+// other tables get inv_d = 0 (binary search).
 void sigma_guess(const double* se, int n, double& lo0, double& inv_d)
 {
     const double l0 = std::log(se[0]), l1 = std::log(se[n - 1]);
@@ -167,6 +173,9 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
     P.src.cmax = so.cutoff_max;
     P.src.gdr = so.gradient_dr;
     P.src.gdth = so.gradient_dtheta_rad;
+    P.src.inv_rc = 1.0 / so.r_center;
+    P.src.inv_sr = 1.0 / so.sigma_r;
+    P.src.inv_tw = 1.0 / so.theta_width_rad;
     P.src.rho_norm_c = rho(P.med, so.r_center, 0.5 * hd::PI);
     P.n_sigma = n;
     P.e_lo = se[0];

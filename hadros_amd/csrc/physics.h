@@ -72,6 +72,7 @@ struct DevMedium {
     int kind;
     int powerlaw_disk, funnel, envelope;
     double rho0, r0, sr, H, p, fd, ft, env0, ealpha, rmin, rmax, floor_;
+    double inv_r0, inv_sr, inv_H, inv_rmax, inv_ft;   // reciprocals: the hot path multiplies
 };
 
 struct DevSource {
@@ -79,6 +80,7 @@ struct DevSource {
     int ring_ok;          // emissivity_collapsar_ring guard (src/radiative_transfer.cpp:343-349)
     double rc, sr, tw, norm, ft, q, s, rho_ref, cmin, cmax, gdr, gdth;
     double rho_norm_c;    // torus.rho(r_center, pi/2), a per-scene constant
+    double inv_rc, inv_sr, inv_tw;
 };
 
 // Spectral factor of every bin: ln S(E) = -gam ln E - E * invcut, valid if spec_ok.
@@ -407,9 +409,14 @@ __host__ __device__ __forceinline__ double cos_theta(double th)
 __host__ __device__ __forceinline__ double pow_pos(double x, double y) { return exp(y * log(x)); }
 
 // TorusProfile::rho (src/torus_profile.cpp:150-272); pow(x, 2.0) is x*x.
+// Products of exponentials are one exponential of the summed exponent, the
+// power laws share ln(r / r0), and divisions by profile parameters are
+// products with host-side reciprocals (relative differences from the
+// reference's form of order |exponent| x 1e-16).
 __host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, double th)
 {
     double shape;
+    const double lr = (m.powerlaw_disk || m.envelope) ? log(r * m.inv_r0) : 0.0;   // ln(r / r0)
     if (m.kind == HADROS_PROFILE_COLLAPSAR_NDAF_LIKE) {
         if (!(r >= m.rmin && r <= m.rmax) || r <= 0.0 || m.r0 <= 0.0) {
             shape = 0.0;
@@ -427,39 +434,38 @@ __host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, dou
         if (!(r >= m.rmin && r <= m.rmax) || r <= 0.0 || m.r0 <= 0.0) {
             shape = 0.0;
         } else {
-            const double vertical = exp(-sq(cos_theta(th) / m.H));
-            const double inner_taper = 1.0 - exp(-sq((r - m.rmin) / m.sr));
-            const double q = r / m.rmax;
-            const double outer_taper = exp(-(q * q) * (q * q));
-            const double radial = pow_pos(r / m.r0, -m.p);
-            shape = radial * vertical * inner_taper * outer_taper;
+            // radial (r/r0)^-p x vertical exp(-(cos th / H)^2) x outer exp(-(r/rmax)^4)
+            const double mu = cos_theta(th) * m.inv_H;
+            const double q = r * m.inv_rmax;
+            const double inner_taper = 1.0 - exp(-sq((r - m.rmin) * m.inv_sr));
+            shape = exp(-m.p * lr - mu * mu - (q * q) * (q * q)) * inner_taper;
         }
     } else {
         const double delta = th - 0.5 * PI;
         if (r <= 4.0 || r >= 18.0 || fabs(delta) >= 0.45) {
             shape = 0.0;
         } else {
-            shape = exp(-sq((r - m.r0) / m.sr)) * exp(-sq(delta / m.H));
+            shape = exp(-sq((r - m.r0) * m.inv_sr) - sq(delta * m.inv_H));
         }
     }
     double F = 1.0;
     if (m.funnel) {
-        const double north = exp(-sq(th / m.ft));
-        const double south = exp(-sq((PI - th) / m.ft));
+        const double north = exp(-sq(th * m.inv_ft));
+        const double south = exp(-sq((PI - th) * m.inv_ft));
         F = dclamp(1.0 - m.fd * (north + south), 0.0, 1.0);
     }
     double env = 0.0;
-    if (m.envelope && !(m.env0 <= 0.0 || r < m.r0 || r > m.rmax)) env = m.env0 * pow_pos(r / m.r0, -m.ealpha);
+    if (m.envelope && !(m.env0 <= 0.0 || r < m.r0 || r > m.rmax)) env = m.env0 * exp(-m.ealpha * lr);
     return dmax(m.rho0 * shape * F + env, m.floor_);
 }
 
-// bipolar_gaussian (src/radiative_transfer.cpp:64-76)
-__device__ __forceinline__ double bipolar(double th, double th0, double w)
+// exp(base) x bipolar_gaussian(th, th0, w) (src/radiative_transfer.cpp:64-76)
+// for w > 0, as two exponentials of summed exponents
+__device__ __forceinline__ double bipolar_times(double th, double th0, double inv_w, double base)
 {
-    if (w <= 0.0) return 0.0;
-    const double n = th - th0;
-    const double s = th - (PI - th0);
-    return exp(-(n / w) * (n / w)) + exp(-(s / w) * (s / w));
+    const double n = (th - th0) * inv_w;
+    const double s = (th - (PI - th0)) * inv_w;
+    return exp(base - n * n) + exp(base - s * s);
 }
 
 // norm x spatial factor of emissivity_uhe (src/radiative_transfer.cpp:209-395):
@@ -470,17 +476,17 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
 {
     if (so.kind == HADROS_SOURCE_INNER_RING) {
         if (!so.ring_ok) return 0.0;
-        const double dr = (r - so.rc) / so.sr;
-        const double dt = (th - 0.5 * PI) / so.tw;
-        return so.norm * exp(-dr * dr) * exp(-dt * dt);
+        const double dr = (r - so.rc) * so.inv_sr;
+        const double dt = (th - 0.5 * PI) * so.inv_tw;
+        return so.norm * exp(-dr * dr - dt * dt);
     }
     if (r <= 0.0 || so.sr <= 0.0 || so.tw <= 0.0) return 0.0;
     double W = 0.0;
     if (so.kind == HADROS_SOURCE_FUNNEL_WALL) {
-        const double dr = (r - so.rc) / so.sr;
-        W = exp(-dr * dr) * bipolar(th, so.ft, so.tw);
+        const double dr = (r - so.rc) * so.inv_sr;
+        W = bipolar_times(th, so.ft, so.inv_tw, -dr * dr);
     } else if (so.kind == HADROS_SOURCE_JET_BASE) {
-        W = exp(-sq(r / so.rc)) * bipolar(th, 0.0, so.tw);
+        W = bipolar_times(th, 0.0, so.inv_tw, -sq(r * so.inv_rc));
     } else if (so.kind == HADROS_SOURCE_DENSITY_WEIGHTED) {
         const double rh = rho_here;
         const double rho_norm = dmax(so.rho_ref > 0.0 ? so.rho_ref : so.rho_norm_c, 1.0e-300);
@@ -520,17 +526,22 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
 // lower_bound of lx in the ascending table lnE[0..n): the first k with
 // lnE[k] >= lx (std::lower_bound of src/sigma_table.cpp:55-58,108-110).  A
 // table uniform in ln E (inv_d > 0) starts from the arithmetic index guess
-// and walks it to the exact bound against the table (one or two LDS reads);
-// other tables use a binary search.
+// and moves it at most one node to the exact bound (two LDS reads); other
+// tables use a binary search.
 template <typename T>
 __device__ __forceinline__ int sigma_lower_bound(const T* __restrict__ lnE, int n, T lx, T lo0, T inv_d)
 {
     if (inv_d > T(0)) {
         const T t = (lx - lo0) * inv_d;
-        int g = !(t > T(0)) ? 0 : (t >= T(n - 1) ? n - 1 : int(ceil(t)));
-        while (g > 0 && lnE[g - 1] >= lx) --g;
-        while (g < n && lnE[g] < lx) ++g;
-        return g;
+        const int g = !(t > T(0)) ? 0 : (t >= T(n - 1) ? n - 1 : int(ceil(t)));
+        // the host accepts the guess only for tables within d/4 of uniform
+        // spacing: then the bound is g - 1, g or g + 1, decided by the two
+        // nodes around the guess (read together, no loop)
+        const T below = lnE[g > 0 ? g - 1 : 0];
+        const T at = lnE[g];
+        const bool down = g > 0 && below >= lx;
+        const bool up = !down && at < lx;
+        return g - int(down) + int(up);
     }
     int lo = 0, cnt = n;
     while (cnt > 0) {

@@ -25,6 +25,11 @@ namespace hd {
 #endif
 
 constexpr int REPLAY_BLOCK = 256;
+// e^x rounds to +0 in FP64 for every x below this (e^-750 < 2^-1081, under half
+// the smallest subnormal 2^-1074)
+constexpr double EXP64_ZERO_BELOW = -750.0;
+// log-domain FP32 emission terms below e^-1000 leave the FP64 intensity unchanged
+constexpr float EXP32_LOG_NEGLIGIBLE = -1000.0f;
 constexpr int REPLAY_WAVES = REPLAY_BLOCK / 64;
 
 // Per-wavefront, per-bin running sums of the current ray, kept in LDS so that
@@ -130,14 +135,20 @@ k_replay(ReplayArgs A)
                     wave_scan_add(inc);
                     const double tau_b = a64.tau[b];
                     const double before = tau_b + (inc[0] - dt);
+                    const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
+                    const double arg = -P.gam * lnEb - E * P.invcut - before;
+                    // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
+                    // every term is zero (behind an opaque column) adds nothing
                     double em[1] = {0.0};
-                    if (ok && P.spec_ok && pq.g3Adl != 0.0)
-                        em[0] = pq.g3Adl * exp(-P.gam * lnEb - E * P.invcut - before);
-                    wave_scan_add(em);
+                    const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
+                    if (live) {
+                        if (emits) em[0] = pq.g3Adl * exp(arg);
+                        wave_scan_add(em);
+                    }
                     __builtin_amdgcn_wave_barrier();
                     if (lane == 63) {
                         a64.tau[b] = tau_b + inc[0];
-                        a64.I[b] += em[0];
+                        if (live) a64.I[b] += em[0];
                     }
                     __builtin_amdgcn_wave_barrier();
                 }
@@ -159,11 +170,17 @@ k_replay(ReplayArgs A)
                     // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
                     float L = -INFINITY;
                     if (ok && P.spec_ok && pq.g3Adl > 0.0) L = lP - gam * lnEb - float(E) * invcut - before;
-                    float m[1] = {L};
-                    wave_scan_max(m);
-                    const float mx = lane63(m[0]);
-                    float w[1] = {(mx > -INFINITY) ? __expf(L - mx) : 0.0f};
-                    wave_scan_add(w);
+                    // terms below e^EXP32_LOG_NEGLIGIBLE vanish in the FP64 result or
+                    // beside any larger term: a chunk of only such terms is skipped
+                    const bool live = __ballot(!(L < EXP32_LOG_NEGLIGIBLE)) != 0ull;
+                    float mx = -INFINITY, w[1] = {0.0f};
+                    if (live) {
+                        float m[1] = {L};
+                        wave_scan_max(m);
+                        mx = lane63(m[0]);
+                        w[0] = (mx > -INFINITY) ? __expf(L - mx) : 0.0f;
+                        wave_scan_add(w);
+                    }
                     __builtin_amdgcn_wave_barrier();
                     if (lane == 63) {
                         if (mx > -INFINITY) a32.I[b].add_log(mx + __logf(w[0]));

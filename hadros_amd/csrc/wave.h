@@ -32,19 +32,32 @@ __device__ __forceinline__ float dpp_move(float v, float identity)
     return __int_as_float(dpp_word<CTRL, ROW_MASK>(__float_as_int(identity), __float_as_int(v)));
 }
 
+// DPP row shift of a double with every row enabled: lanes whose source is
+// outside the row read 0 (bound_ctrl), so no identity needs to be written first
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v)
+{
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned lo = unsigned(__builtin_amdgcn_mov_dpp(int(u & 0xffffffffull), CTRL, 0xf, 0xf, true));
+    const unsigned hi = unsigned(__builtin_amdgcn_mov_dpp(int(u >> 32), CTRL, 0xf, 0xf, true));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // inclusive prefix sums over the 64 lanes of N independent values per lane
 template <int N>
 __device__ __forceinline__ void wave_scan_add(double (&x)[N])
 {
+#define HD_SHIFT_STEP(CTRL) _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_shift<CTRL>(x[b]);
 #define HD_SCAN_STEP(CTRL, RM)                                            \
     _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_move<CTRL, RM>(x[b]);
-    HD_SCAN_STEP(0x111, 0xf)   // row_shr:1
-    HD_SCAN_STEP(0x112, 0xf)   // row_shr:2
-    HD_SCAN_STEP(0x114, 0xf)   // row_shr:4
-    HD_SCAN_STEP(0x118, 0xf)   // row_shr:8
+    HD_SHIFT_STEP(0x111)       // row_shr:1
+    HD_SHIFT_STEP(0x112)       // row_shr:2
+    HD_SHIFT_STEP(0x114)       // row_shr:4
+    HD_SHIFT_STEP(0x118)       // row_shr:8
     HD_SCAN_STEP(0x142, 0xa)   // row_bcast:15 into rows 1, 3
     HD_SCAN_STEP(0x143, 0xc)   // row_bcast:31 into rows 2, 3
 #undef HD_SCAN_STEP
+#undef HD_SHIFT_STEP
 }
 
 template <int N>
