@@ -11,8 +11,9 @@
 //
 // Per ray the arithmetic is the plain tracer's (same device functions, same
 // operation order), so every image equals the per-camera plain run bit for
-// bit.  Camera c owns global ray indices [pix0[c], pix0[c+1]); pixel
-// k = i*ny + j within it, as in the plain tracer.
+// bit.  Camera c owns global ray indices [pix0[c], pix0[c+1]), handed out in
+// the tile order of batch_pixel; its outputs are stored at pix0[c] + k with
+// pixel k = i*ny + j, as in the plain tracer.
 #include "kernels.h"
 
 #ifndef HD_BATCH_WAVES_PER_EU
@@ -22,7 +23,8 @@
 namespace hd {
 
 struct BatchLane {
-    long ray;        // global ray index, -1 = idle
+    long ray;        // global ray index in queue order, -1 = idle
+    long out;        // global output index of the ray's pixel
     int cam;
     int step, npts, captured, exhausted, att_in_step;
     long attempts;
@@ -31,22 +33,18 @@ struct BatchLane {
 };
 
 template <int MODE>
-__device__ __forceinline__ void batch_init_ray(const BatchArgs& A, const long* __restrict__ pix0, BatchLane& L,
-                                               long ray)
+__device__ __forceinline__ void batch_init_ray(const DevCam* __restrict__ cams, int n_cams,
+                                               const long* __restrict__ pix0, BatchLane& L, long ray)
 {
-    // camera of this ray: last c with pix0[c] <= ray
-    int lo = 0, hi = A.n_cams - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pix0[mid] <= ray) lo = mid;
-        else hi = mid - 1;
-    }
+    const int lo = batch_camera(pix0, n_cams, ray);
     L.ray = ray;
     L.cam = lo;
-    const DevCam& C = A.cams[lo];
-    const long k = ray - pix0[lo];   // index in the camera's (sub-)lattice
-    const int i = C.sub_offset + C.sub_stride * int(k / C.sub_ny);
-    const int j = C.sub_offset + C.sub_stride * int(k % C.sub_ny);
+    const DevCam& C = cams[lo];
+    int p, q;   // lattice pixel of the camera's k-th ray in queue order
+    batch_pixel(C, ray - pix0[lo], p, q);
+    L.out = pix0[lo] + long(p) * C.sub_ny + q;
+    const int i = C.sub_offset + C.sub_stride * p;
+    const int j = C.sub_offset + C.sub_stride * q;
     // src/kerr_camera.cpp:110-186 with the observer metric from the host
     const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
     const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;
@@ -124,7 +122,7 @@ k_trace_batch(BatchArgs A)
                     const unsigned long long below = idle & ((1ull << lane) - 1ull);
                     const long my = long(base) + __popcll(below);
                     if (my < A.n_rays) {
-                        batch_init_ray<MODE>(A, pix0, L, my);
+                        batch_init_ray<MODE>(A.cams, A.n_cams, pix0, L, my);
 #pragma unroll
                         for (int b = 0; b < NBMAX; ++b) {
                             tau[b] = 0.0;
@@ -222,7 +220,7 @@ k_trace_batch(BatchArgs A)
         if (finished) {
             // ---- outputs of ray L.ray (aggregation of src/radiative_transfer.cpp:711-769)
             const DevRt& P = A.rts[A.cams[L.cam].pad_rt];
-            const long k = L.ray;
+            const long k = L.out;
             double bt[NBMAX], bp[NBMAX], bi[NBMAX];
 #pragma unroll
             for (int b = 0; b < NBMAX; ++b) {
@@ -281,6 +279,172 @@ k_trace_batch(BatchArgs A)
         }
     }
     wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
+}
+
+#ifndef HD_BATCH_POINTS_WAVES_PER_EU
+#define HD_BATCH_POINTS_WAVES_PER_EU 3
+#endif
+
+// Geodesic half of the split batch (the transfer runs afterwards in k_replay
+// over the chunk pool): the same queue, refill and RKF45 stepping as
+// k_trace_batch, with each emitted point's (r, theta, dl, redshift) stored in
+// the ray's chunks instead of integrated.  Works on the sub-batch of global
+// rays [ray0, ray0 + n_rays); the queue hands out local indices.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == HADROS_MODE_CFG ? HD_BATCH_POINTS_WAVES_PER_EU : 2)))
+k_batch_points(BatchPointsArgs A)
+{
+    extern __shared__ double smem[];
+    long* pix0_lds = reinterpret_cast<long*>(smem);   // [n_cams + 1]
+    const bool pix_in_lds = A.n_cams <= BATCH_LDS_CAMS;
+    const long* pix0 = pix_in_lds ? pix0_lds : A.pix0;
+    if (pix_in_lds)
+        for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0_lds[t] = A.pix0[t];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    BatchLane L;
+    L.ray = -1;
+    ChunkWriter w;
+    w.begin(-1);
+    bool drained = false;   // wave-uniform: the queue is empty
+    unsigned long long c_pts = 0, c_att = 0, c_cap = 0, c_ex = 0;
+    int since_refill = A.steps_per_chunk;
+
+    while (true) {
+        // ---- refill finished lanes from the queue (wave-uniform control)
+        if (!drained && since_refill >= A.steps_per_chunk) {
+            since_refill = 0;
+            const unsigned long long idle = __ballot(L.ray < 0);
+            if (idle) {
+                const int cnt = __popcll(idle);
+                const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(A.queue, (unsigned long long)cnt);
+                base = __shfl(base, leader, 64);
+                if (base + cnt >= (unsigned long long)A.n_rays) drained = true;
+                if (L.ray < 0) {
+                    const unsigned long long below = idle & ((1ull << lane) - 1ull);
+                    const long my = long(base) + __popcll(below);
+                    if (my < A.n_rays) {
+                        batch_init_ray<MODE>(A.cams, A.n_cams, pix0, L, A.ray0 + my);
+                        w.begin(my);
+                        const bool cap0 = (MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop);
+                        if (cap0) L.captured = 1;
+                        else L.npts = 1;
+                    }
+                }
+            }
+        }
+        if (__ballot(L.ray >= 0) == 0ull) {
+            if (drained) break;
+            since_refill = A.steps_per_chunk;
+            continue;
+        }
+        ++since_refill;
+
+        bool finished = false;
+        if (L.ray >= 0 && !(L.captured && L.step == 0)) {
+            const DevCam& C = A.cams[L.cam];
+            const double a = C.a;
+            Y5 y4, y5;
+            rkf45<MODE>(a, L.y, L.pt, L.pph, L.h, y4, y5);
+            ++L.attempts;
+            bool acc;
+            if constexpr (MODE == HADROS_MODE_REF) {
+                const double h_min = C.h * 1.0e-5;
+                const double err = err_ref(y4, y5);
+                acc = (err < C.tol) || (L.h <= h_min);
+                if (!acc) {
+                    const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
+                    L.h *= dclamp(factor, 0.1, 5.0);
+                    if (L.h < h_min) L.h = h_min;
+                    if (++L.att_in_step == 50) {
+                        y5 = rk4_ref(a, L.y, L.pt, L.pph, C.h);
+                        acc = true;
+                    }
+                }
+                if (acc) {
+                    L.h = C.h;
+                    L.att_in_step = 0;
+                }
+            } else {
+                const double err = err_cfg(L.y, y4, y5, C.tol);
+                acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
+                if (acc) {
+                    double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
+                    fac = fmin(5.0, fmax(0.2, fac));
+                    L.h = fmax(L.h * fac, CFG_H_MIN);
+                } else {
+                    double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
+                    fac = fmin(0.9, fmax(0.1, fac));
+                    L.h = fmax(L.h * fac, CFG_H_MIN);
+                    if (L.attempts >= 8L * C.max_steps + 64) {
+                        L.exhausted = 1;
+                        finished = true;
+                    }
+                }
+            }
+            if (acc) {
+                L.yp = L.y;
+                L.y = y5;
+                ++L.step;
+                if (L.step >= C.max_steps) {
+                    L.exhausted = 1;
+                    finished = true;
+                } else if ((MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop)) {
+                    L.captured = 1;
+                    finished = true;
+                } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
+                    finished = true;
+                } else {
+                    // ---- path point L.step (src/kerr_camera.cpp:236-273)
+                    ++L.npts;
+                    w.put(A.cp, L.y.r, L.y.th, zamo_dl(a, L.y, L.yp),
+                          zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) / L.e_obs_c);
+                }
+            }
+        } else if (L.ray >= 0) {
+            finished = true;   // captured at the camera: no points
+        }
+
+        if (finished) {
+            const long g = L.out;
+            w.finish(A.cp, L.npts - 1);   // every point after the observer's was put()
+            A.n_st[L.ray - A.ray0] = w.stored;
+            const TraceOut& O = A.out;
+            if (O.captured) O.captured[g] = L.captured;
+            if (O.npoints) O.npoints[g] = L.npts;
+            if (O.flags) O.flags[g] = L.exhausted ? HADROS_FLAG_EXHAUSTED : 0;
+            c_pts += (unsigned long long)L.npts;
+            c_att += (unsigned long long)L.attempts;
+            c_cap += (unsigned long long)L.captured;
+            c_ex += (unsigned long long)L.exhausted;
+            L.ray = -1;
+        }
+    }
+    wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
+}
+
+static size_t batch_points_lds_bytes(int n_cams)
+{
+    return n_cams <= BATCH_LDS_CAMS ? size_t(n_cams + 1) * sizeof(long) : 0;
+}
+
+hipError_t batch_points_resident_blocks(int mode, int block, int n_cams, int* blocks_per_cu)
+{
+    const void* f = mode == HADROS_MODE_CFG ? reinterpret_cast<const void*>(&k_batch_points<HADROS_MODE_CFG>)
+                                            : reinterpret_cast<const void*>(&k_batch_points<HADROS_MODE_REF>);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, batch_points_lds_bytes(n_cams));
+}
+
+hipError_t launch_batch_points(const BatchPointsArgs& A, int mode, int grid, int block, hipStream_t s)
+{
+    const size_t lds = batch_points_lds_bytes(A.n_cams);
+    if (mode == HADROS_MODE_CFG) hipLaunchKernelGGL(k_batch_points<HADROS_MODE_CFG>, dim3(grid), dim3(block), lds, s, A);
+    else hipLaunchKernelGGL(k_batch_points<HADROS_MODE_REF>, dim3(grid), dim3(block), lds, s, A);
+    return hipGetLastError();
 }
 
 template <int MODE, int PREC>

@@ -372,7 +372,21 @@ struct hadros_batch {
     int *d_cap = nullptr, *d_np = nullptr, *d_flags = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ran = false;
+    // split form: sub-batches of `sub` rays, each traced into the chunk pool by
+    // k_batch_points and then integrated by k_replay
+    int path = HADROS_PATH_AUTO;
+    bool split = false;
+    long sub = 0, capacity = 0, last_limit = 0;
+    double pool_max = 0.0;          // bytes the pool may take
+    double* d_pool = nullptr;
+    int *d_next = nullptr, *d_nst = nullptr;
+    unsigned long long* d_ctr = nullptr;   // [n_ctr] chunk counter of each sub-batch
+    int n_ctr = 0;
 };
+
+namespace {
+int batch_settle(hadros_batch* b);
+}
 
 extern "C" {
 
@@ -604,7 +618,6 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
                      t->d_cap, t->d_np, t->d_flags, t->d_cnt};
     PointsArgs& Q = t->pargs;
     Q.cam = A.cam;
-    Q.n_rays = t->npix;
     Q.out = A.out;
     ReplayArgs& R = t->rargs;
     R.rt = A.rt;
@@ -635,11 +648,8 @@ int tracer_pool(hadros_tracer* t, long extra)
     if (!t->d_nst && (rc = dalloc(&t->d_nst, size_t(t->npix)))) return rc;
     if (!t->d_ctr && (rc = dalloc(&t->d_ctr, 1))) return rc;
     t->capacity = cap;
-    t->pargs.capacity = cap;
-    t->pargs.pool = t->d_pool;
-    t->pargs.next = t->d_next;
+    t->pargs.cp = ChunkPool{t->d_pool, t->d_next, reinterpret_cast<unsigned*>(t->d_ctr), t->npix, cap};
     t->pargs.n_st = t->d_nst;
-    t->pargs.chunk_ctr = reinterpret_cast<unsigned*>(t->d_ctr);
     t->rargs.pool = t->d_pool;
     t->rargs.next = t->d_next;
     t->rargs.n_st = t->d_nst;
@@ -1079,7 +1089,10 @@ int hadros_batch_create(const hadros_camera* cams, const hadros_scene* scenes, i
 int hadros_batch_fetch_all(hadros_batch* b, hadros_image_out* o)
 {
     if (!b || !b->ran || !o) return fail(HADROS_ERR_ARG, "batch has not run");
-    HIPC(hipEventSynchronize(b->ev1));
+    {
+        const int rs = batch_settle(b);
+        if (rs) return rs;
+    }
     const size_t n = size_t(b->n_rays), nbo = size_t(b->nb_max);
     auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
         if (dst && src && bytes) HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
@@ -1099,6 +1112,10 @@ int hadros_batch_fetch_all_device(hadros_batch* b, const hadros_image_out* o, vo
 {
     if (!b || !b->ran || !o) return fail(HADROS_ERR_ARG, "batch has not run");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (b->split) {   // a split run may have to grow its pool and run again
+        const int rs = batch_settle(b);
+        if (rs) return rs;
+    }
     HIPC(hipStreamWaitEvent(s, b->ev1, 0));
     const size_t n = size_t(b->n_rays), nbo = size_t(b->nb_max);
     auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -1126,8 +1143,151 @@ int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk)
     return HADROS_OK;
 }
 
+namespace {
+
+constexpr double CHUNK_BYTES = double(CHUNK_DOUBLES) * sizeof(double);
+// default bound of the split batch's chunk pool
+constexpr double BATCH_POOL_MAX_BYTES = 24.0e9;
+// chunks per ray the pool is first sized for: the ray's own and two more
+constexpr long BATCH_CHUNKS_PER_RAY = 3;
+// sub-batch rays are int indices in the kernels
+constexpr long BATCH_SUB_MAX = 1L << 30;
+
+// the split form only on request: with the queue refilling idle lanes the
+// fused kernel keeps its lanes as busy as the replay does, and on the
+// 256-camera workload it is the faster of the two (508 vs 516 ms)
+bool batch_want_split(const hadros_batch* b) { return b->path == HADROS_PATH_SPLIT; }
+
+double batch_pool_bound(const hadros_batch* b) { return b->pool_max > 0.0 ? b->pool_max : BATCH_POOL_MAX_BYTES; }
+
+// (re)allocate the pool for sub-batches of `sub` rays and `cap` chunks
+int batch_pool(hadros_batch* b, long sub, long cap)
+{
+    dfree(b->d_pool);
+    dfree(b->d_next);
+    dfree(b->d_nst);
+    b->capacity = 0;
+    b->sub = 0;
+    int rc = 0;
+    if ((rc = dalloc(&b->d_pool, size_t(cap) * CHUNK_DOUBLES)) || (rc = dalloc(&b->d_next, size_t(cap))) ||
+        (rc = dalloc(&b->d_nst, size_t(sub))))
+        return rc;
+    b->capacity = cap;
+    b->sub = sub;
+    return HADROS_OK;
+}
+
+// first sizing: as many rays per sub-batch as BATCH_CHUNKS_PER_RAY chunks each fit in the bound
+int batch_pool_init(hadros_batch* b)
+{
+    const long fit = long(batch_pool_bound(b) / (CHUNK_BYTES * double(BATCH_CHUNKS_PER_RAY)));
+    const long sub = std::max(1L, std::min(std::min(b->n_rays, fit), BATCH_SUB_MAX));
+    return batch_pool(b, sub, sub * BATCH_CHUNKS_PER_RAY);
+}
+
+// the first ray_limit rays as sub-batches: k_batch_points into the pool, then
+// k_replay of the same rays, all queued on stream s
+int batch_launch_split(hadros_batch* b, hipStream_t s, long ray_limit)
+{
+    int rc = 0;
+    if (!b->d_pool && (rc = batch_pool_init(b))) return rc;
+    const long n_sub = std::max(1L, (ray_limit + b->sub - 1) / b->sub);
+    if (n_sub > b->n_ctr) {
+        dfree(b->d_ctr);
+        b->n_ctr = 0;
+        if ((rc = dalloc(&b->d_ctr, size_t(n_sub)))) return rc;
+        b->n_ctr = int(n_sub);
+    }
+    int per_cu = 0;
+    HIPC(batch_points_resident_blocks(b->mode, b->block, b->args.n_cams, &per_cu));
+    const int grid = std::max(1, per_cu) * b->cu;
+    HIPC(launch_zero_u64(b->d_cnt, 4, s));
+    HIPC(launch_zero_u64(b->d_ctr, int(n_sub), s));
+    HIPC(hipEventRecord(b->ev0, s));
+    for (long i = 0; i < n_sub; ++i) {
+        const long r0 = i * b->sub, n = std::min(b->sub, ray_limit - r0);
+        if (n <= 0) break;
+        BatchPointsArgs Q{};
+        Q.cams = b->d_cams;
+        Q.pix0 = b->d_pix0;
+        Q.n_cams = b->args.n_cams;
+        Q.steps_per_chunk = b->spc;
+        Q.ray0 = r0;
+        Q.n_rays = n;
+        Q.queue = b->d_queue;
+        Q.cp = ChunkPool{b->d_pool, b->d_next, reinterpret_cast<unsigned*>(b->d_ctr + i), n, b->capacity};
+        Q.n_st = b->d_nst;
+        Q.out = b->args.out;
+        HIPC(launch_zero_u64(b->d_queue, 1, s));
+        HIPC(launch_batch_points(Q, b->mode, grid, b->block, s));
+        ReplayArgs R{};
+        R.rt.n_sigma = b->args.n_sigma;
+        R.sig = b->d_sig;
+        R.sigf = b->d_sigf;
+        R.n_rays = int(n);
+        R.pool = b->d_pool;
+        R.next = b->d_next;
+        R.n_st = b->d_nst;
+        R.rts = b->d_rts;
+        R.cams = b->d_cams;
+        R.pix0 = b->d_pix0;
+        R.n_cams = b->args.n_cams;
+        R.nb_out = b->nb_max;
+        R.ray0 = r0;
+        R.tau = b->d_tau;
+        R.p = b->d_p;
+        R.I = b->d_I;
+        R.btau = b->d_bt;
+        R.bp = b->d_bp;
+        R.bi = b->d_bi;
+        HIPC(launch_replay(R, b->precision, s));
+    }
+    HIPC(hipEventRecord(b->ev1, s));
+    b->last_limit = ray_limit;
+    return HADROS_OK;
+}
+
+// after a split run: a sub-batch whose rays ran out of chunks leaves its
+// counter at the chunks it needed; grow the pool to that (past the bound,
+// with fewer rays per sub-batch) and run again
+int batch_settle(hadros_batch* b)
+{
+    HIPC(hipEventSynchronize(b->ev1));
+    if (!b->split) return HADROS_OK;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        const long n_sub = std::max(1L, (b->last_limit + b->sub - 1) / b->sub);
+        std::vector<unsigned long long> ctr(static_cast<size_t>(n_sub));
+        HIPC(hipMemcpy(ctr.data(), b->d_ctr, ctr.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        long need = 0;   // chunks of the most demanding sub-batch
+        for (long i = 0; i < n_sub; ++i) {
+            const long n = std::min(b->sub, b->last_limit - i * b->sub);
+            need = std::max(need, n + long(unsigned(ctr[size_t(i)])));
+        }
+        if (need <= b->capacity) return HADROS_OK;
+        if (attempt == 3) break;
+        long cap = long(double(need) * 1.25) + 1, sub = b->sub;
+        const double bound = batch_pool_bound(b);
+        if (double(cap) * CHUNK_BYTES > bound) {
+            sub = std::max(1L, long(double(sub) * bound / (double(cap) * CHUNK_BYTES)));
+            cap = std::max(sub + 1, long(bound / CHUNK_BYTES));
+        }
+        int rc = 0;
+        if ((rc = batch_pool(b, sub, cap)) || (rc = batch_launch_split(b, nullptr, b->last_limit))) return rc;
+        HIPC(hipEventSynchronize(b->ev1));
+    }
+    return fail(HADROS_ERR_MAX_POINTS, "split batch: chunk pool too small");
+}
+
+}  // namespace
+
 static int batch_launch(hadros_batch* b, hipStream_t s, long ray_limit)
 {
+    b->split = batch_want_split(b);
+    if (b->split) {
+        const int rc = batch_launch_split(b, s, ray_limit);
+        if (!rc) b->ran = true;
+        return rc;
+    }
     int per_cu = 0;
     HIPC(batch_resident_blocks(b->mode, b->precision, b->nb_max, b->block, b->args.n_sigma, b->args.n_cams, &per_cu));
     const int grid = std::max(1, per_cu) * b->cu;
@@ -1148,10 +1308,37 @@ int hadros_batch_run(hadros_batch* b, void* stream)
     return batch_launch(b, reinterpret_cast<hipStream_t>(stream), b->n_rays);
 }
 
+int hadros_batch_set_path(hadros_batch* b, int path)
+{
+    if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
+    if (path != HADROS_PATH_AUTO && path != HADROS_PATH_FUSED && path != HADROS_PATH_SPLIT)
+        return fail(HADROS_ERR_ARG, "path must be HADROS_PATH_AUTO, _FUSED or _SPLIT");
+    b->path = path;
+    return HADROS_OK;
+}
+
+int hadros_batch_set_pool(hadros_batch* b, double max_bytes)
+{
+    if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
+    if (!(max_bytes >= 0.0)) return fail(HADROS_ERR_ARG, "pool bound must be >= 0 bytes");
+    if (max_bytes > 0.0 && max_bytes < 2.0 * CHUNK_BYTES) return fail(HADROS_ERR_ARG, "pool bound below two chunks");
+    if (b->ran) HIPC(hipEventSynchronize(b->ev1));   // the pool may still be in use
+    b->pool_max = max_bytes;
+    dfree(b->d_pool);   // sized again at the next run
+    dfree(b->d_next);
+    dfree(b->d_nst);
+    b->capacity = 0;
+    b->sub = 0;
+    return HADROS_OK;
+}
+
 int hadros_batch_last_ms(hadros_batch* b, double* ms)
 {
     if (!b || !ms || !b->ran) return fail(HADROS_ERR_ARG, "batch has not run");
-    HIPC(hipEventSynchronize(b->ev1));
+    {
+        const int rs = batch_settle(b);
+        if (rs) return rs;
+    }
     float f = 0.0f;
     HIPC(hipEventElapsedTime(&f, b->ev0, b->ev1));
     *ms = f;
@@ -1197,7 +1384,10 @@ int hadros_batch_fetch(hadros_batch* b, int cam, hadros_image_out* o)
 {
     if (!b || !b->ran) return fail(HADROS_ERR_ARG, "batch has not run");
     if (cam < 0 || cam >= b->args.n_cams) return fail(HADROS_ERR_ARG, "camera index out of range");
-    HIPC(hipEventSynchronize(b->ev1));
+    {
+        const int rs = batch_settle(b);
+        if (rs) return rs;
+    }
     const size_t k0 = size_t(b->pix0[cam]), n = size_t(b->pix0[cam + 1] - b->pix0[cam]);
     const size_t nbo = size_t(b->nb_max), nbc = size_t(std::max(b->nbc[cam], 0));
     auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -1237,7 +1427,10 @@ int hadros_batch_fetch(hadros_batch* b, int cam, hadros_image_out* o)
 int hadros_batch_stats(hadros_batch* b, hadros_stats* st)
 {
     if (!b || !st || !b->ran) return fail(HADROS_ERR_ARG, "batch has not run");
-    HIPC(hipEventSynchronize(b->ev1));
+    {
+        const int rs = batch_settle(b);
+        if (rs) return rs;
+    }
     unsigned long long c[4];
     HIPC(hipMemcpy(c, b->d_cnt, sizeof(c), hipMemcpyDeviceToHost));
     float f = 0.0f;
@@ -1257,7 +1450,8 @@ int hadros_batch_info(hadros_batch* b, int64_t* n_rays, int* block, int* steps_p
 {
     if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
     int per_cu = 0;
-    HIPC(batch_resident_blocks(b->mode, b->precision, b->nb_max, b->block, b->args.n_sigma, b->args.n_cams, &per_cu));
+    if (batch_want_split(b)) HIPC(batch_points_resident_blocks(b->mode, b->block, b->args.n_cams, &per_cu));
+    else HIPC(batch_resident_blocks(b->mode, b->precision, b->nb_max, b->block, b->args.n_sigma, b->args.n_cams, &per_cu));
     if (n_rays) *n_rays = b->n_rays;
     if (block) *block = b->block;
     if (steps_per_chunk) *steps_per_chunk = b->spc;
@@ -1284,6 +1478,10 @@ int hadros_batch_destroy(hadros_batch* b)
     dfree(b->d_cap);
     dfree(b->d_np);
     dfree(b->d_flags);
+    dfree(b->d_pool);
+    dfree(b->d_next);
+    dfree(b->d_nst);
+    dfree(b->d_ctr);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     delete b;

@@ -20,14 +20,62 @@ struct TraceArgs {
     TraceOut out;
 };
 
+// chunk pool of the split tracer: chunk c holds up to 64 points as 32-byte
+// records (r, theta, dl, z) at pool + c * CHUNK_DOUBLES + 4 p, so each lane's
+// stores fill whole 128-byte lines every 4 points
+constexpr int CHUNK_DOUBLES = 256;
+enum { REPLAY_SRC_RANGES = 0, REPLAY_SRC_CHUNKS = 1, REPLAY_SRC_BATCH = 2 };
+
+// The pool as the geodesic kernels see it: chunks [0, base) are the rays'
+// first chunks (ray r starts in chunk r), further chunks are handed out by
+// the counter from `base` on.
+struct ChunkPool {
+    double* pool;        // capacity * CHUNK_DOUBLES
+    int* next;           // [capacity] next chunk of the same ray
+    unsigned* ctr;       // extra chunks requested
+    long base, capacity;
+};
+
+// One lane's writer of its ray's path points.  Two registers of state: the
+// chunk being filled (-1 once the pool has run out) and the points stored.
+// After running out the ray keeps integrating; finish() then adds the chunks
+// it still lacked to the counter, so the counter ends at the total demand.
+struct ChunkWriter {
+    int cur, stored;
+    __device__ __forceinline__ void begin(long first) { cur = int(first); stored = 0; }
+    __device__ __forceinline__ void put(const ChunkPool& P, double r, double th, double dl, double z)
+    {
+        if (cur < 0) return;
+        const int fill = stored & 63;
+        if (fill == 0 && stored > 0) {
+            const long c = P.base + long(atomicAdd(P.ctr, 1u));
+            if (c >= P.capacity) {
+                cur = -1;
+                return;
+            }
+            P.next[cur] = int(c);
+            cur = int(c);
+        }
+        // one 32-byte record: (r, theta) and (dl, z) as two 16-byte stores
+        double2* o = reinterpret_cast<double2*>(P.pool + long(cur) * CHUNK_DOUBLES + 4 * fill);
+        o[0] = make_double2(r, th);
+        o[1] = make_double2(dl, z);
+        ++stored;
+    }
+    // emitted: points put() was called with; only for a ray that began
+    __device__ __forceinline__ void finish(const ChunkPool& P, int emitted) const
+    {
+        if (cur >= 0) return;
+        // the failed request was the ray's stored / 64-th; it needed ceil(emitted / 64) - 1
+        const int need = (emitted + 63) / 64 - 1, requested = stored / 64;
+        if (need > requested) atomicAdd(P.ctr, unsigned(need - requested));
+    }
+};
+
 struct PointsArgs {
     DevCam cam;
-    long n_rays;            // nx * ny: chunk k < n_rays is pixel k's first chunk
-    long capacity;          // chunks in the pool
-    double* pool;           // capacity * CHUNK_DOUBLES
-    int* next;              // [capacity]
-    int* n_st;              // [n_rays] points stored per pixel
-    unsigned* chunk_ctr;    // extra chunks requested
+    ChunkPool cp;           // base = nx * ny: chunk k < base is pixel k's first chunk
+    int* n_st;              // [nx * ny] points stored per pixel
     TraceOut out;           // captured, npoints, flags, alpha, beta, counters
 };
 
@@ -40,12 +88,6 @@ struct CacheArgs {
     int* captured;        // [npix] (count pass)
 };
 
-// chunk pool of the split tracer: chunk c holds up to 64 points as 32-byte
-// records (r, theta, dl, z) at pool + c * CHUNK_DOUBLES + 4 p, so each lane's
-// stores fill whole 128-byte lines every 4 points
-constexpr int CHUNK_DOUBLES = 256;
-enum { REPLAY_SRC_RANGES = 0, REPLAY_SRC_CHUNKS = 1 };
-
 struct ReplayArgs {
     DevRt rt;
     const double* sig;
@@ -56,6 +98,14 @@ struct ReplayArgs {
     const double* pool;                 // chunks source (non-null selects it)
     const int* next;                    // [capacity] next chunk of the same ray
     const int* n_st;                    // [n_rays] points stored per ray
+    // batch source (non-null rts selects it, with the pool): local ray r is
+    // global ray ray0 + r, whose camera (last c with pix0[c] <= ray) has the
+    // transfer parameters rts[cams[c].pad_rt]; bins are written nb_out apart
+    const DevRt* rts;
+    const DevCam* cams;
+    const long* pix0;
+    int n_cams, nb_out;
+    long ray0;
     double *tau, *p, *I, *btau, *bp, *bi;
     unsigned long long* counters;
 };
@@ -84,6 +134,61 @@ struct BatchArgs {
     unsigned long long* queue;   // next unclaimed ray
     TraceOut out;                // indexed by global ray (bins: ray*nb_out + b)
 };
+
+// Geodesic half of the split batch: the rays [ray0, ray0 + n_rays) of the
+// batch's global ray order (a sub-batch), each storing its path points in the
+// chunk pool (local ray r starts in chunk r, so cp.base = n_rays).
+struct BatchPointsArgs {
+    const DevCam* cams;   // [n_cams]
+    const long* pix0;     // [n_cams+1] first global ray of each camera
+    int n_cams, steps_per_chunk;
+    long ray0, n_rays;
+    unsigned long long* queue;   // next unclaimed local ray
+    ChunkPool cp;
+    int* n_st;                   // [n_rays] points stored per local ray
+    TraceOut out;                // captured, npoints, flags by global ray; counters
+};
+
+// camera of global ray `ray`: the last c with pix0[c] <= ray
+__device__ __forceinline__ int batch_camera(const long* __restrict__ pix0, int n_cams, long ray)
+{
+    int lo = 0, hi = n_cams - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pix0[mid] <= ray) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Queue order of a batch camera's (sub-)lattice pixels: first the 8x8 tiles
+// that fit whole, tile by tile (64 consecutive rays are one tile, so the rays
+// a wavefront takes are neighbours on the sky and step alike), then the
+// pixels right of them row by row, then those below them row by row.  Maps
+// the index k in that order to the lattice pixel (p, q); the ray's outputs go
+// to pixel p * sub_ny + q of the camera.
+__device__ __forceinline__ void batch_pixel(const DevCam& C, long k, int& p, int& q)
+{
+    const int TX = C.sub_nx >> 3, TY = C.sub_ny >> 3;
+    const long full = 64L * TX * TY;
+    if (k < full) {
+        const int t = int(k >> 6), l = int(k & 63);
+        p = (t / TY) * 8 + (l >> 3);
+        q = (t % TY) * 8 + (l & 7);
+        return;
+    }
+    long r = k - full;
+    const int wr = C.sub_ny - 8 * TY;   // right strip: every row, columns 8 TY ..
+    const long right = long(C.sub_nx) * wr;
+    if (r < right) {
+        p = int(r / wr);
+        q = 8 * TY + int(r % wr);
+        return;
+    }
+    r -= right;   // bottom strip: rows 8 TX .., columns < 8 TY (TY > 0 here)
+    p = 8 * TX + int(r / (8 * TY));
+    q = int(r % (8 * TY));
+}
 
 // Sum four 64-bit per-lane values over the wavefront and add them to
 // counters[0..3] from lane 0 (one atomic per wave, not per lane).
@@ -117,6 +222,8 @@ hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s);
 hipError_t launch_opacity_surface(const OpacityArgs& A, hipStream_t s);
 hipError_t launch_trace_batch(const BatchArgs& A, int mode, int precision, int nb, int grid, int block, hipStream_t s);
 hipError_t batch_resident_blocks(int mode, int precision, int nb, int block, int n_sigma, int n_cams, int* blocks_per_cu);
+hipError_t launch_batch_points(const BatchPointsArgs& A, int mode, int grid, int block, hipStream_t s);
+hipError_t batch_points_resident_blocks(int mode, int block, int n_cams, int* blocks_per_cu);
 
 // point-wise hooks (eval.hip)
 hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th, double* out, long n);

@@ -601,6 +601,14 @@ __device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, dou
     return true;
 }
 
+// e^x rounds to +0 in FP64 for every x below this (e^-750 < 2^-1081, under half
+// the smallest subnormal 2^-1074): emission terms behind such a column are
+// skipped, which leaves the FP64 intensity bit for bit unchanged
+constexpr double EXP64_ZERO_BELOW = -750.0;
+// log-domain FP32 emission terms below e^-1000 leave the FP64 intensity
+// unchanged (zero on their own, below FP32 resolution beside any larger term)
+constexpr float EXP32_LOG_NEGLIGIBLE = -1000.0f;
+
 // FP64 transfer update of every bin for one path point:
 //   I_b   += g^3 j(E_b) e^{-tau_b} dl       (emission uses tau before the point)
 //   tau_b += n_N sigma(E_b) dl              if rho > 0 and E_b inside the table
@@ -614,7 +622,10 @@ __device__ __forceinline__ void point_rt64(const DevRt& P, const PointQ& q, cons
         if (b >= P.nb) continue;
         const double E = P.emid[b] * q.z;
         const double lnEb = P.lnemid[b] + q.lnz;
-        if (P.spec_ok && q.g3Adl != 0.0) I[b] += q.g3Adl * exp(-P.gam * lnEb - E * P.invcut - tau[b]);
+        if (P.spec_ok && q.g3Adl != 0.0) {
+            const double arg = -P.gam * lnEb - E * P.invcut - tau[b];
+            if (!(arg < EXP64_ZERO_BELOW)) I[b] += q.g3Adl * exp(arg);   // else the term is exactly 0
+        }
         if (q.rho > 0.0 && E >= P.e_lo && E <= P.e_hi) {
             const double ls = sigma_ln(lnE, lnS, slope, P.n_sigma, lnEb, P.sig_lo, P.sig_inv_d);
             tau[b] += q.nbar_dl * exp(ls);
@@ -673,7 +684,10 @@ __device__ __forceinline__ void point_rt32(const DevRt& P, const PointQ& q, cons
         if (b >= P.nb) continue;
         const double E = P.emid[b] * q.z;
         const float lnEb = float(P.lnemid[b]) + lnz;
-        if (P.spec_ok && q.g3Adl > 0.0) kI[b].add_log(lP - gam * lnEb - float(E) * invcut - ktau[b].s);
+        if (P.spec_ok && q.g3Adl > 0.0) {
+            const float L = lP - gam * lnEb - float(E) * invcut - ktau[b].s;
+            if (!(L < EXP32_LOG_NEGLIGIBLE)) kI[b].add_log(L);
+        }
         if (q.rho > 0.0 && q.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
             ktau[b].add(__expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d))));
     }

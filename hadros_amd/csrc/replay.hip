@@ -25,11 +25,6 @@ namespace hd {
 #endif
 
 constexpr int REPLAY_BLOCK = 256;
-// e^x rounds to +0 in FP64 for every x below this (e^-750 < 2^-1081, under half
-// the smallest subnormal 2^-1074)
-constexpr double EXP64_ZERO_BELOW = -750.0;
-// log-domain FP32 emission terms below e^-1000 leave the FP64 intensity unchanged
-constexpr float EXP32_LOG_NEGLIGIBLE = -1000.0f;
 constexpr int REPLAY_WAVES = REPLAY_BLOCK / 64;
 
 // Per-wavefront, per-bin running sums of the current ray, kept in LDS so that
@@ -45,6 +40,181 @@ struct ReplayAcc32 {
     ScaledKahan I[HADROS_MAX_BINS];
 };
 
+// One ray of the replay: points from the ray's source, transfer with the
+// scene's parameters P, outputs at global ray index g (bins nb_out apart).
+template <int PREC, int SRC>
+__device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, const DevRt& P, long ray, long g,
+                                                         int nb_out, int lane, ReplayAcc64& a64, ReplayAcc32& a32,
+                                                         double* bin_e, double* bin_lne, double* bin_w,
+                                                         double* bin_de, const double* lnE, const double* lnS,
+                                                         const double* slope, const float* lnEf, const float* lnSf,
+                                                         const float* slopef)
+{
+    const int n = P.n_sigma;
+    const int nb = P.nb;
+    // points of this ray: [b0, b1) of the SoA arrays, or n_st points in linked chunks
+    long b0 = 0, b1 = 0;
+    int chunk = int(ray);
+    if constexpr (SRC == REPLAY_SRC_RANGES) {
+        b0 = A.offset[ray];
+        b1 = A.offset[ray + 1];
+    } else {
+        b1 = A.n_st[ray];
+    }
+    if (lane < nb) {
+        a64.tau[lane] = 0.0;
+        a64.I[lane] = 0.0;
+        a32.tau[lane] = {0.0f, 0.0f};
+        a32.I[lane] = {{0.0f, 0.0f}, 0.0f, 0};
+        if constexpr (SRC == REPLAY_SRC_BATCH) {   // this ray's scene
+            bin_e[lane] = P.emid[lane];
+            bin_lne[lane] = P.lnemid[lane];
+            bin_w[lane] = P.w[lane];
+            bin_de[lane] = P.de[lane];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (long base = b0; base < b1; base += 64) {
+        PointQ pq;
+        bool ok = false;
+        if constexpr (SRC == REPLAY_SRC_RANGES) {
+            const long q = base + lane;
+            if (q < b1) ok = point_q(P, A.r[q], A.th[q], A.dl[q], A.z[q], pq);
+        } else {
+            if (base + lane < b1) {
+                const double2* c = reinterpret_cast<const double2*>(A.pool + long(chunk) * CHUNK_DOUBLES + 4 * lane);
+                const double2 rt = c[0], dz = c[1];
+                ok = point_q(P, rt.x, rt.y, dz.x, dz.y, pq);
+            }
+            if (base + 64 < b1) chunk = A.next[chunk];
+        }
+        if (!ok) {
+            pq.z = 1.0;
+            pq.lnz = 0.0;
+            pq.g3Adl = 0.0;
+            pq.nbar_dl = 0.0;
+            pq.rho = 0.0;
+        }
+        if constexpr (PREC == HADROS_PREC_FP64) {
+#pragma unroll 1
+            for (int b = 0; b < nb; ++b) {
+                const double E = bin_e[b] * pq.z;
+                const double lnEb = bin_lne[b] + pq.lnz;
+                double dt = 0.0;
+                if (ok && pq.rho > 0.0 && E >= P.e_lo && E <= P.e_hi)
+                    dt = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d));
+                double inc[1] = {dt};
+                wave_scan_add(inc);
+                const double tau_b = a64.tau[b];
+                const double before = tau_b + (inc[0] - dt);
+                const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
+                const double arg = -P.gam * lnEb - E * P.invcut - before;
+                // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
+                // every term is zero (behind an opaque column) adds nothing
+                double em[1] = {0.0};
+                const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
+                if (live) {
+                    if (emits) em[0] = pq.g3Adl * exp(arg);
+                    wave_scan_add(em);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 63) {
+                    a64.tau[b] = tau_b + inc[0];
+                    if (live) a64.I[b] += em[0];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else {
+            const float lnz = float(pq.lnz);
+            const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
+            const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
+            const float gam = float(P.gam), invcut = float(P.invcut);
+#pragma unroll 1
+            for (int b = 0; b < nb; ++b) {
+                const double E = bin_e[b] * pq.z;
+                const float lnEb = float(bin_lne[b]) + lnz;
+                float dt = 0.0f;
+                if (ok && pq.rho > 0.0 && pq.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
+                    dt = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d)));
+                float inc[1] = {dt};
+                wave_scan_add(inc);
+                const float before = a32.tau[b].s + (inc[0] - dt);
+                // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
+                float L = -INFINITY;
+                if (ok && P.spec_ok && pq.g3Adl > 0.0) L = lP - gam * lnEb - float(E) * invcut - before;
+                // terms below e^EXP32_LOG_NEGLIGIBLE vanish in the FP64 result or
+                // beside any larger term: a chunk of only such terms is skipped
+                const bool live = __ballot(!(L < EXP32_LOG_NEGLIGIBLE)) != 0ull;
+                float mx = -INFINITY, w[1] = {0.0f};
+                if (live) {
+                    float m[1] = {L};
+                    wave_scan_max(m);
+                    mx = lane63(m[0]);
+                    w[0] = (mx > -INFINITY) ? __expf(L - mx) : 0.0f;
+                    wave_scan_add(w);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 63) {
+                    if (mx > -INFINITY) a32.I[b].add_log(mx + __logf(w[0]));
+                    a32.tau[b].add(inc[0]);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    if (lane == 0) {
+        double t0 = 0.0, p0 = 1.0, i0 = 0.0;
+        double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
+#pragma unroll 1
+        for (int b = 0; b < nb_out; ++b) {
+            double bt = 0.0, bi = 0.0, bp = 1.0;   // bins beyond the scene's own: tau 0, P 1, I 0
+            if (b < nb) {
+                if (PREC == HADROS_PREC_FP64) {
+                    bt = a64.tau[b];
+                    bi = a64.I[b];
+                } else {
+                    const Kahan kt = a32.tau[b];
+                    const ScaledKahan ki = a32.I[b];
+                    bt = double(kt.s) - double(kt.c);
+                    bi = ki.set ? (double(ki.k.s) - double(ki.k.c)) * exp(double(ki.off)) : 0.0;
+                }
+                bp = exp(-bt);
+                if (b == 0) {
+                    t0 = bt;
+                    p0 = bp;
+                    i0 = bi;
+                }
+                ws += bin_w[b];
+                ts += bin_w[b] * bt;
+                ps += bin_w[b] * bp;
+                is += bi * bin_de[b];
+            }
+            if (A.btau) A.btau[g * nb_out + b] = bt;
+            if (A.bp) A.bp[g * nb_out + b] = bp;
+            if (A.bi) A.bi[g * nb_out + b] = bi;
+        }
+        double agg_t = 0.0, agg_p = 1.0, agg_i = 0.0;
+        if (b1 > b0) {
+            if (P.mono) {
+                if (nb > 0) {
+                    agg_t = t0;
+                    agg_p = p0;
+                    agg_i = i0;
+                }
+            } else if (ws > 0.0) {
+                agg_t = ts / ws;
+                agg_p = ps / ws;
+                agg_i = is;
+            }
+        }
+        if (A.tau) A.tau[g] = agg_t;
+        if (A.p) A.p[g] = agg_p;
+        if (A.I) A.I[g] = agg_i;
+    }
+    __builtin_amdgcn_wave_barrier();
+    return (unsigned long long)(b1 - b0);
+}
+
 template <int PREC, int SRC>
 __global__ void __launch_bounds__(REPLAY_BLOCK) __attribute__((amdgpu_waves_per_eu(HD_REPLAY_WAVES_PER_EU)))
 k_replay(ReplayArgs A)
@@ -52,18 +222,26 @@ k_replay(ReplayArgs A)
     extern __shared__ double smem[];
     __shared__ ReplayAcc64 acc64[REPLAY_WAVES];
     __shared__ ReplayAcc32 acc32[REPLAY_WAVES];
-    // per-bin constants, indexed by the rolled bin loop
-    __shared__ double bin_e[HADROS_MAX_BINS], bin_lne[HADROS_MAX_BINS], bin_w[HADROS_MAX_BINS],
-        bin_de[HADROS_MAX_BINS];
-    const DevRt& P = A.rt;
-    if (threadIdx.x < HADROS_MAX_BINS) {
-        const int b = threadIdx.x;
-        bin_e[b] = P.emid[b];
-        bin_lne[b] = P.lnemid[b];
-        bin_w[b] = P.w[b];
-        bin_de[b] = P.de[b];
+    // per-wavefront bin constants (the batch source changes scene per ray),
+    // indexed by the rolled bin loop
+    __shared__ double bin_tab[REPLAY_WAVES][4][HADROS_MAX_BINS];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double* bin_e = bin_tab[wv][0];
+    double* bin_lne = bin_tab[wv][1];
+    double* bin_w = bin_tab[wv][2];
+    double* bin_de = bin_tab[wv][3];
+    if constexpr (SRC != REPLAY_SRC_BATCH) {
+        const DevRt& P = A.rt;
+        if (lane < HADROS_MAX_BINS) {
+            bin_e[lane] = P.emid[lane];
+            bin_lne[lane] = P.lnemid[lane];
+            bin_w[lane] = P.w[lane];
+            bin_de[lane] = P.de[lane];
+        }
     }
-    const int n = P.n_sigma;
+    // sigma table (shared by every scene), then the batch's camera start table
+    const int n = A.rt.n_sigma;
     const double* lnE = smem;
     const double* lnS = smem + n;
     const double* slope = smem + 2 * n;
@@ -76,169 +254,35 @@ k_replay(ReplayArgs A)
         float* sf = reinterpret_cast<float*>(smem);
         for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) sf[t] = A.sigf[t];
     }
+    long* pix0_lds = reinterpret_cast<long*>(smem + 3 * n);
+    const bool pix_in_lds = SRC == REPLAY_SRC_BATCH && A.n_cams <= BATCH_LDS_CAMS;
+    const long* pix0 = pix_in_lds ? pix0_lds : A.pix0;
+    if (pix_in_lds)
+        for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0_lds[t] = A.pix0[t];
     __syncthreads();
 
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
     ReplayAcc64& a64 = acc64[wv];
     ReplayAcc32& a32 = acc32[wv];
-    const int nb = P.nb;
     const long nwaves = long(gridDim.x) * REPLAY_WAVES;
     unsigned long long pts_done = 0;
     for (long ray = long(blockIdx.x) * REPLAY_WAVES + wv; ray < A.n_rays; ray += nwaves) {
-        // points of this ray: [b0, b1) of the SoA arrays, or n_st points in linked chunks
-        long b0 = 0, b1 = 0;
-        int chunk = int(ray);
-        if constexpr (SRC == REPLAY_SRC_RANGES) {
-            b0 = A.offset[ray];
-            b1 = A.offset[ray + 1];
+        if constexpr (SRC == REPLAY_SRC_BATCH) {
+            // queue index -> camera -> output pixel (batch_pixel's order); the
+            // camera and scene tables are read-only here, so they are read
+            // through the constant address space (wave-uniform scalar loads)
+            const long gq = A.ray0 + ray;
+            const int c = __builtin_amdgcn_readfirstlane(batch_camera(pix0, A.n_cams, gq));
+            const DevCam& C = *(const DevCam*)(const __attribute__((address_space(4))) DevCam*)(A.cams + c);
+            int p, q;
+            batch_pixel(C, gq - pix0[c], p, q);
+            const long g = pix0[c] + long(p) * C.sub_ny + q;
+            const DevRt* P = (const DevRt*)(const __attribute__((address_space(4))) DevRt*)(A.rts + C.pad_rt);
+            pts_done += replay_ray<PREC, SRC>(A, *P, ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w, bin_de,
+                                              lnE, lnS, slope, lnEf, lnSf, slopef);
         } else {
-            b1 = A.n_st[ray];
+            pts_done += replay_ray<PREC, SRC>(A, A.rt, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
+                                              bin_de, lnE, lnS, slope, lnEf, lnSf, slopef);
         }
-        if (lane < nb) {
-            a64.tau[lane] = 0.0;
-            a64.I[lane] = 0.0;
-            a32.tau[lane] = {0.0f, 0.0f};
-            a32.I[lane] = {{0.0f, 0.0f}, 0.0f, 0};
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (long base = b0; base < b1; base += 64) {
-            PointQ pq;
-            bool ok = false;
-            if constexpr (SRC == REPLAY_SRC_RANGES) {
-                const long q = base + lane;
-                if (q < b1) ok = point_q(P, A.r[q], A.th[q], A.dl[q], A.z[q], pq);
-            } else {
-                if (base + lane < b1) {
-                    const double2* c = reinterpret_cast<const double2*>(A.pool + long(chunk) * CHUNK_DOUBLES + 4 * lane);
-                    const double2 rt = c[0], dz = c[1];
-                    ok = point_q(P, rt.x, rt.y, dz.x, dz.y, pq);
-                }
-                if (base + 64 < b1) chunk = A.next[chunk];
-            }
-            if (!ok) {
-                pq.z = 1.0;
-                pq.lnz = 0.0;
-                pq.g3Adl = 0.0;
-                pq.nbar_dl = 0.0;
-                pq.rho = 0.0;
-            }
-            if constexpr (PREC == HADROS_PREC_FP64) {
-#pragma unroll 1
-                for (int b = 0; b < nb; ++b) {
-                    const double E = bin_e[b] * pq.z;
-                    const double lnEb = bin_lne[b] + pq.lnz;
-                    double dt = 0.0;
-                    if (ok && pq.rho > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                        dt = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d));
-                    double inc[1] = {dt};
-                    wave_scan_add(inc);
-                    const double tau_b = a64.tau[b];
-                    const double before = tau_b + (inc[0] - dt);
-                    const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
-                    const double arg = -P.gam * lnEb - E * P.invcut - before;
-                    // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
-                    // every term is zero (behind an opaque column) adds nothing
-                    double em[1] = {0.0};
-                    const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
-                    if (live) {
-                        if (emits) em[0] = pq.g3Adl * exp(arg);
-                        wave_scan_add(em);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 63) {
-                        a64.tau[b] = tau_b + inc[0];
-                        if (live) a64.I[b] += em[0];
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                }
-            } else {
-                const float lnz = float(pq.lnz);
-                const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
-                const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
-                const float gam = float(P.gam), invcut = float(P.invcut);
-#pragma unroll 1
-                for (int b = 0; b < nb; ++b) {
-                    const double E = bin_e[b] * pq.z;
-                    const float lnEb = float(bin_lne[b]) + lnz;
-                    float dt = 0.0f;
-                    if (ok && pq.rho > 0.0 && pq.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                        dt = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d)));
-                    float inc[1] = {dt};
-                    wave_scan_add(inc);
-                    const float before = a32.tau[b].s + (inc[0] - dt);
-                    // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
-                    float L = -INFINITY;
-                    if (ok && P.spec_ok && pq.g3Adl > 0.0) L = lP - gam * lnEb - float(E) * invcut - before;
-                    // terms below e^EXP32_LOG_NEGLIGIBLE vanish in the FP64 result or
-                    // beside any larger term: a chunk of only such terms is skipped
-                    const bool live = __ballot(!(L < EXP32_LOG_NEGLIGIBLE)) != 0ull;
-                    float mx = -INFINITY, w[1] = {0.0f};
-                    if (live) {
-                        float m[1] = {L};
-                        wave_scan_max(m);
-                        mx = lane63(m[0]);
-                        w[0] = (mx > -INFINITY) ? __expf(L - mx) : 0.0f;
-                        wave_scan_add(w);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane == 63) {
-                        if (mx > -INFINITY) a32.I[b].add_log(mx + __logf(w[0]));
-                        a32.tau[b].add(inc[0]);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                }
-            }
-        }
-        pts_done += (unsigned long long)(b1 - b0);
-        if (lane == 0) {
-            double t0 = 0.0, p0 = 1.0, i0 = 0.0;
-            double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
-#pragma unroll 1
-            for (int b = 0; b < nb; ++b) {
-                double bt, bi;
-                if (PREC == HADROS_PREC_FP64) {
-                    bt = a64.tau[b];
-                    bi = a64.I[b];
-                } else {
-                    const Kahan kt = a32.tau[b];
-                    const ScaledKahan ki = a32.I[b];
-                    bt = double(kt.s) - double(kt.c);
-                    bi = ki.set ? (double(ki.k.s) - double(ki.k.c)) * exp(double(ki.off)) : 0.0;
-                }
-                const double bp = exp(-bt);
-                if (b == 0) {
-                    t0 = bt;
-                    p0 = bp;
-                    i0 = bi;
-                }
-                ws += bin_w[b];
-                ts += bin_w[b] * bt;
-                ps += bin_w[b] * bp;
-                is += bi * bin_de[b];
-                if (A.btau) A.btau[ray * nb + b] = bt;
-                if (A.bp) A.bp[ray * nb + b] = bp;
-                if (A.bi) A.bi[ray * nb + b] = bi;
-            }
-            double agg_t = 0.0, agg_p = 1.0, agg_i = 0.0;
-            if (b1 > b0) {
-                if (P.mono) {
-                    if (nb > 0) {
-                        agg_t = t0;
-                        agg_p = p0;
-                        agg_i = i0;
-                    }
-                } else if (ws > 0.0) {
-                    agg_t = ts / ws;
-                    agg_p = ps / ws;
-                    agg_i = is;
-                }
-            }
-            if (A.tau) A.tau[ray] = agg_t;
-            if (A.p) A.p[ray] = agg_p;
-            if (A.I) A.I[ray] = agg_i;
-        }
-        __builtin_amdgcn_wave_barrier();
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);
 }
@@ -255,13 +299,18 @@ hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s)
     const long max_grid = 256L * 64;   // persistent-ish: wavefronts loop over rays
     if (grid > max_grid) grid = max_grid;
     if (grid < 1) grid = 1;
-    const size_t lds = size_t(3) * A.rt.n_sigma * (precision == HADROS_PREC_FP64 ? sizeof(double) : sizeof(float));
+    const bool batch = A.rts != nullptr;
+    // the FP32 sigma table takes the first half of the FP64 one's space
+    size_t lds = size_t(3) * A.rt.n_sigma * sizeof(double);
+    if (batch && A.n_cams <= BATCH_LDS_CAMS) lds += size_t(A.n_cams + 1) * sizeof(long);
     const bool chunks = A.pool != nullptr;
     if (precision == HADROS_PREC_FP64) {
-        if (chunks) launch_p<HADROS_PREC_FP64, REPLAY_SRC_CHUNKS>(A, int(grid), lds, s);
+        if (batch) launch_p<HADROS_PREC_FP64, REPLAY_SRC_BATCH>(A, int(grid), lds, s);
+        else if (chunks) launch_p<HADROS_PREC_FP64, REPLAY_SRC_CHUNKS>(A, int(grid), lds, s);
         else launch_p<HADROS_PREC_FP64, REPLAY_SRC_RANGES>(A, int(grid), lds, s);
     } else {
-        if (chunks) launch_p<HADROS_PREC_FP32K, REPLAY_SRC_CHUNKS>(A, int(grid), lds, s);
+        if (batch) launch_p<HADROS_PREC_FP32K, REPLAY_SRC_BATCH>(A, int(grid), lds, s);
+        else if (chunks) launch_p<HADROS_PREC_FP32K, REPLAY_SRC_CHUNKS>(A, int(grid), lds, s);
         else launch_p<HADROS_PREC_FP32K, REPLAY_SRC_RANGES>(A, int(grid), lds, s);
     }
     return hipGetLastError();

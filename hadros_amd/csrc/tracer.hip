@@ -348,15 +348,15 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
 
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
-    if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = 0ull;
 }
 
 // ---- split form, geodesic half: the same trace as k_trace_rt, with every
 // emitted point's (r, theta, dl, redshift) stored in the chunk pool for
 // k_replay instead of being integrated here.  Without the per-point transfer
-// in the loop the kernel needs 139 VGPRs instead of k_trace_rt's 256 (with
+// in the loop the kernel needs 131 VGPRs instead of k_trace_rt's 256 (with
 // spills), so 3 wavefronts per SIMD stay resident instead of 2.  Pixel k's first chunk is chunk k;
-// further chunks come from a global counter (beyond n_rays) and are linked
+// further chunks come from a global counter (beyond nx * ny) and are linked
 // through next[].  A ray whose chunk lies beyond the pool stops storing
 // (the host sees the counter, grows the pool and runs again).
 #ifndef HD_POINTS_WAVES_PER_EU
@@ -364,7 +364,8 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 #endif
 
 template <int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HD_POINTS_WAVES_PER_EU)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MODE == HADROS_MODE_CFG ? HD_POINTS_WAVES_PER_EU : 2)))
 k_trace_points(PointsArgs A)
 {
     const DevCam& C = A.cam;
@@ -389,32 +390,12 @@ k_trace_points(PointsArgs A)
     const double e_obs_c = dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);
     const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
     const long k = long(i) * C.ny + j;
+    // the camera-plane coordinates are stored now, so they hold no registers through the loop
+    if (valid && A.out.alpha) A.out.alpha[k] = u;
+    if (valid && A.out.beta) A.out.beta[k] = v;
 
-    long cur = valid ? k : -1;   // chunk being filled
-    int fill = 0, stored = 0, emitted = 0, requested = 0;
-    bool full = false;
-    auto store = [&](double r, double th, double dl, double z) {
-        ++emitted;
-        if (full) return;
-        if (fill == 64) {
-            ++requested;
-            const long c = A.n_rays + long(atomicAdd(A.chunk_ctr, 1u));
-            if (c >= A.capacity) {
-                full = true;
-                return;
-            }
-            A.next[cur] = int(c);
-            cur = c;
-            fill = 0;
-        }
-        // one 32-byte point record: (r, theta) and (dl, z) as two 16-byte stores
-        double2* o = reinterpret_cast<double2*>(A.pool + cur * CHUNK_DOUBLES + 4 * fill);
-        o[0] = make_double2(r, th);
-        o[1] = make_double2(dl, z);
-        ++fill;
-        ++stored;
-    };
-
+    ChunkWriter w;
+    w.begin(valid ? k : -1);
     int step = 0, captured = 0, exhausted = 0, npts = 0;
     long attempts = 0;
     bool done = !valid;
@@ -488,26 +469,20 @@ k_trace_points(PointsArgs A)
         }
         // ---- emit path point `step` (src/kerr_camera.cpp:236-273)
         ++npts;
-        store(y.r, y.th, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+        w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
     }
 
     const unsigned long long c_pts = valid ? (unsigned long long)npts : 0ull;
     const unsigned long long c_att = valid ? (unsigned long long)attempts : 0ull;
     const unsigned long long c_cap = (valid && captured) ? 1ull : 0ull;
     const unsigned long long c_ex = (valid && exhausted) ? 1ull : 0ull;
-    if (full) {
-        // count every chunk this ray needed, so the counter tells the host the pool size to retry with
-        const int need = (emitted + 63) / 64 - 1;
-        if (need > requested) atomicAdd(A.chunk_ctr, unsigned(need - requested));
-    }
     if (valid) {
-        A.n_st[k] = stored;
+        w.finish(A.cp, npts - 1);   // every point after the observer's was put()
+        A.n_st[k] = w.stored;
         const TraceOut& O = A.out;
         if (O.captured) O.captured[k] = captured;
         if (O.npoints) O.npoints[k] = npts;
         if (O.flags) O.flags[k] = exhausted ? HADROS_FLAG_EXHAUSTED : 0;
-        if (O.alpha) O.alpha[k] = u;
-        if (O.beta) O.beta[k] = v;
     }
     wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
 }

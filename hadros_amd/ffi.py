@@ -32,7 +32,8 @@ EXPORTS = [
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
     "hadros_batch_create", "hadros_batch_create_sub", "hadros_batch_fetch_all", "hadros_batch_fetch_all_device",
-    "hadros_batch_set_tuning", "hadros_batch_autotune", "hadros_batch_run", "hadros_batch_last_ms",
+    "hadros_batch_set_tuning", "hadros_batch_set_path", "hadros_batch_set_pool", "hadros_batch_autotune",
+    "hadros_batch_run", "hadros_batch_last_ms",
     "hadros_batch_fetch", "hadros_batch_stats", "hadros_batch_info", "hadros_batch_destroy",
     "hadros_mlp_create", "hadros_mlp_n_params", "hadros_mlp_get_params", "hadros_mlp_set_params",
     "hadros_mlp_set_norm", "hadros_mlp_forward", "hadros_mlp_survival", "hadros_mlp_set_infer_path",
@@ -100,6 +101,8 @@ def lib():
     L.hadros_batch_fetch_all.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut)]
     L.hadros_batch_fetch_all_device.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.c_void_p]
     L.hadros_batch_set_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.hadros_batch_set_path.argtypes = [C.c_void_p, C.c_int]
+    L.hadros_batch_set_pool.argtypes = [C.c_void_p, C.c_double]
     L.hadros_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hadros_batch_run.argtypes = [C.c_void_p, C.c_void_p]
     L.hadros_batch_last_ms.argtypes = [C.c_void_p, _dp]
@@ -363,7 +366,8 @@ def trace_cache(cam):
 class Batch:
     """Persistent multi-camera tracer (hadros_batch_*): one launch for all cameras."""
 
-    def __init__(self, cams, scenes, sig_e, sig_s, precision=abi.PREC_FP64, stride=1, offset=0):
+    def __init__(self, cams, scenes, sig_e, sig_s, precision=abi.PREC_FP64, stride=1, offset=0,
+                 path=abi.PATH_AUTO):
         L = lib()
         self._se, self._ss = _sig(sig_e, sig_s)
         n = len(cams)
@@ -375,6 +379,16 @@ class Batch:
         self.h = h
         self.n = n
         self.stride, self.offset = stride, offset
+        if path != abi.PATH_AUTO:
+            self.set_path(path)
+
+    def set_path(self, path):
+        """abi.PATH_AUTO, PATH_FUSED or PATH_SPLIT (hadros_batch_set_path)."""
+        check(lib().hadros_batch_set_path(self.h, path))
+
+    def set_pool(self, max_bytes):
+        """Bound of the split form's chunk pool in bytes (0 = default)."""
+        check(lib().hadros_batch_set_pool(self.h, float(max_bytes)))
 
     ALL_FIELDS = ("tau", "p_surv", "intensity", "captured", "npoints", "flags", "bin_tau", "bin_p", "bin_i")
 

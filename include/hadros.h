@@ -327,7 +327,8 @@ int hadros_trace_cache_range(const hadros_camera* cam, int64_t k0, int64_t k1, i
    resolution, medium, source and spectrum; one integrator mode and one sigma
    table for all) traced by one persistent kernel that pulls rays from a
    global queue and refills finished lanes by wavefront ballot.  Each image
-   equals the plain hadros_trace_rt of its camera bit for bit.  There is no
+   equals the plain hadros_trace_rt of its camera, run in the same form
+   (hadros_batch_set_path / hadros_tracer_set_path), bit for bit.  There is no
    single reference interface: the reference runs one camera per process
    (apps/compute_kerr_image_cuda.cu:2418-2483 tiles one image; the parameter
    scans of scripts/run_robustness_scans.py:103-165 launch one process per
@@ -356,6 +357,18 @@ int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk);
 /* times every (block in {64,128,256}) x (steps_per_chunk in {1,4,16,64}) on
    the first eighth of the rays and keeps the fastest pair */
 int hadros_batch_autotune(hadros_batch* b, int* block, int* steps_per_chunk);
+/* How the batch runs (no reference counterpart): HADROS_PATH_FUSED, the
+   persistent kernel integrating the transfer at every point; HADROS_PATH_SPLIT,
+   sub-batches of rays whose persistent geodesic kernel stores the path points
+   in a device chunk pool, each followed by the cache-replay kernel over those
+   rays; HADROS_PATH_AUTO (default): fused.  The split form makes
+   hadros_batch_run's results final only once a fetch, stats or last_ms call
+   has checked the pool (it grows the pool and runs again when rays ran out of
+   chunks). */
+int hadros_batch_set_path(hadros_batch* b, int path);
+/* Bound (bytes) of the split form's chunk pool; the rays per sub-batch follow
+   from it.  0 = the default, 24 GB. */
+int hadros_batch_set_pool(hadros_batch* b, double max_bytes);
 int hadros_batch_run(hadros_batch* b, void* stream);
 int hadros_batch_last_ms(hadros_batch* b, double* ms);
 /* outputs of camera `cam` (pixel k = i*ny + j of that camera) */

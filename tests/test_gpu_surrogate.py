@@ -129,14 +129,16 @@ def test_batch_beyond_lds_camera_table():
     prm["log_rho0"] = prm["log_rho0"] - 8.0
     b = S.make_batch(prm, 6)
     try:
-        b.run()
-        for c in (0, 1, 517, 1023, 1024, 1099):
-            got = b.fetch(c)
-            cam = S.camera_for(prm["spin"][c], prm["inc"][c], 6, prm["fov_scale"][c])
-            sc = S.scene_for(prm["profile"][c], prm["log_rho0"][c], prm["slope"][c])
-            img, _ = ffi.trace_rt(cam, sc, E, SIG, path=abi.PATH_FUSED)
-            for name in ("tau", "p", "I", "captured", "npoints", "bin_tau", "bin_p", "bin_i"):
-                assert same(getattr(got, name), getattr(img, name)), (c, name)
+        for path in (abi.PATH_FUSED, abi.PATH_SPLIT):
+            b.set_path(path)
+            b.run()
+            for c in (0, 1, 517, 1023, 1024, 1099):
+                got = b.fetch(c)
+                cam = S.camera_for(prm["spin"][c], prm["inc"][c], 6, prm["fov_scale"][c])
+                sc = S.scene_for(prm["profile"][c], prm["log_rho0"][c], prm["slope"][c])
+                img, _ = ffi.trace_rt(cam, sc, E, SIG, path=path)
+                for name in ("tau", "p", "I", "captured", "npoints", "bin_tau", "bin_p", "bin_i"):
+                    assert same(getattr(got, name), getattr(img, name)), (path, c, name)
     finally:
         b.close()
 
