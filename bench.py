@@ -29,7 +29,10 @@ METRIC = "rays/s on 1 MI355X, plain vs batched+surrogate, at ≤1e-3 rel err vs 
 # analytic method, not of this implementation's instruction stream):
 #   per RKF45 attempt: 6 right-hand sides x F_RHS + F_COMB stage/err flops
 #   per emitted path point: F_PT (dl metric, redshift, rho, source) + n_bins x F_BIN
-ALGO = {"F_RHS": 69, "F_COMB": 390, "F_PT": 150, "F_BIN": 20}
+ALGO = {"F_RHS": 69, "F_COMB": 390, "F_PT": 150, "F_BIN": 20,
+        # F_PT split between the split form's kernels: dl metric + redshift in
+        # the geodesic kernel, density + source factor in the transfer kernel
+        "F_PT_GEO": 60, "F_PT_RT": 90}
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix), AMD datasheet
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
@@ -39,6 +42,14 @@ MLP_FLOP_PER_ROW = 2 * (14 * 256 + 3 * 256 * 256 + 256 * 16)
 
 def tracer_flops(attempts, points, nbins):
     return attempts * (6 * ALGO["F_RHS"] + ALGO["F_COMB"]) + points * (ALGO["F_PT"] + nbins * ALGO["F_BIN"])
+
+
+def geodesic_flops(attempts, points):
+    return attempts * (6 * ALGO["F_RHS"] + ALGO["F_COMB"]) + points * ALGO["F_PT_GEO"]
+
+
+def transfer_flops(points, nbins):
+    return points * (ALGO["F_PT_RT"] + nbins * ALGO["F_BIN"])
 
 
 def image_errors(img, ref):
@@ -272,7 +283,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-target", nargs="?", const="tracer", choices=["tracer", "replay"],
                     help="for rocprofv3 --pmc passes: run the plain tracer (or the cache replay) twice, then copy "
-                         "a 256 MiB device buffer twice (calibration of FETCH_SIZE / WRITE_SIZE), print nothing else")
+                         "a 1 GiB device buffer twice (calibration of FETCH_SIZE / WRITE_SIZE), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the per-ray work has no "
@@ -299,7 +310,9 @@ def main():
             t.run()
             t.last_ms()
         t.close()
-        x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        # calibration copy of 1 GiB: larger than the 256 MB last-level cache,
+        # so its reads come from HBM
+        x = torch.empty(128 * 1024 * 1024, dtype=torch.float64, device="cuda")
         y = torch.ones_like(x)
         for _ in range(2):
             x.copy_(y)
@@ -312,25 +325,43 @@ def main():
             t.run()
             t.last_ms()
         torch.cuda.synchronize()
-        kern = []
+        kern, parts = [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             t.run()
             kern.append(t.last_ms())
+            parts.append(t.kernel_ms())
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         img, st = t.fetch()
         t.close()
-        return wall, kern, img, st
+        return wall, kern, np.mean(np.array(parts), axis=0), img, st
 
-    wall, kern, img, st = timed(abi.PREC_FP64)
+    wall, kern, (geo_ms, rt_ms), img, st = timed(abi.PREC_FP64)
     ms_step = 1e3 * wall / args.steps
     value = args.steps * npix / wall
     kavg = float(np.mean(kern))
-    flops = tracer_flops(st.total_attempts, st.total_points, st.n_bins)
-    achieved = flops / (kavg * 1e-3) / 1e12
+    # per kernel of the split form (HIP events between the two launches)
+    kernels = {}
+    if rt_ms > 0:
+        gf = geodesic_flops(st.total_attempts, st.total_points)
+        rf = transfer_flops(st.total_points, st.n_bins)
+        kernels = {
+            "geodesic": {"kernel": "hd::k_trace_points<CFG>", "ms": float(geo_ms), "flop": gf,
+                         "achieved": gf / (geo_ms * 1e-3) / 1e12},
+            "transfer": {"kernel": "hd::k_replay<FP64, chunks>", "ms": float(rt_ms), "flop": rf,
+                         "achieved": rf / (rt_ms * 1e-3) / 1e12},
+        }
+        for k in kernels.values():
+            k["frac"] = k["achieved"] / PEAK_FP64_TFLOPS
+        dom = max(kernels.values(), key=lambda k: k["ms"])
+        flops, dom_ms, dom_name = dom["flop"], dom["ms"], dom["kernel"]
+    else:
+        flops, dom_ms, dom_name = tracer_flops(st.total_attempts, st.total_points, st.n_bins), kavg, \
+            "hd::k_trace_rt<CFG, FP64, 8>"
+    achieved = flops / (dom_ms * 1e-3) / 1e12
 
-    wall32, kern32, img32, st32 = timed(abi.PREC_FP32K)
+    wall32, kern32, _, img32, st32 = timed(abi.PREC_FP32K)
     # the same camera through a medium 1e-8 as dense (rho0 100 g/cm^3): the
     # 512x512 workload's medium is opaque at every pixel, this one is not
     thin = workloads.c2_scene()
@@ -363,8 +394,9 @@ def main():
                    "rays_per_step": npix, "device": dev_name, "block": st.block_size},
         "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix peak)", "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": None, "kernel_ms": kavg, "algorithmic_flop_per_launch": flops,
-                     "points_per_launch": st.total_points, "attempts_per_launch": st.total_attempts},
+                     "traffic": None, "kernel": dom_name, "kernel_ms": dom_ms, "tracer_ms": kavg,
+                     "algorithmic_flop_per_launch": flops, "points_per_launch": st.total_points,
+                     "attempts_per_launch": st.total_attempts, "kernels": kernels},
         "precision_form": {"value": args.steps * npix / wall32, "unit": "rays/s", "dtype": "f64 geodesic + f32 Kahan",
                            "kernel_ms": float(np.mean(kern32)), "max_rel_tau_vs_fp64": err_tau,
                            "max_abs_p_vs_fp64": err_p, "rel_sum_i_vs_fp64": err_i},
@@ -381,9 +413,15 @@ def main():
         out["surrogate_scan"] = surrogate_leg()
     tr = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tr):
+        # HBM bytes per launch of each kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
         t = json.load(open(tr))
         if t.get("workload") == out["config"]["workload"]:
-            out["roofline"]["traffic"] = t["bytes_per_launch"]
+            per = t.get("kernels", {})
+            if dom_name in per:
+                out["roofline"]["traffic"] = per[dom_name]["bytes_per_launch"]
+            for k in out["roofline"]["kernels"].values():
+                if k["kernel"] in per:
+                    k["traffic"] = per[k["kernel"]]["bytes_per_launch"]
             out["roofline"]["traffic_source"] = t["source"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds,

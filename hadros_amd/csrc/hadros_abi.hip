@@ -339,6 +339,7 @@ struct hadros_tracer {
     unsigned long long* d_cnt = nullptr;
     DevRt* d_rt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evm = nullptr;   // split form: between the geodesic and the transfer kernel
     bool ran = false;
 };
 
@@ -610,6 +611,7 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
     HIPC(hipMemcpy(t->d_sigf, sf.data(), sf.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPC(hipEventCreate(&t->ev0));
     HIPC(hipEventCreate(&t->ev1));
+    HIPC(hipEventCreate(&t->evm));
     TraceArgs& A = t->args;
     A.sig = t->d_sig;
     A.sigf = t->d_sigf;
@@ -679,6 +681,7 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
     HIPC(hipEventRecord(t->ev0, s));
     if (t->split) {
         HIPC(launch_trace_points(t->pargs, t->block ? t->block : 64, s));
+        HIPC(hipEventRecord(t->evm, s));
         HIPC(launch_replay(t->rargs, t->precision, s));
     } else {
         HIPC(launch_trace_rt(t->args, t->precision, t->block ? t->block : 128, s));
@@ -761,6 +764,23 @@ int hadros_tracer_last_ms(hadros_tracer* t, double* ms)
     return HADROS_OK;
 }
 
+int hadros_tracer_kernel_ms(hadros_tracer* t, double* geodesic_ms, double* transfer_ms)
+{
+    if (!t || !t->ran) return fail(HADROS_ERR_ARG, "tracer has not run");
+    const int rc = tracer_settle(t);
+    if (rc) return rc;
+    float g = 0.0f, r = 0.0f;
+    if (t->split) {
+        HIPC(hipEventElapsedTime(&g, t->ev0, t->evm));
+        HIPC(hipEventElapsedTime(&r, t->evm, t->ev1));
+    } else {
+        HIPC(hipEventElapsedTime(&g, t->ev0, t->ev1));
+    }
+    if (geodesic_ms) *geodesic_ms = g;
+    if (transfer_ms) *transfer_ms = r;
+    return HADROS_OK;
+}
+
 int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* o, hadros_stats* st)
 {
     if (!t || !t->ran) return fail(HADROS_ERR_ARG, "tracer has not run");
@@ -821,6 +841,7 @@ int hadros_tracer_destroy(hadros_tracer* t)
     dfree(t->d_nst);
     dfree(t->d_ctr);
     dfree(t->d_rt);
+    if (t->evm) (void)hipEventDestroy(t->evm);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
     delete t;

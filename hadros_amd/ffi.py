@@ -27,7 +27,8 @@ EXPORTS = [
     "hadros_scene_defaults", "hadros_parse_profile", "hadros_parse_source", "hadros_parse_spectrum",
     "hadros_profile_name", "hadros_source_name", "hadros_spectrum_name", "hadros_sigma_load",
     "hadros_sigma_check", "hadros_trace_rt", "hadros_tracer_create", "hadros_tracer_set_block", "hadros_tracer_set_path",
-    "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_fetch", "hadros_tracer_destroy",
+    "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_kernel_ms", "hadros_tracer_fetch",
+    "hadros_tracer_destroy",
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
@@ -82,6 +83,7 @@ def lib():
     L.hadros_tracer_set_path.argtypes = [C.c_void_p, C.c_int]
     L.hadros_tracer_run.argtypes = [C.c_void_p, C.c_void_p]
     L.hadros_tracer_last_ms.argtypes = [C.c_void_p, _dp]
+    L.hadros_tracer_kernel_ms.argtypes = [C.c_void_p, _dp, _dp]
     L.hadros_tracer_fetch.argtypes = [C.c_void_p, C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
     L.hadros_tracer_destroy.argtypes = [C.c_void_p]
     L.hadros_rt_from_cache.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
@@ -229,6 +231,13 @@ class Tracer:
         ms = C.c_double()
         check(lib().hadros_tracer_last_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def kernel_ms(self):
+        """(geodesic kernel ms, transfer kernel ms) of the last run; the fused
+        form reports its one kernel as the first."""
+        g, r = C.c_double(), C.c_double()
+        check(lib().hadros_tracer_kernel_ms(self.h, C.byref(g), C.byref(r)))
+        return g.value, r.value
 
     def fetch(self):
         st = abi.Stats()

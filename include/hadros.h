@@ -285,6 +285,11 @@ int hadros_tracer_run(hadros_tracer* t, void* stream);
 /* Device time (ms) of the last run's kernel, measured with HIP events on the
    launch stream; waits for the run to finish. */
 int hadros_tracer_last_ms(hadros_tracer* t, double* ms);
+/* The last run's time split by kernel (HIP events on the launch stream): the
+   split form's geodesic kernel and its transfer kernel; the fused form puts
+   its one kernel in geodesic_ms and 0 in transfer_ms.  No reference
+   counterpart (measurement hook). */
+int hadros_tracer_kernel_ms(hadros_tracer* t, double* geodesic_ms, double* transfer_ms);
 int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* out, hadros_stats* stats);
 int hadros_tracer_destroy(hadros_tracer* t);
 
