@@ -94,7 +94,7 @@ struct hadros_mlp {
 namespace {
 
 constexpr int SPLITS = 16;
-constexpr int COLSUM_ROWS = 256;
+constexpr int COLSUM_ROWS = 32;   // 256 row chunks of a 8192-row batch: one workgroup each
 
 void layout(hadros_mlp* m)
 {

@@ -560,11 +560,13 @@ class MLP:
                                     fptr(g), C.byref(loss)))
         return g, loss.value
 
-    def train_step(self, x, t, lr):
+    def train_step(self, x, t, lr, want_loss=True):
+        """One Adam step; returns the batch loss, or None with want_loss=False
+        (then nothing is read back and the step stays asynchronous)."""
         loss = C.c_float()
         check(lib().hadros_mlp_train_step(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
-                                          lr, C.byref(loss)))
-        return loss.value
+                                          lr, C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
 
     def close(self):
         if self.h:

@@ -168,14 +168,18 @@ class Surrogate:
         for ep in range(epochs):
             perm = torch.randperm(n, device="cuda", generator=g)
             Xp, Yp = Xd[perm], Yd[perm]
-            ep_loss = 0.0
+            ep_loss = float("nan")
             for k in range(steps_per_epoch):
                 # cosine decay from lr to lr_final
                 cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * step / max(total - 1, 1)))
                 sl = slice(k * self.batch, (k + 1) * self.batch)
-                ep_loss += self.net.train_step(Xp[sl], Yp[sl], cur)
+                # the loss is read back (a host synchronisation) for the epoch's last batch only
+                last = k == steps_per_epoch - 1
+                loss = self.net.train_step(Xp[sl], Yp[sl], cur, want_loss=last)
+                if last:
+                    ep_loss = loss
                 step += 1
-            hist.append(ep_loss / max(steps_per_epoch, 1))
+            hist.append(ep_loss)
             if log:
                 log(f"epoch {ep}: loss {hist[-1]:.4e}")
         torch.cuda.synchronize()
