@@ -65,6 +65,24 @@ SPECTRUM_NAMES = ["monochromatic", "powerlaw", "powerlaw_cutoff"]
 
 DEG = math.pi / 180.0
 
+# MeV channel (include/hadros.h hadros_mev; src/mev_neutrino_physics.cpp:36-233 names)
+MEV_MODELS = {"toy": 0, "TOY": 0, "physical": 1, "PHYSICAL": 1, "phys": 1}
+MEV_FLAVORS = {"nu_e": 0, "nue": 0, "NU_E": 0, "anti_nu_e": 1, "antinue": 1, "anti-nu-e": 1, "ANTI_NU_E": 1,
+               "nu_x": 2, "nux": 2, "NU_X": 2}
+MEV_THERMAL = {"constant": 0, "CONSTANT": 0, "inner_hot_torus": 1, "INNER_HOT_TORUS": 1, "radial_powerlaw": 2,
+               "RADIAL_POWERLAW": 2, "torus_plus_cool_envelope": 3, "TORUS_PLUS_COOL_ENVELOPE": 3,
+               "collapsar_inner_hot": 4, "COLLAPSAR_INNER_HOT": 4}
+MEV_YE = {"constant": 0, "CONSTANT": 0, "neutron_rich_torus": 1, "NEUTRON_RICH_TORUS": 1, "funnel_proton_rich": 2,
+          "FUNNEL_PROTON_RICH": 2, "torus_envelope_contrast": 3, "TORUS_ENVELOPE_CONTRAST": 3,
+          "collapsar_neutron_rich": 4, "COLLAPSAR_NEUTRON_RICH": 4}
+MEV_SPECTRAL = {"monochromatic": 0, "MONOCHROMATIC": 0, "mono": 0, "fermi_dirac_band": 1, "FERMI_DIRAC_BAND": 1,
+                "fd_band": 1}
+# process bits, in the order of argv 47-51 and 66-71 of apps/compute_kerr_image_from_cache.cpp
+MEV_URCA, MEV_PAIR, MEV_BREMS, MEV_ABSORPTION, MEV_SCATTERING, MEV_DEGENERACY = 1, 2, 4, 8, 16, 32
+MEV_ABS_N, MEV_ABS_P, MEV_SCAT_N, MEV_SCAT_P, MEV_SCAT_E = 64, 128, 256, 512, 1024
+MEV_DEFAULT_PROCESSES = (MEV_URCA | MEV_PAIR | MEV_BREMS | MEV_ABSORPTION | MEV_SCATTERING | MEV_ABS_N | MEV_ABS_P
+                         | MEV_SCAT_N | MEV_SCAT_P | MEV_SCAT_E)
+
 
 class Camera(C.Structure):
     _fields_ = [("a_spin", C.c_double), ("r_obs", C.c_double), ("theta_obs_rad", C.c_double),
@@ -102,6 +120,43 @@ class Scene(C.Structure):
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+
+
+class Mev(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("model", C.c_int32), ("flavor", C.c_int32),
+                ("thermal_profile", C.c_int32), ("ye_profile", C.c_int32), ("spectral_mode", C.c_int32),
+                ("n_bins", C.c_int32), ("processes", C.c_uint32),
+                ("enu_obs_MeV", C.c_double), ("norm", C.c_double), ("sigma_abs0_cm2", C.c_double),
+                ("sigma_scat0_cm2", C.c_double), ("sigma_scat_e0_cm2", C.c_double),
+                ("neutrinosphere_tau", C.c_double), ("T0_MeV", C.c_double), ("T_floor_MeV", C.c_double),
+                ("T_power", C.c_double), ("Ye_torus", C.c_double), ("Ye_funnel", C.c_double),
+                ("Ye_envelope", C.c_double), ("Ye_floor", C.c_double), ("Ye_ceil", C.c_double),
+                ("E_min_MeV", C.c_double), ("E_max_MeV", C.c_double)]
+
+
+class MevOut(C.Structure):
+    _fields_ = [("tau", C.c_void_p), ("p_surv", C.c_void_p), ("intensity", C.c_void_p),
+                ("r_neutrinosphere", C.c_void_p), ("leakage", C.c_void_p)]
+
+
+def mev(model="physical", flavor="anti_nu_e", thermal="inner_hot_torus", ye="neutron_rich_torus",
+        spectral="monochromatic", processes=None, enu=10.0, norm=1.0, sigma_abs0=9.6e-44, sigma_scat0=1.7e-44,
+        sigma_scat_e0=3.4e-45, neutrinosphere_tau=2.0 / 3.0, T0=6.0, T_floor=0.1, T_power=0.2, Ye_torus=0.25,
+        Ye_funnel=0.55, Ye_envelope=0.45, Ye_floor=0.01, Ye_ceil=0.60, E_min=3.0, E_max=50.0, n_bins=8):
+    """hadros_mev from MeVThermalParams fields (include/radiative_transfer.hpp:66-106 defaults)."""
+    m = Mev()
+    m.enabled = 1
+    m.model, m.flavor = MEV_MODELS[model], MEV_FLAVORS[flavor]
+    m.thermal_profile, m.ye_profile, m.spectral_mode = MEV_THERMAL[thermal], MEV_YE[ye], MEV_SPECTRAL[spectral]
+    m.n_bins = n_bins
+    m.processes = MEV_DEFAULT_PROCESSES if processes is None else processes
+    m.enu_obs_MeV, m.norm = enu, norm
+    m.sigma_abs0_cm2, m.sigma_scat0_cm2, m.sigma_scat_e0_cm2 = sigma_abs0, sigma_scat0, sigma_scat_e0
+    m.neutrinosphere_tau = neutrinosphere_tau
+    m.T0_MeV, m.T_floor_MeV, m.T_power = T0, T_floor, T_power
+    m.Ye_torus, m.Ye_funnel, m.Ye_envelope, m.Ye_floor, m.Ye_ceil = Ye_torus, Ye_funnel, Ye_envelope, Ye_floor, Ye_ceil
+    m.E_min_MeV, m.E_max_MeV = E_min, E_max
+    return m
 
 
 class ImageOut(C.Structure):

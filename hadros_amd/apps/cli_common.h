@@ -154,14 +154,13 @@ struct KgeoReader {
 };
 
 // ---- KIMG writer (apps/compute_kerr_image_cuda.cu:279-332) -----------------
+// MeV columns tau_mev P_surv_mev I_obs_mev r_neutrinosphere_rg leakage_mev; a row
+// without them carries the RTResult defaults (include/radiative_transfer.hpp:11-21).
 struct Row {
     int i, j, captured;
     double alpha, beta, tau, p, I;
+    double mev[5] = {0.0, 1.0, 0.0, -1.0, 1.0};
 };
-
-// MeV columns: not computed on this path; written as the RTResult defaults
-// (include/radiative_transfer.hpp:11-21).
-constexpr double MEV_TAU = 0.0, MEV_P = 1.0, MEV_I = 0.0, MEV_RNS = -1.0, MEV_LEAK = 1.0;
 
 inline bool write_kimg(const std::string& path, const std::vector<Row>& rows, int nx, int ny, double enu,
                        double mev_enu, double mev_norm, double cam_theta_deg, int32_t backend)
@@ -174,7 +173,7 @@ inline bool write_kimg(const std::string& path, const std::vector<Row>& rows, in
     out.write(reinterpret_cast<const char*>(f), sizeof(f));
     for (const Row& r : rows) {
         const double v[13] = {double(r.i), double(r.j), r.alpha, r.beta, r.tau, r.p, r.I, double(r.captured),
-                              MEV_TAU, MEV_P, MEV_I, MEV_RNS, MEV_LEAK};
+                              r.mev[0], r.mev[1], r.mev[2], r.mev[3], r.mev[4]};
         out.write(reinterpret_cast<const char*>(v), sizeof(v));
     }
     return bool(out);
@@ -183,8 +182,8 @@ inline bool write_kimg(const std::string& path, const std::vector<Row>& rows, in
 inline void write_row(std::ostream& out, const Row& r)
 {
     out << std::scientific << std::setprecision(8) << r.i << " " << r.j << " " << r.alpha << " " << r.beta << " "
-        << r.tau << " " << r.p << " " << r.I << " " << r.captured << " " << MEV_TAU << " " << MEV_P << " " << MEV_I
-        << " " << MEV_RNS << " " << MEV_LEAK << "\n";
+        << r.tau << " " << r.p << " " << r.I << " " << r.captured << " " << r.mev[0] << " " << r.mev[1] << " "
+        << r.mev[2] << " " << r.mev[3] << " " << r.mev[4] << "\n";
 }
 
 inline const char* row_columns()

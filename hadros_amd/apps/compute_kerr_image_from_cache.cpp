@@ -12,10 +12,11 @@
 
 #include <chrono>
 #include <sstream>
+#include <utility>
 
 namespace {
 
-struct MevArgs {   // argv 14-15 and 45-71: echoed into the header only
+struct MevArgs {   // argv 14-15 and 45-71 (:200-305)
     double enu = 10.0, norm = 1.0;
     std::string model = "physical", flavor = "anti_nu_e", thermal = "inner_hot_torus",
                 ye = "neutron_rich_torus", spectral = "monochromatic";
@@ -123,6 +124,40 @@ int main(int argc, char** argv)
     mev.scat_p = argi(argc, argv, 70, 1) != 0;
     mev.scat_e = argi(argc, argv, 71, 1) != 0;
 
+    // MeVThermalParams of argv 14-15 and 45-71; the reference's parsers throw on
+    // an unknown name, here the message is printed and the program exits 1
+    hadros_mev hm;
+    hadros_mev_defaults(&hm);
+    hm.enu_obs_MeV = mev.enu;
+    hm.norm = mev.norm;
+    const std::pair<int (*)(const char*, int32_t*), std::pair<const std::string*, int32_t*>> names[] = {
+        {hadros_parse_mev_model, {&mev.model, &hm.model}},
+        {hadros_parse_mev_flavor, {&mev.flavor, &hm.flavor}},
+        {hadros_parse_mev_thermal, {&mev.thermal, &hm.thermal_profile}},
+        {hadros_parse_mev_ye, {&mev.ye, &hm.ye_profile}},
+        {hadros_parse_mev_spectral, {&mev.spectral, &hm.spectral_mode}}};
+    for (const auto& n : names)
+        if (n.first(n.second.first->c_str(), n.second.second)) {
+            std::cerr << hadros_last_error() << "\n";
+            return 1;
+        }
+    const int bits[] = {mev.urca, mev.pair, mev.brems, mev.absorption, mev.scattering, mev.degeneracy,
+                        mev.abs_n, mev.abs_p, mev.scat_n, mev.scat_p, mev.scat_e};
+    hm.processes = 0;
+    for (int b = 0; b < 11; ++b)
+        if (bits[b]) hm.processes |= 1u << b;
+    hm.T0_MeV = mev.T0;
+    hm.T_floor_MeV = mev.T_floor;
+    hm.T_power = mev.T_power;
+    hm.Ye_torus = mev.Ye_torus;
+    hm.Ye_funnel = mev.Ye_funnel;
+    hm.Ye_envelope = mev.Ye_envelope;
+    hm.Ye_floor = mev.Ye_floor;
+    hm.Ye_ceil = mev.Ye_ceil;
+    hm.E_min_MeV = mev.E_min;
+    hm.E_max_MeV = mev.E_max;
+    hm.n_bins = mev.n_bins;
+
     const std::string sigma_model = sigma_model_tag(sigma_path);
 
     const Sigma sig = load_sigma(sigma_path);
@@ -156,7 +191,7 @@ int main(int argc, char** argv)
         std::cerr << "Warning: cache spin a=" << rd.a << " differs from requested ASPIN=" << a_spin << "\n";
 
     const std::string mev_band_tag =
-        mev.spectral == "monochromatic"
+        hm.spectral_mode == HADROS_MEV_SPECTRAL_MONOCHROMATIC   // the parsed mode: aliases such as "mono" too
             ? std::string("")
             : std::string("_MeVSpectrum_") + mev.spectral + "_E" + make_tag("%.0e", mev.E_min) + "_" +
                   make_tag("%.0e", mev.E_max);
@@ -212,7 +247,7 @@ int main(int argc, char** argv)
         << "# spectral_E_max_GeV " << sp.e_max_GeV << "\n"
         << "# spectral_N_bins " << sp.n_bins << "\n"
         << "# MEV_MODEL " << mev.model << "\n# MEV_CPU_MODEL " << mev.model << "\n"
-        << "# MEV_CUDA_STATUS not_computed_uhe_gpu_path\n"
+        << "# MEV_CUDA_STATUS computed_gpu_" << (hm.model == HADROS_MEV_MODEL_TOY ? "toy" : "physical") << "\n"
         << "# MEV_FLAVOR " << mev.flavor << "\n"
         << "# MEV_ENERGY_MEV " << mev.enu << "\n"
         << "# MEV_INCLUDE_URCA " << mev.urca << "\n# MEV_INCLUDE_PAIR " << mev.pair << "\n"
@@ -262,19 +297,29 @@ int main(int argc, char** argv)
         v.theta = th.data();
         v.dl = dl.data();
         v.redshift = z.data();
-        std::vector<double> tau(hdrs.size()), p(hdrs.size()), I(hdrs.size());
+        const size_t n = hdrs.size();
+        std::vector<double> tau(n), p(n), I(n), mcol[5];
+        for (auto& c : mcol) c.resize(n);
         hadros_image_out o;
         std::memset(&o, 0, sizeof(o));
         o.tau = tau.data();
         o.p_surv = p.data();
         o.intensity = I.data();
+        hadros_mev_out mo{mcol[0].data(), mcol[1].data(), mcol[2].data(), mcol[3].data(), mcol[4].data()};
         hadros_stats st;
-        check(hadros_rt_from_cache(&v, &sc, sig.e.data(), sig.s.data(), int(sig.e.size()), HADROS_PREC_FP64, &o, &st),
+        hadros_replay* plan = nullptr;
+        check(hadros_replay_create(&v, &sc, sig.e.data(), sig.s.data(), int(sig.e.size()), HADROS_PREC_FP64, &plan),
               "replay");
+        check(hadros_replay_set_mev(plan, &hm), "MeV channel");
+        check(hadros_replay_run(plan, nullptr), "replay");
+        check(hadros_replay_fetch(plan, &o, &st), "replay");
+        check(hadros_replay_fetch_mev(plan, &mo), "MeV channel");
+        hadros_replay_destroy(plan);
         gpu_ms += st.kernel_ms;
-        for (size_t k = 0; k < hdrs.size(); ++k) {
+        for (size_t k = 0; k < n; ++k) {
             Row row{hdrs[k].pixel_i, hdrs[k].pixel_j, hdrs[k].captured != 0 ? 1 : 0, hdrs[k].alpha_rg,
                     hdrs[k].beta_rg, tau[k], p[k], I[k]};
+            for (int c = 0; c < 5; ++c) row.mev[c] = mcol[c][k];
             write_row(out, row);
         }
         processed += hdrs.size();

@@ -39,6 +39,24 @@ __global__ void k_eval_emis(DevRt P, const double* r, const double* th, const do
     out[k] = j;
 }
 
+// the MeV block of integrate_kerr_ray at one point with redshift factor g:
+// T, Ye, kappa, j (src/radiative_transfer.cpp:500-600)
+__global__ void k_eval_mev(DevMedium med, DevMev M, const double* r, const double* th, const double* g, double* out,
+                           long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double rh = rho(med, r[k], th[k]);
+    const double T = mev_temperature(M, r[k], th[k], rh);
+    const double Ye = mev_ye(M, r[k], th[k], rh);
+    double kappa = 0.0, j = 0.0;
+    mev_kappa_j(M, g[k], rh, T, Ye, kappa, j);
+    out[4 * k] = T;
+    out[4 * k + 1] = Ye;
+    out[4 * k + 2] = kappa;
+    out[4 * k + 3] = j;
+}
+
 __global__ void k_eval_rhs(double a, int mode, const double* y, double* dy, long n)
 {
     const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -90,6 +108,12 @@ hipError_t launch_eval_sigma(const double* sig, int ns, double e_lo, double e_hi
 hipError_t launch_eval_emis(const DevRt& P, const double* r, const double* th, const double* e, double* out, long n)
 {
     hipLaunchKernelGGL(k_eval_emis, dim3(blocks(n)), dim3(256), 0, 0, P, r, th, e, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_eval_mev(const DevMedium& med, const DevMev& M, const double* r, const double* th, const double* g,
+                           double* out, long n)
+{
+    hipLaunchKernelGGL(k_eval_mev, dim3(blocks(n)), dim3(256), 0, 0, med, M, r, th, g, out, n);
     return hipGetLastError();
 }
 hipError_t launch_eval_rhs(double a, int mode, const double* y, double* dy, long n)

@@ -12,7 +12,9 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <sstream>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -317,6 +319,111 @@ int hd::set_error(int code, const char* msg) { return fail(code, msg); }
 int hd::device_ready() { return ensure_device(); }
 
 // ================================================================ plans
+// MeV channel of a plan: device parameters and the five per-ray output columns
+struct MevState {
+    bool on = false;
+    long n = 0;
+    DevMev* d_mev = nullptr;
+    double* d_out[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // tau, P, I, r_ns, leakage
+};
+
+namespace {
+
+int check_mev(const hadros_mev& m)
+{
+    if (m.model < 0 || m.model > 1) return fail(HADROS_ERR_KIND, "unknown MeV model");
+    if (m.flavor < 0 || m.flavor > 2) return fail(HADROS_ERR_KIND, "unknown MeV flavor");
+    if (m.thermal_profile < 0 || m.thermal_profile > 4) return fail(HADROS_ERR_KIND, "unknown MeV thermal profile");
+    if (m.ye_profile < 0 || m.ye_profile > 4) return fail(HADROS_ERR_KIND, "unknown MeV Ye profile");
+    if (m.spectral_mode < 0 || m.spectral_mode > 1) return fail(HADROS_ERR_KIND, "unknown MeV spectral mode");
+    return HADROS_OK;
+}
+
+// make_mev_physics_params (src/radiative_transfer.cpp:14-53) plus the
+// TorusProfile accessors the profiles read (src/torus_profile.cpp:12-42)
+DevMev make_mev(const hadros_mev& m, const DevMedium& med)
+{
+    DevMev d{};
+    d.model = m.model;
+    d.flavor = m.flavor;
+    d.thermal = m.thermal_profile;
+    d.ye = m.ye_profile;
+    d.spectral = m.spectral_mode;
+    d.n_bins = m.n_bins;
+    d.proc = m.processes;
+    d.enu = m.enu_obs_MeV;
+    d.norm = m.norm;
+    d.sabs = m.sigma_abs0_cm2;
+    d.sscat = m.sigma_scat0_cm2;
+    d.sscat_e = m.sigma_scat_e0_cm2;
+    d.ns_tau = m.neutrinosphere_tau;
+    d.T0 = m.T0_MeV;
+    d.T_floor = m.T_floor_MeV;
+    d.T_power = m.T_power;
+    d.Ye_torus = m.Ye_torus;
+    d.Ye_funnel = m.Ye_funnel;
+    d.Ye_envelope = m.Ye_envelope;
+    d.Ye_floor = m.Ye_floor;
+    d.Ye_ceil = m.Ye_ceil;
+    d.E_min = m.E_min_MeV;
+    d.E_max = m.E_max_MeV;
+    d.rho0 = med.rho0;
+    d.r0 = med.r0;
+    d.r_min = med.rmin;
+    d.r_max = med.rmax;
+    return d;
+}
+
+void mev_free(MevState& M)
+{
+    dfree(M.d_mev);
+    for (auto& p : M.d_out) dfree(p);
+    M.on = false;
+}
+
+int mev_set(MevState& M, const hadros_mev* m, const DevMedium& med, long n)
+{
+    if (!m || !m->enabled) {
+        M.on = false;
+        return HADROS_OK;
+    }
+    int rc = check_mev(*m);
+    if (rc) return rc;
+    const DevMev d = make_mev(*m, med);
+    if (!M.d_mev && (rc = dalloc(&M.d_mev, 1))) return rc;
+    for (auto& p : M.d_out)
+        if (!p && (rc = dalloc(&p, size_t(std::max(n, 1L))))) return rc;
+    HIPC(hipMemcpy(M.d_mev, &d, sizeof(DevMev), hipMemcpyHostToDevice));
+    M.n = n;
+    M.on = true;
+    return HADROS_OK;
+}
+
+void mev_apply(const MevState& M, ReplayArgs& R)
+{
+    R.mev = M.on ? M.d_mev : nullptr;
+    R.tau_m = M.on ? M.d_out[0] : nullptr;
+    R.p_m = M.on ? M.d_out[1] : nullptr;
+    R.I_m = M.on ? M.d_out[2] : nullptr;
+    R.rns_m = M.on ? M.d_out[3] : nullptr;
+    R.leak_m = M.on ? M.d_out[4] : nullptr;
+}
+
+// the last run's columns, or the reference's "not computed" values
+int mev_fetch(const MevState& M, bool ran_with_mev, long n, hadros_mev_out* o)
+{
+    double* dst[5] = {o->tau, o->p_surv, o->intensity, o->r_neutrinosphere, o->leakage};
+    const double dflt[5] = {0.0, 1.0, 0.0, -1.0, 1.0};
+    for (int c = 0; c < 5; ++c) {
+        if (!dst[c] || n <= 0) continue;
+        if (ran_with_mev) HIPC(hipMemcpy(dst[c], M.d_out[c], size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+        else std::fill(dst[c], dst[c] + n, dflt[c]);
+    }
+    return HADROS_OK;
+}
+
+}  // namespace
+
 struct hadros_tracer {
     TraceArgs args{};
     int precision = 0;
@@ -341,6 +448,8 @@ struct hadros_tracer {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evm = nullptr;   // split form: between the geodesic and the transfer kernel
     bool ran = false;
+    MevState mev;
+    bool ran_mev = false;       // the last run computed the MeV columns
 };
 
 struct hadros_replay {
@@ -355,6 +464,8 @@ struct hadros_replay {
     unsigned long long* d_cnt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ran = false;
+    MevState mev;
+    bool ran_mev = false;
 };
 
 struct hadros_batch {
@@ -523,6 +634,93 @@ int hadros_parse_spectrum(const char* name, int32_t* kind)
     return HADROS_OK;
 }
 
+void hadros_mev_defaults(hadros_mev* m)
+{
+    if (!m) return;
+    *m = hadros_mev{};
+    m->enabled = 1;
+    m->model = HADROS_MEV_MODEL_PHYSICAL;
+    m->flavor = HADROS_MEV_ANTI_NU_E;
+    m->thermal_profile = HADROS_MEV_T_INNER_HOT_TORUS;
+    m->ye_profile = HADROS_MEV_YE_NEUTRON_RICH_TORUS;
+    m->spectral_mode = HADROS_MEV_SPECTRAL_MONOCHROMATIC;
+    m->n_bins = 8;
+    m->processes = HADROS_MEV_URCA | HADROS_MEV_PAIR | HADROS_MEV_BREMS | HADROS_MEV_ABSORPTION |
+                   HADROS_MEV_SCATTERING | HADROS_MEV_ABS_N | HADROS_MEV_ABS_P | HADROS_MEV_SCAT_N |
+                   HADROS_MEV_SCAT_P | HADROS_MEV_SCAT_E;
+    m->enu_obs_MeV = 10.0;
+    m->norm = 1.0;
+    m->sigma_abs0_cm2 = 9.6e-44;
+    m->sigma_scat0_cm2 = 1.7e-44;
+    m->sigma_scat_e0_cm2 = 3.4e-45;
+    m->neutrinosphere_tau = 2.0 / 3.0;
+    m->T0_MeV = 6.0;
+    m->T_floor_MeV = 0.1;
+    m->T_power = 0.2;
+    m->Ye_torus = 0.25;
+    m->Ye_funnel = 0.55;
+    m->Ye_envelope = 0.45;
+    m->Ye_floor = 0.01;
+    m->Ye_ceil = 0.60;
+    m->E_min_MeV = 3.0;
+    m->E_max_MeV = 50.0;
+}
+
+namespace {
+int parse_names(const char* name, int32_t* kind, std::initializer_list<std::pair<const char*, int>> table,
+                const std::string& what)
+{
+    if (!name || !kind) return fail(HADROS_ERR_ARG, "NULL argument");
+    for (const auto& e : table)
+        if (std::strcmp(name, e.first) == 0) {
+            *kind = e.second;
+            return HADROS_OK;
+        }
+    return fail(HADROS_ERR_KIND, "Unknown " + what.substr(0, what.find('|')) + " '" + name + "'. " +
+                                     what.substr(what.find('|') + 1));
+}
+}  // namespace
+
+// parse_mev_* (src/mev_neutrino_physics.cpp:36-233): names, aliases, messages
+int hadros_parse_mev_model(const char* name, int32_t* kind)
+{
+    return parse_names(name, kind, {{"toy", 0}, {"TOY", 0}, {"physical", 1}, {"PHYSICAL", 1}, {"phys", 1}},
+                       "MEV_MODEL|Use toy or physical.");
+}
+int hadros_parse_mev_flavor(const char* name, int32_t* kind)
+{
+    return parse_names(name, kind,
+                       {{"nu_e", 0}, {"nue", 0}, {"NU_E", 0}, {"anti_nu_e", 1}, {"antinue", 1}, {"anti-nu-e", 1},
+                        {"ANTI_NU_E", 1}, {"nu_x", 2}, {"nux", 2}, {"NU_X", 2}},
+                       "MEV_FLAVOR|Use nu_e, anti_nu_e, or nu_x.");
+}
+int hadros_parse_mev_thermal(const char* name, int32_t* kind)
+{
+    return parse_names(name, kind,
+                       {{"constant", 0}, {"CONSTANT", 0}, {"inner_hot_torus", 1}, {"INNER_HOT_TORUS", 1},
+                        {"radial_powerlaw", 2}, {"RADIAL_POWERLAW", 2}, {"torus_plus_cool_envelope", 3},
+                        {"TORUS_PLUS_COOL_ENVELOPE", 3}, {"collapsar_inner_hot", 4}, {"COLLAPSAR_INNER_HOT", 4}},
+                       "MEV_THERMAL_PROFILE|Use constant, inner_hot_torus, radial_powerlaw, "
+                       "torus_plus_cool_envelope, or collapsar_inner_hot.");
+}
+int hadros_parse_mev_ye(const char* name, int32_t* kind)
+{
+    return parse_names(name, kind,
+                       {{"constant", 0}, {"CONSTANT", 0}, {"neutron_rich_torus", 1}, {"NEUTRON_RICH_TORUS", 1},
+                        {"funnel_proton_rich", 2}, {"FUNNEL_PROTON_RICH", 2}, {"torus_envelope_contrast", 3},
+                        {"TORUS_ENVELOPE_CONTRAST", 3}, {"collapsar_neutron_rich", 4},
+                        {"COLLAPSAR_NEUTRON_RICH", 4}},
+                       "MEV_YE_PROFILE|Use constant, neutron_rich_torus, funnel_proton_rich, "
+                       "torus_envelope_contrast, or collapsar_neutron_rich.");
+}
+int hadros_parse_mev_spectral(const char* name, int32_t* kind)
+{
+    return parse_names(name, kind,
+                       {{"monochromatic", 0}, {"MONOCHROMATIC", 0}, {"mono", 0}, {"fermi_dirac_band", 1},
+                        {"FERMI_DIRAC_BAND", 1}, {"fd_band", 1}},
+                       "MEV_SPECTRAL_MODE|Use monochromatic or fermi_dirac_band.");
+}
+
 const char* hadros_profile_name(int32_t k)
 {
     static const char* names[] = {"gaussian", "powerlaw", "gaussian_funnel", "powerlaw_funnel",
@@ -664,15 +862,19 @@ constexpr double SPLIT_POOL_MAX_BYTES = 8.0e9;
 bool want_split(const hadros_tracer* t)
 {
     if (t->path == HADROS_PATH_FUSED) return false;
-    if (t->path == HADROS_PATH_SPLIT) return true;
+    if (t->path == HADROS_PATH_SPLIT || t->mev.on) return true;   // the MeV channel lives in the transfer kernel
     return t->args.cam.mode == HADROS_MODE_CFG &&
            3.0 * double(t->npix) * CHUNK_DOUBLES * sizeof(double) <= SPLIT_POOL_MAX_BYTES;
 }
 
 int tracer_launch(hadros_tracer* t, hipStream_t s)
 {
+    if (t->mev.on && t->path == HADROS_PATH_FUSED)
+        return fail(HADROS_ERR_ARG, "the MeV channel runs in the split form: HADROS_PATH_FUSED cannot compute it");
     HIPC(launch_zero_u64(t->d_cnt, 4, s));
     t->split = want_split(t);
+    mev_apply(t->mev, t->rargs);
+    t->ran_mev = t->split && t->mev.on;
     if (t->split && !t->d_pool) {
         const int rc = tracer_pool(t, 2 * t->npix);
         if (rc) return rc;
@@ -706,7 +908,7 @@ int tracer_settle(hadros_tracer* t)
         const long grown = long(double(need) * 1.25);
         const double bytes = double(grown) * CHUNK_DOUBLES * sizeof(double);
         int rc = 0;
-        if (t->path == HADROS_PATH_SPLIT || bytes <= SPLIT_POOL_MAX_BYTES) {
+        if (t->path == HADROS_PATH_SPLIT || t->mev.on || bytes <= SPLIT_POOL_MAX_BYTES) {
             if ((rc = tracer_pool(t, grown - t->npix))) return rc;
         } else {
             dfree(t->d_pool);
@@ -762,6 +964,21 @@ int hadros_tracer_last_ms(hadros_tracer* t, double* ms)
     HIPC(hipEventElapsedTime(&f, t->ev0, t->ev1));
     *ms = f;
     return HADROS_OK;
+}
+
+int hadros_tracer_set_mev(hadros_tracer* t, const hadros_mev* mev)
+{
+    if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
+    if (t->ran) HIPC(hipEventSynchronize(t->ev1));   // the parameters may still be in use
+    return mev_set(t->mev, mev, t->args.rt.med, t->npix);
+}
+
+int hadros_tracer_fetch_mev(hadros_tracer* t, hadros_mev_out* o)
+{
+    if (!t || !t->ran || !o) return fail(HADROS_ERR_ARG, "tracer has not run");
+    const int rc = tracer_settle(t);
+    if (rc) return rc;
+    return mev_fetch(t->mev, t->ran_mev, t->npix, o);
 }
 
 int hadros_tracer_kernel_ms(hadros_tracer* t, double* geodesic_ms, double* transfer_ms)
@@ -841,6 +1058,7 @@ int hadros_tracer_destroy(hadros_tracer* t)
     dfree(t->d_nst);
     dfree(t->d_ctr);
     dfree(t->d_rt);
+    mev_free(t->mev);
     if (t->evm) (void)hipEventDestroy(t->evm);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
@@ -937,6 +1155,8 @@ int hadros_replay_run(hadros_replay* p, void* stream)
     if (!p) return fail(HADROS_ERR_ARG, "NULL replay");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     HIPC(launch_zero_u64(p->d_cnt, 1, s));
+    mev_apply(p->mev, p->args);
+    p->ran_mev = p->mev.on;
     HIPC(hipEventRecord(p->ev0, s));
     if (p->n_rays > 0) HIPC(launch_replay(p->args, p->precision, s));
     HIPC(hipEventRecord(p->ev1, s));
@@ -985,9 +1205,24 @@ int hadros_replay_fetch(hadros_replay* p, hadros_image_out* o, hadros_stats* st)
     return HADROS_OK;
 }
 
+int hadros_replay_set_mev(hadros_replay* p, const hadros_mev* mev)
+{
+    if (!p) return fail(HADROS_ERR_ARG, "NULL replay");
+    if (p->ran) HIPC(hipEventSynchronize(p->ev1));   // the parameters may still be in use
+    return mev_set(p->mev, mev, p->args.rt.med, p->n_rays);
+}
+
+int hadros_replay_fetch_mev(hadros_replay* p, hadros_mev_out* o)
+{
+    if (!p || !p->ran || !o) return fail(HADROS_ERR_ARG, "replay has not run");
+    HIPC(hipEventSynchronize(p->ev1));
+    return mev_fetch(p->mev, p->ran_mev, p->n_rays, o);
+}
+
 int hadros_replay_destroy(hadros_replay* p)
 {
     if (!p) return HADROS_OK;
+    mev_free(p->mev);
     dfree(p->d_sig);
     dfree(p->d_sigf);
     dfree(p->d_off);
@@ -1728,6 +1963,18 @@ int hadros_eval_emissivity(const hadros_scene* scene, const double* r, const dou
     if (rc) return rc;
     return run_eval({{r, size_t(n)}, {th, size_t(n)}, {e, size_t(n)}}, out, size_t(n),
                     [&](std::vector<double*>& d, double* o) { return launch_eval_emis(P, d[0], d[1], d[2], o, long(n)); });
+}
+
+int hadros_eval_mev(const hadros_scene* scene, const hadros_mev* mev, const double* r, const double* th,
+                    const double* g, double* out, int64_t n)
+{
+    if (!scene || !mev) return fail(HADROS_ERR_ARG, "NULL argument");
+    int rc = check_kinds(*scene);
+    if (rc || (rc = check_mev(*mev))) return rc;
+    const DevMedium med = make_medium(scene->medium);
+    const DevMev M = make_mev(*mev, med);
+    return run_eval({{r, size_t(n)}, {th, size_t(n)}, {g, size_t(n)}}, out, size_t(n) * 4,
+                    [&](std::vector<double*>& d, double* o) { return launch_eval_mev(med, M, d[0], d[1], d[2], o, long(n)); });
 }
 
 int hadros_eval_rhs(double a, int mode, const double* y, double* dy, int64_t n)

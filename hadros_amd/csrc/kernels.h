@@ -108,6 +108,9 @@ struct ReplayArgs {
     long ray0;
     double *tau, *p, *I, *btau, *bp, *bi;
     unsigned long long* counters;
+    // MeV channel (non-null mev selects it; not with the batch source)
+    const DevMev* mev;
+    double *tau_m, *p_m, *I_m, *rns_m, *leak_m;
 };
 
 struct OpacityArgs {
@@ -232,6 +235,8 @@ hipError_t launch_eval_sigma(const double* sig, int n_sigma, double e_lo, double
 hipError_t launch_eval_emis(const DevRt& P, const double* r, const double* th, const double* e, double* out,
                             long n);
 hipError_t launch_eval_rhs(double a, int mode, const double* y, double* dy, long n);
+hipError_t launch_eval_mev(const DevMedium& med, const DevMev& M, const double* r, const double* th, const double* g,
+                           double* out, long n);
 hipError_t launch_eval_metric(double a, const double* r, const double* th, double* out, long n);
 
 }  // namespace hd

@@ -578,6 +578,200 @@ __device__ __forceinline__ float sigma_ln_f(const float* __restrict__ lnE, const
     return lnS[lo - 1] + (lx - lnE[lo - 1]) * slope[lo];
 }
 
+// ------------------------------------------------------------------ MeV channel
+// MeVPhysicsParams (include/mev_neutrino_physics.hpp:42-80) with the enums as
+// ints (HADROS_MEV_*), plus the TorusProfile constants its profiles read
+// (rho0, r0, r_min, r_max after the constructor's clamps).
+struct DevMev {
+    int model, flavor, thermal, ye, spectral, n_bins;
+    unsigned proc;   // HADROS_MEV_* process bits
+    int pad;
+    double enu, norm, sabs, sscat, sscat_e, ns_tau;
+    double T0, T_floor, T_power, Ye_torus, Ye_funnel, Ye_envelope, Ye_floor, Ye_ceil, E_min, E_max;
+    double rho0, r0, r_min, r_max;
+};
+
+constexpr double MEV_HBAR_C = 1.973269804e-11;   // MeV cm
+constexpr double MEV_M_E = 0.510998950;          // MeV
+
+// density_shape (src/mev_neutrino_physics.cpp:26-33)
+__device__ __forceinline__ double mev_shape(const DevMev& M, double rho)
+{
+    return dclamp(rho / dmax(M.rho0, 1.0e-300), 0.0, 1.0);
+}
+
+// mev_temperature_profile_MeV (src/mev_neutrino_physics.cpp:235-286)
+__device__ __forceinline__ double mev_temperature(const DevMev& M, double r, double th, double rho)
+{
+    const double T0 = dmax(M.T0, 1.0e-12);
+    const double floor_ = dmax(M.T_floor, 1.0e-12);
+    const double shape = mev_shape(M, rho);
+    const double eq = (th - 0.5 * PI) / 0.45;
+    double T = T0;
+    if (M.thermal == HADROS_MEV_T_INNER_HOT_TORUS) {
+        T = floor_ + T0 * pow(shape, dmax(M.T_power, 0.0));
+    } else if (M.thermal == HADROS_MEV_T_RADIAL_POWERLAW) {
+        const double radial = pow(dmax(r / dmax(M.r0, 1.0e-300), 1.0e-300), -M.T_power);
+        T = floor_ + T0 * radial * (0.35 + 0.65 * exp(-(eq * eq)));
+    } else if (M.thermal == HADROS_MEV_T_TORUS_PLUS_COOL_ENVELOPE) {
+        T = floor_ + T0 * pow(shape, dmax(M.T_power, 0.0));
+        if (r > M.r_max || r < M.r_min || shape < 1.0e-3) {
+            const double radial = pow(dmax(r / dmax(M.r0, 1.0e-300), 1.0e-300), -M.T_power);
+            T = floor_ + 0.15 * T0 * radial;
+        }
+    } else if (M.thermal == HADROS_MEV_T_COLLAPSAR_INNER_HOT) {
+        const double q = (r - M.r_min) / dmax(1.15 * M.r0, 1.0e-6);
+        const double inner_hot = exp(-(q * q));
+        const double radial_cooling = pow(1.0 + dmax(r - M.r_min, 0.0) / dmax(M.r0, 1.0e-6), -0.75);
+        T = floor_ + T0 * (0.30 + 0.70 * pow(shape, 0.18)) * (0.45 + 0.55 * exp(-(eq * eq))) *
+                         (0.55 + 0.45 * inner_hot) * radial_cooling;
+    }
+    return dmax(T, floor_);
+}
+
+// mev_ye_profile (src/mev_neutrino_physics.cpp:288-336)
+__device__ __forceinline__ double mev_ye(const DevMev& M, double r, double th, double rho)
+{
+    const double shape = mev_shape(M, rho);
+    double Ye = M.Ye_torus;
+    if (M.ye == HADROS_MEV_YE_NEUTRON_RICH_TORUS) {
+        const double q = (r - M.r0) / 4.0;
+        Ye = M.Ye_torus + 0.05 * exp(-(q * q));
+    } else if (M.ye == HADROS_MEV_YE_FUNNEL_PROTON_RICH || M.ye == HADROS_MEV_YE_COLLAPSAR_NEUTRON_RICH) {
+        const double a = th / 0.35, b = (PI - th) / 0.35;
+        const double pw = dclamp(exp(-(a * a)) + exp(-(b * b)), 0.0, 1.0);
+        if (M.ye == HADROS_MEV_YE_FUNNEL_PROTON_RICH) {
+            Ye = M.Ye_torus * (1.0 - pw) + M.Ye_funnel * pw;
+        } else {
+            const double q = (r - M.r0) / dmax(1.2 * M.r0, 1.0e-6);
+            const double torus_ye = M.Ye_torus - 0.05 * pow(shape, 0.25) * exp(-(q * q));
+            Ye = torus_ye * (1.0 - pw) + M.Ye_funnel * pw;
+        }
+    } else if (M.ye == HADROS_MEV_YE_TORUS_ENVELOPE_CONTRAST) {
+        const bool envelope_like = r > M.r0 && (r > M.r_max || shape < 1.0e-2);
+        Ye = envelope_like ? M.Ye_envelope : M.Ye_torus;
+        if (r < M.r_min && shape < 1.0e-2) Ye = M.Ye_funnel;
+    }
+    return dclamp(Ye, dmax(M.Ye_floor, 1.0e-6), dmin(M.Ye_ceil, 1.0 - 1.0e-6));
+}
+
+// mev_thermal_spectral_shape without the 1/T^3 (mev_fermi_dirac_weight,
+// src/mev_neutrino_physics.cpp:338-372)
+__device__ __forceinline__ double mev_fd(double E, double T)
+{
+    if (E <= 0.0 || T <= 0.0) return 0.0;
+    const double x = E / T;
+    if (x > 120.0) return 0.0;
+    return E * E / (exp(x) + 1.0);
+}
+
+// mev_total_emissivity (src/mev_neutrino_physics.cpp:424-593, the urca / pair /
+// brems terms and the legacy toy branch)
+__device__ __forceinline__ double mev_emissivity(const DevMev& M, double rho, double T, double Ye, double E)
+{
+    if (M.model == HADROS_MEV_MODEL_TOY) {
+        if (rho <= 0.0 || T <= 0.0 || E <= 0.0) return 0.0;
+        const double nb = rho / M_U_G;
+        const double ccw = dclamp(1.0 - Ye, 0.0, 1.0) + 0.5 * dclamp(Ye, 0.0, 1.0);
+        const double spec = E * E / (exp(dmin(E / T, 120.0)) + 1.0);
+        const double capture_like = nb * ccw * pow(T, 6.0);
+        const double pair_like = 0.05 * nb * pow(T, 9.0);
+        return M.norm * (capture_like + pair_like) * spec;
+    }
+    if (T <= 0.0 || E <= 0.0) return 0.0;
+    const double x = E / T;
+    // thermal spectral shape E^2 / (e^x + 1) / T^3, shared by the three processes
+    const double shp = (x > 120.0) ? 0.0 : E * E / (exp(x) + 1.0) / dmax(T * T * T, 1.0e-300);
+    const double T10 = T / 10.0;
+    const double rho10 = rho / 1.0e10;
+    double total = 0.0;
+    if ((M.proc & HADROS_MEV_URCA) && rho > 0.0 && M.flavor != HADROS_MEV_NU_X) {
+        const double ye = dclamp(Ye, 0.0, 1.0);
+        const double ff = (M.flavor == HADROS_MEV_NU_E) ? ye : (1.0 - ye);
+        double urca = 1.0e30 * pow(dmax(rho10, 0.0), 1.0) * pow(dmax(T10, 0.0), 6.0) * ff * shp;
+        if (M.proc & HADROS_MEV_DEGENERACY) {
+            // mev_urca_degeneracy_correction (:415-441) with the zero-temperature
+            // Fermi momentum proxy (:380-400)
+            const double ne = rho * ye / M_U_G;
+            const double pF = (ne <= 0.0) ? 0.0 : MEV_HBAR_C * cbrt(3.0 * PI * PI * ne);
+            const double eta = sqrt(pF * pF + MEV_M_E * MEV_M_E) / T;
+            double fac = 1.0;
+            if (isfinite(eta)) {
+                const double shift = 0.25 * tanh((eta - 1.0) / 4.0);
+                fac = dclamp((M.flavor == HADROS_MEV_NU_E) ? (1.0 + shift) : (1.0 - shift), 0.5, 1.5);
+            }
+            urca *= fac;
+        }
+        total += urca;
+    }
+    if (M.proc & HADROS_MEV_PAIR)
+        total += 3.0e28 * pow(dmax(T10, 0.0), 9.0) * ((M.flavor == HADROS_MEV_NU_X) ? 1.0 : 0.7) * shp;
+    if ((M.proc & HADROS_MEV_BREMS) && rho > 0.0)
+        total += 1.0e27 * pow(dmax(rho10, 0.0), 2.0) * pow(dmax(T10, 0.0), 5.0) *
+                 ((M.flavor == HADROS_MEV_NU_X) ? 1.0 : 0.8) * shp;
+    return M.norm * total;
+}
+
+// mev_total_opacity_cm_inv: absorption + scattering in cm^-1
+// (src/mev_neutrino_physics.cpp:595-769)
+__device__ __forceinline__ double mev_opacity(const DevMev& M, double rho, double Ye, double E)
+{
+    if (rho <= 0.0 || E <= 0.0) return 0.0;
+    const double nb = rho / M_U_G;
+    if (M.model == HADROS_MEV_MODEL_TOY) {
+        double k = 0.0;
+        if (M.proc & HADROS_MEV_ABSORPTION)
+            k = nb * M.sabs * E * E * (dclamp(1.0 - Ye, 0.0, 1.0) + 0.25 * dclamp(Ye, 0.0, 1.0));
+        double sc = 0.0;
+        if (M.proc & HADROS_MEV_SCATTERING) sc = nb * M.sscat * E * E * (1.0 + 0.5 * dclamp(Ye, 0.0, 1.0));
+        return k + sc;
+    }
+    const double cy = dclamp(Ye, 0.0, 1.0);
+    double abs_n = 0.0, abs_p = 0.0, sn = 0.0, sp = 0.0, se = 0.0;
+    if (M.proc & HADROS_MEV_ABSORPTION) {
+        if ((M.proc & HADROS_MEV_ABS_N) && M.flavor == HADROS_MEV_NU_E) abs_n = nb * (1.0 - cy) * M.sabs * E * E;
+        if ((M.proc & HADROS_MEV_ABS_P) && M.flavor == HADROS_MEV_ANTI_NU_E) abs_p = nb * cy * M.sabs * E * E;
+    }
+    if (M.proc & HADROS_MEV_SCATTERING) {
+        if (M.proc & HADROS_MEV_SCAT_N) sn = nb * (1.0 - cy) * M.sscat * E * E;
+        if (M.proc & HADROS_MEV_SCAT_P) sp = nb * cy * 0.5 * M.sscat * E * E;
+        if (M.proc & HADROS_MEV_SCAT_E) se = rho * cy / M_U_G * M.sscat_e * E * E;
+    }
+    return (abs_n + abs_p) + (sn + sp + se);
+}
+
+// kappa and j of one path point (src/radiative_transfer.cpp:541-600): the
+// monochromatic channel at E_obs / g, or the Fermi-Dirac band average of kappa
+// and band sum of j
+__device__ __forceinline__ void mev_kappa_j(const DevMev& M, double g, double rho, double T, double Ye, double& kappa,
+                                            double& j)
+{
+    const double gc = dmax(g, 1.0e-300);
+    if (M.spectral == HADROS_MEV_SPECTRAL_FERMI_DIRAC_BAND) {
+        const int nbins = M.n_bins > 1 ? M.n_bins : 1;
+        const double e_min = dmax(M.E_min, 1.0e-6);
+        const double e_max = dmax(M.E_max, e_min);
+        const double dE = (e_max - e_min) / double(nbins);
+        double ws = 0.0, kw = 0.0, js = 0.0;
+#pragma unroll 1
+        for (int ib = 0; ib < nbins; ++ib) {
+            const double E_local = (e_min + (double(ib) + 0.5) * dE) / gc;
+            const double w = mev_fd(E_local, T) * dE;
+            if (w <= 0.0 || !isfinite(w)) continue;
+            const double lk = mev_opacity(M, rho, Ye, E_local);
+            js += mev_emissivity(M, rho, T, Ye, E_local) * dE;
+            kw += w * lk;
+            ws += w;
+        }
+        kappa = ws > 0.0 ? kw / ws : 0.0;
+        j = js;
+    } else {
+        const double E = M.enu / gc;
+        kappa = mev_opacity(M, rho, Ye, E);
+        j = mev_emissivity(M, rho, T, Ye, E);
+    }
+}
+
 // ------------------------------------------------------------------ transfer
 // Per-path-point quantities shared by all bins (integrate_kerr_ray,
 // src/radiative_transfer.cpp:501-672): g = 1/z, g^3, rho, n_N dl, emissivity

@@ -42,7 +42,7 @@ struct ReplayAcc32 {
 
 // One ray of the replay: points from the ray's source, transfer with the
 // scene's parameters P, outputs at global ray index g (bins nb_out apart).
-template <int PREC, int SRC>
+template <int PREC, int SRC, bool MEV>
 __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, const DevRt& P, long ray, long g,
                                                          int nb_out, int lane, ReplayAcc64& a64, ReplayAcc32& a32,
                                                          double* bin_e, double* bin_lne, double* bin_w,
@@ -74,19 +74,68 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
         }
     }
     __builtin_amdgcn_wave_barrier();
+    // MeV channel of integrate_kerr_ray (src/radiative_transfer.cpp:500-640), kept
+    // wave-uniform: tau_mev, I_mev, the summed j dl (leakage) and r_ns.  The
+    // spectral reference takes it from its first integrated bin, so a scene
+    // with no integrated UHE bin keeps the defaults.
+    const bool mev_on = MEV && (P.mono || nb > 0);
+    double m_tau = 0.0, m_I = 0.0, m_leak = 0.0, m_rns = -1.0;
     for (long base = b0; base < b1; base += 64) {
         PointQ pq;
         bool ok = false;
+        double pr = 0.0, pth = 0.0, pdl = 0.0;
         if constexpr (SRC == REPLAY_SRC_RANGES) {
             const long q = base + lane;
-            if (q < b1) ok = point_q(P, A.r[q], A.th[q], A.dl[q], A.z[q], pq);
+            if (q < b1) {
+                pr = A.r[q];
+                pth = A.th[q];
+                pdl = A.dl[q];
+                ok = point_q(P, pr, pth, pdl, A.z[q], pq);
+            }
         } else {
             if (base + lane < b1) {
                 const double2* c = reinterpret_cast<const double2*>(A.pool + long(chunk) * CHUNK_DOUBLES + 4 * lane);
                 const double2 rt = c[0], dz = c[1];
+                pr = rt.x;
+                pth = rt.y;
+                pdl = dz.x;
                 ok = point_q(P, rt.x, rt.y, dz.x, dz.y, pq);
             }
             if (base + 64 < b1) chunk = A.next[chunk];
+        }
+        if constexpr (MEV) {
+            if (mev_on) {
+                // per point: I -> a I + b, tau += dtau (lane order = path order)
+                const DevMev& M = *A.mev;
+                double dtau[1] = {0.0}, lk[1] = {0.0}, a = 1.0, b = 0.0;
+                if (ok) {
+                    const double gz = 1.0 / pq.z;
+                    const double dl_cm = pdl * P.rg_cm;
+                    const double T = mev_temperature(M, pr, pth, pq.rho);
+                    const double Ye = mev_ye(M, pr, pth, pq.rho);
+                    double kappa = 0.0, j = 0.0;
+                    mev_kappa_j(M, gz, pq.rho, T, Ye, kappa, j);
+                    dtau[0] = kappa * dl_cm;
+                    const double g3 = pow(gz, 3.0);
+                    if (kappa > 0.0 && dtau[0] > 1.0e-12) {
+                        a = exp(-dmin(dtau[0], 700.0));
+                        b = g3 * (j / kappa) * (1.0 - a);
+                    } else {
+                        b = g3 * j * dl_cm;
+                    }
+                    lk[0] = j * dl_cm;
+                }
+                wave_scan_add(dtau);
+                wave_scan_add(lk);
+                wave_scan_affine(a, b);
+                if (m_rns < 0.0) {
+                    const unsigned long long hit = __ballot(ok && m_tau + dtau[0] >= M.ns_tau);
+                    if (hit) m_rns = read_lane(pr, __ffsll(static_cast<long long>(hit)) - 1);
+                }
+                m_tau += lane63(dtau[0]);
+                m_leak += lane63(lk[0]);
+                m_I = lane63(a) * m_I + lane63(b);
+            }
         }
         if (!ok) {
             pq.z = 1.0;
@@ -210,13 +259,21 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
         if (A.tau) A.tau[g] = agg_t;
         if (A.p) A.p[g] = agg_p;
         if (A.I) A.I[g] = agg_i;
+        if constexpr (MEV) {
+            if (A.tau_m) A.tau_m[g] = m_tau;
+            if (A.p_m) A.p_m[g] = exp(-m_tau);
+            if (A.I_m) A.I_m[g] = m_I;
+            if (A.rns_m) A.rns_m[g] = m_rns;
+            if (A.leak_m) A.leak_m[g] = m_leak > 0.0 ? m_leak / m_leak : 1.0;
+        }
     }
     __builtin_amdgcn_wave_barrier();
     return (unsigned long long)(b1 - b0);
 }
 
-template <int PREC, int SRC>
-__global__ void __launch_bounds__(REPLAY_BLOCK) __attribute__((amdgpu_waves_per_eu(HD_REPLAY_WAVES_PER_EU)))
+template <int PREC, int SRC, bool MEV>
+__global__ void __launch_bounds__(REPLAY_BLOCK)
+__attribute__((amdgpu_waves_per_eu(MEV ? 2 : HD_REPLAY_WAVES_PER_EU)))   // the MeV channel needs the registers
 k_replay(ReplayArgs A)
 {
     extern __shared__ double smem[];
@@ -277,11 +334,11 @@ k_replay(ReplayArgs A)
             batch_pixel(C, gq - pix0[c], p, q);
             const long g = pix0[c] + long(p) * C.sub_ny + q;
             const DevRt* P = (const DevRt*)(const __attribute__((address_space(4))) DevRt*)(A.rts + C.pad_rt);
-            pts_done += replay_ray<PREC, SRC>(A, *P, ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w, bin_de,
-                                              lnE, lnS, slope, lnEf, lnSf, slopef);
+            pts_done += replay_ray<PREC, SRC, MEV>(A, *P, ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w,
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef);
         } else {
-            pts_done += replay_ray<PREC, SRC>(A, A.rt, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                              bin_de, lnE, lnS, slope, lnEf, lnSf, slopef);
+            pts_done += replay_ray<PREC, SRC, MEV>(A, A.rt, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef);
         }
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);
@@ -290,7 +347,14 @@ k_replay(ReplayArgs A)
 template <int PREC, int SRC>
 static void launch_p(const ReplayArgs& A, int grid, size_t lds, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_replay<PREC, SRC>), dim3(grid), dim3(REPLAY_BLOCK), lds, s, A);
+    // the MeV channel is a separate instantiation, so the UHE-only kernels keep their registers
+    if constexpr (SRC != REPLAY_SRC_BATCH) {
+        if (A.mev) {
+            hipLaunchKernelGGL((k_replay<PREC, SRC, true>), dim3(grid), dim3(REPLAY_BLOCK), lds, s, A);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_replay<PREC, SRC, false>), dim3(grid), dim3(REPLAY_BLOCK), lds, s, A);
 }
 
 hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s)

@@ -89,6 +89,38 @@ __device__ __forceinline__ void wave_scan_max(float (&x)[N])
 #undef HD_SCAN_STEP
 }
 
+// DPP move of a double whose lanes without a source keep `identity`
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move_id(double v, double identity)
+{
+    const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+    const unsigned long long id = static_cast<unsigned long long>(__double_as_longlong(identity));
+    const unsigned lo = unsigned(dpp_word<CTRL, ROW_MASK>(int(id & 0xffffffffull), int(u & 0xffffffffull)));
+    const unsigned hi = unsigned(dpp_word<CTRL, ROW_MASK>(int(id >> 32), int(u >> 32)));
+    return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// Inclusive scan of affine maps x -> a x + b in lane order: lane l ends with
+// the composition of the maps of lanes 0..l (lane 0's applied first), i.e.
+// the pair (a_0 ... a_l, sum_k b_k a_{k+1} ... a_l).  Identity (1, 0).
+__device__ __forceinline__ void wave_scan_affine(double& a, double& b)
+{
+#define HD_AFFINE_STEP(CTRL, RM)                                   \
+    {                                                               \
+        const double pa = dpp_move_id<CTRL, RM>(a, 1.0);            \
+        const double pb = dpp_move_id<CTRL, RM>(b, 0.0);            \
+        b = pb * a + b;                                             \
+        a = pa * a;                                                 \
+    }
+    HD_AFFINE_STEP(0x111, 0xf)
+    HD_AFFINE_STEP(0x112, 0xf)
+    HD_AFFINE_STEP(0x114, 0xf)
+    HD_AFFINE_STEP(0x118, 0xf)
+    HD_AFFINE_STEP(0x142, 0xa)
+    HD_AFFINE_STEP(0x143, 0xc)
+#undef HD_AFFINE_STEP
+}
+
 __device__ __forceinline__ double lane63(double v)
 {
     const unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));

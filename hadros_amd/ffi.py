@@ -31,7 +31,10 @@ EXPORTS = [
     "hadros_tracer_destroy",
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
-    "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric",
+    "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric", "hadros_eval_mev",
+    "hadros_tracer_set_mev", "hadros_tracer_fetch_mev", "hadros_replay_set_mev", "hadros_replay_fetch_mev",
+    "hadros_mev_defaults", "hadros_parse_mev_model", "hadros_parse_mev_flavor", "hadros_parse_mev_thermal",
+    "hadros_parse_mev_ye", "hadros_parse_mev_spectral",
     "hadros_batch_create", "hadros_batch_create_sub", "hadros_batch_fetch_all", "hadros_batch_fetch_all_device",
     "hadros_batch_set_tuning", "hadros_batch_set_path", "hadros_batch_set_pool", "hadros_batch_autotune",
     "hadros_batch_run", "hadros_batch_last_ms",
@@ -130,6 +133,15 @@ def lib():
                                          C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _ip]
     L.hadros_eval_sigma.argtypes = [_dp, _dp, C.c_int, _dp, _dp, C.c_int64]
     L.hadros_eval_emissivity.argtypes = [C.POINTER(abi.Scene), _dp, _dp, _dp, _dp, C.c_int64]
+    L.hadros_eval_mev.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.Mev), _dp, _dp, _dp, _dp, C.c_int64]
+    for f in ("hadros_tracer_set_mev", "hadros_replay_set_mev"):
+        getattr(L, f).argtypes = [C.c_void_p, C.POINTER(abi.Mev)]
+    for f in ("hadros_tracer_fetch_mev", "hadros_replay_fetch_mev"):
+        getattr(L, f).argtypes = [C.c_void_p, C.POINTER(abi.MevOut)]
+    L.hadros_mev_defaults.argtypes = [C.POINTER(abi.Mev)]
+    L.hadros_mev_defaults.restype = None
+    for f in ("model", "flavor", "thermal", "ye", "spectral"):
+        getattr(L, "hadros_parse_mev_" + f).argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
     L.hadros_eval_rhs.argtypes = [C.c_double, C.c_int, _dp, _dp, C.c_int64]
     L.hadros_eval_metric.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int64]
     _lib = L
@@ -208,6 +220,33 @@ def _sig(sig_e, sig_s):
     return np.ascontiguousarray(sig_e, np.float64), np.ascontiguousarray(sig_s, np.float64)
 
 
+class MevImage:
+    """The five MeV columns per ray (hadros_mev_out) as numpy arrays."""
+
+    FIELDS = ("tau", "p_surv", "intensity", "r_neutrinosphere", "leakage")
+
+    def __init__(self, n):
+        for f in self.FIELDS:
+            setattr(self, f, np.zeros(n))
+
+    def out(self):
+        o = abi.MevOut()
+        for f in self.FIELDS:
+            setattr(o, f, getattr(self, f).ctypes.data)
+        return o
+
+    def stack(self):
+        """[n x 5] in .dat column order (tau_mev P_surv_mev I_obs_mev r_neutrinosphere_rg leakage_mev)."""
+        return np.stack([getattr(self, f) for f in self.FIELDS], 1)
+
+
+def _fetch_mev(fn, h, n):
+    img = MevImage(n)
+    o = img.out()
+    check(fn(h, C.byref(o)))
+    return img
+
+
 class Tracer:
     """Re-usable plan of hadros_trace_rt (device buffers allocated once)."""
 
@@ -231,6 +270,13 @@ class Tracer:
         ms = C.c_double()
         check(lib().hadros_tracer_last_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def set_mev(self, mev):
+        """MeV channel for the next runs (abi.mev(...)), or None to turn it off."""
+        check(lib().hadros_tracer_set_mev(self.h, C.byref(mev) if mev is not None else None))
+
+    def fetch_mev(self):
+        return _fetch_mev(lib().hadros_tracer_fetch_mev, self.h, self.cam.nx * self.cam.ny)
 
     def kernel_ms(self):
         """(geodesic kernel ms, transfer kernel ms) of the last run; the fused
@@ -320,6 +366,13 @@ class Replay:
         ms = C.c_double()
         check(lib().hadros_replay_last_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def set_mev(self, mev):
+        """MeV channel for the next runs (abi.mev(...)), or None to turn it off."""
+        check(lib().hadros_replay_set_mev(self.h, C.byref(mev) if mev is not None else None))
+
+    def fetch_mev(self):
+        return _fetch_mev(lib().hadros_replay_fetch_mev, self.h, self.n)
 
     def fetch(self):
         st = abi.Stats()
@@ -613,6 +666,14 @@ def eval_emissivity(scene, r, th, e):
     r, th, e = (np.ascontiguousarray(x, np.float64) for x in (r, th, e))
     out = np.zeros(r.size)
     check(lib().hadros_eval_emissivity(C.byref(scene), dptr(r), dptr(th), dptr(e), dptr(out), r.size))
+    return out
+
+
+def eval_mev(scene, mev, r, th, g):
+    """(T_MeV, Ye, kappa, j) per point, [n x 4] (hadros_eval_mev)."""
+    r, th, g = (np.ascontiguousarray(x, np.float64) for x in (r, th, g))
+    out = np.zeros((r.size, 4))
+    check(lib().hadros_eval_mev(C.byref(scene), C.byref(mev), dptr(r), dptr(th), dptr(g), dptr(out), r.size))
     return out
 
 

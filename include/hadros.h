@@ -157,6 +157,63 @@ typedef struct hadros_spectrum {
     double e_min_GeV, e_max_GeV;
 } hadros_spectrum;
 
+/* MeV thermal channel: replaces MeVThermalParams (include/radiative_transfer.hpp:66-106)
+   with mev_neutrino::MeVPhysicsParams (include/mev_neutrino_physics.hpp:42-80).  With
+   enabled = 0 the MeV outputs keep the reference's "not computed" values
+   (tau 0, P 1, I 0, r_ns -1, leakage 1). */
+enum {
+    HADROS_MEV_MODEL_TOY = 0, HADROS_MEV_MODEL_PHYSICAL = 1
+};
+enum {
+    HADROS_MEV_NU_E = 0, HADROS_MEV_ANTI_NU_E = 1, HADROS_MEV_NU_X = 2
+};
+enum {
+    HADROS_MEV_T_CONSTANT = 0, HADROS_MEV_T_INNER_HOT_TORUS = 1, HADROS_MEV_T_RADIAL_POWERLAW = 2,
+    HADROS_MEV_T_TORUS_PLUS_COOL_ENVELOPE = 3, HADROS_MEV_T_COLLAPSAR_INNER_HOT = 4
+};
+enum {
+    HADROS_MEV_YE_CONSTANT = 0, HADROS_MEV_YE_NEUTRON_RICH_TORUS = 1, HADROS_MEV_YE_FUNNEL_PROTON_RICH = 2,
+    HADROS_MEV_YE_TORUS_ENVELOPE_CONTRAST = 3, HADROS_MEV_YE_COLLAPSAR_NEUTRON_RICH = 4
+};
+enum {
+    HADROS_MEV_SPECTRAL_MONOCHROMATIC = 0, HADROS_MEV_SPECTRAL_FERMI_DIRAC_BAND = 1
+};
+/* bits of hadros_mev.processes (all set by default, except DEGENERACY) */
+enum {
+    HADROS_MEV_URCA = 1 << 0, HADROS_MEV_PAIR = 1 << 1, HADROS_MEV_BREMS = 1 << 2,
+    HADROS_MEV_ABSORPTION = 1 << 3, HADROS_MEV_SCATTERING = 1 << 4, HADROS_MEV_DEGENERACY = 1 << 5,
+    HADROS_MEV_ABS_N = 1 << 6, HADROS_MEV_ABS_P = 1 << 7, HADROS_MEV_SCAT_N = 1 << 8,
+    HADROS_MEV_SCAT_P = 1 << 9, HADROS_MEV_SCAT_E = 1 << 10
+};
+typedef struct hadros_mev {
+    int32_t enabled;
+    int32_t model;            /* HADROS_MEV_MODEL_*                             */
+    int32_t flavor;           /* HADROS_MEV_NU_E / _ANTI_NU_E / _NU_X           */
+    int32_t thermal_profile;  /* HADROS_MEV_T_*                                 */
+    int32_t ye_profile;       /* HADROS_MEV_YE_*                                */
+    int32_t spectral_mode;    /* HADROS_MEV_SPECTRAL_*                          */
+    int32_t n_bins;           /* Fermi-Dirac band bins                          */
+    uint32_t processes;       /* HADROS_MEV_* bits                              */
+    double enu_obs_MeV;
+    double norm;
+    double sigma_abs0_cm2, sigma_scat0_cm2, sigma_scat_e0_cm2;
+    double neutrinosphere_tau;
+    double T0_MeV, T_floor_MeV, T_power;
+    double Ye_torus, Ye_funnel, Ye_envelope, Ye_floor, Ye_ceil;
+    double E_min_MeV, E_max_MeV;
+} hadros_mev;
+
+/* Per-ray MeV outputs (the .dat columns tau_mev P_surv_mev I_obs_mev
+   r_neutrinosphere_rg leakage_mev, src/radiative_transfer.cpp:763-769).
+   Caller-owned host arrays, any may be NULL. */
+typedef struct hadros_mev_out {
+    double* tau;
+    double* p_surv;
+    double* intensity;
+    double* r_neutrinosphere;
+    double* leakage;
+} hadros_mev_out;
+
 /* Everything the transfer integral needs besides the ray and the sigma table:
    the argv block of apps/compute_kerr_image_from_cache.cpp:57-205 that is on
    the UHE path. */
@@ -235,6 +292,16 @@ int hadros_parse_profile(const char* name, int32_t* kind);
 int hadros_parse_source(const char* name, int32_t* kind);
 /* parse_uhe_spectral_model (src/radiative_transfer.cpp:132-156). */
 int hadros_parse_spectrum(const char* name, int32_t* kind);
+/* MeVThermalParams defaults (include/radiative_transfer.hpp:66-106), enabled = 1. */
+void hadros_mev_defaults(hadros_mev* mev);
+/* parse_mev_model / _flavor / _thermal_profile / _ye_profile / _spectral_mode
+   (src/mev_neutrino_physics.cpp:36-233): the reference's names and aliases;
+   HADROS_ERR_KIND with the reference's message otherwise. */
+int hadros_parse_mev_model(const char* name, int32_t* kind);
+int hadros_parse_mev_flavor(const char* name, int32_t* kind);
+int hadros_parse_mev_thermal(const char* name, int32_t* kind);
+int hadros_parse_mev_ye(const char* name, int32_t* kind);
+int hadros_parse_mev_spectral(const char* name, int32_t* kind);
 const char* hadros_profile_name(int32_t kind);
 const char* hadros_source_name(int32_t kind);
 const char* hadros_spectrum_name(int32_t kind);
@@ -291,6 +358,12 @@ int hadros_tracer_last_ms(hadros_tracer* t, double* ms);
    counterpart (measurement hook). */
 int hadros_tracer_kernel_ms(hadros_tracer* t, double* geodesic_ms, double* transfer_ms);
 int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* out, hadros_stats* stats);
+/* MeV channel of integrate_kerr_ray (src/radiative_transfer.cpp:500-640) for the
+   plan's next runs (NULL or enabled = 0 turns it off).  It runs in the split
+   form's transfer kernel, so a plan with MeV runs split (HADROS_PATH_FUSED
+   is then an error).  hadros_tracer_fetch_mev copies the last run's columns. */
+int hadros_tracer_set_mev(hadros_tracer* t, const hadros_mev* mev);
+int hadros_tracer_fetch_mev(hadros_tracer* t, hadros_mev_out* out);
 int hadros_tracer_destroy(hadros_tracer* t);
 
 /* Cache replay: radiative_transfer::integrate_kerr_ray_spectral over every ray
@@ -309,6 +382,9 @@ int hadros_replay_create(const hadros_cache_view* cache, const hadros_scene* sce
 int hadros_replay_run(hadros_replay* p, void* stream);
 int hadros_replay_last_ms(hadros_replay* p, double* ms);
 int hadros_replay_fetch(hadros_replay* p, hadros_image_out* out, hadros_stats* stats);
+/* MeV channel of the replay (as hadros_tracer_set_mev / _fetch_mev). */
+int hadros_replay_set_mev(hadros_replay* p, const hadros_mev* mev);
+int hadros_replay_fetch_mev(hadros_replay* p, hadros_mev_out* out);
 int hadros_replay_destroy(hadros_replay* p);
 
 /* Geodesic cache on the GPU: the --geodesic-cache mode of
@@ -447,6 +523,14 @@ int hadros_eval_sigma(const double* sigma_e_gev, const double* sigma_cm2, int n_
    with the spectral model of `scene`. */
 int hadros_eval_emissivity(const hadros_scene* scene, const double* r, const double* th,
                            const double* e_local, double* out, int64_t n);
+/* The MeV channel at points (r, theta) of the scene's medium, for a path
+   point whose frequency ratio is g = 1 / redshift: out is n x 4 =
+   (T_MeV, Ye, kappa_cm^-1, j) as mev_temperature_profile_MeV, mev_ye_profile,
+   and the opacity / emissivity of integrate_kerr_ray's MeV block at
+   E_local = E_obs / g, or the Fermi-Dirac band's kappa average and j sum
+   (src/mev_neutrino_physics.cpp:235-771, src/radiative_transfer.cpp:500-600). */
+int hadros_eval_mev(const hadros_scene* scene, const hadros_mev* mev, const double* r, const double* th,
+                    const double* g, double* out, int64_t n);
 /* KerrGeodesic::rhs (src/kerr_geodesic.cpp:83-135) for mode REF (finite
    differences) or the analytic derivative for mode CFG; y and dy are n x 8
    (t, r, theta, phi, p_t, p_r, p_theta, p_phi). */

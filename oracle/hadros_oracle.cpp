@@ -914,6 +914,303 @@ int or_trace_image(const hadros_camera* cam, const hadros_scene* s, const double
     return or_trace_image_range(cam, s, sig_e, sig_s, n_sig, nthreads, 0, int64_t(cam->nx) * cam->ny, out, st);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- MeV channel
+// src/mev_neutrino_physics.cpp restated with the reference's operation order
+// and libm calls (std::pow, std::exp, std::cbrt, std::tanh).
+namespace {
+
+const double MEV_HBAR_C = 1.973269804e-11;   // :13
+const double MEV_M_E = 0.510998950;          // :14
+
+double mev_clamp_ye(double Ye) { return clamp_(Ye, 0.0, 1.0); }                        // :17-20
+double mev_safe_pow(double x, double p) { return std::pow(max_(x, 0.0), p); }           // :22-25
+double mev_density_shape(double rho, double rho0)                                       // :26-33
+{
+    return clamp_(rho / max_(rho0, 1.0e-300), 0.0, 1.0);
+}
+
+struct MevP {
+    const hadros_mev* m;
+    double rho0, r0, r_min, r_max;   // TorusProfile accessors (src/torus_profile.cpp:12-42)
+    bool has(unsigned bit) const { return (m->processes & bit) != 0; }
+};
+
+double mev_temperature(const MevP& P, double r, double th, double rho)   // :235-286
+{
+    const hadros_mev& m = *P.m;
+    const double T0 = max_(m.T0_MeV, 1.0e-12);
+    const double floor_ = max_(m.T_floor_MeV, 1.0e-12);
+    const double shape = mev_density_shape(rho, P.rho0);
+    const double radial = std::pow(max_(r / max_(P.r0, 1.0e-300), 1.0e-300), -m.T_power);
+    const double equatorial = std::exp(-std::pow((th - 0.5 * PI) / 0.45, 2.0));
+    const bool outside_torus = r > P.r_max || r < P.r_min;
+    double T = T0;
+    if (m.thermal_profile == HADROS_MEV_T_CONSTANT) {
+        T = T0;
+    } else if (m.thermal_profile == HADROS_MEV_T_INNER_HOT_TORUS) {
+        T = floor_ + T0 * std::pow(shape, max_(m.T_power, 0.0));
+    } else if (m.thermal_profile == HADROS_MEV_T_RADIAL_POWERLAW) {
+        T = floor_ + T0 * radial * (0.35 + 0.65 * equatorial);
+    } else if (m.thermal_profile == HADROS_MEV_T_TORUS_PLUS_COOL_ENVELOPE) {
+        T = floor_ + T0 * std::pow(shape, max_(m.T_power, 0.0));
+        if (outside_torus || shape < 1.0e-3) T = floor_ + 0.15 * T0 * radial;
+    } else if (m.thermal_profile == HADROS_MEV_T_COLLAPSAR_INNER_HOT) {
+        const double inner_hot = std::exp(-std::pow((r - P.r_min) / max_(1.15 * P.r0, 1.0e-6), 2.0));
+        const double radial_cooling = std::pow(1.0 + max_(r - P.r_min, 0.0) / max_(P.r0, 1.0e-6), -0.75);
+        T = floor_ + T0 * (0.30 + 0.70 * std::pow(shape, 0.18)) * (0.45 + 0.55 * equatorial) *
+                         (0.55 + 0.45 * inner_hot) * radial_cooling;
+    }
+    return max_(T, floor_);
+}
+
+double mev_ye(const MevP& P, double r, double th, double rho)   // :288-336
+{
+    const hadros_mev& m = *P.m;
+    const double shape = mev_density_shape(rho, P.rho0);
+    const double polar = std::exp(-std::pow(th / 0.35, 2.0)) + std::exp(-std::pow((PI - th) / 0.35, 2.0));
+    const bool envelope_like = r > P.r0 && (r > P.r_max || shape < 1.0e-2);
+    double Ye = m.Ye_torus;
+    if (m.ye_profile == HADROS_MEV_YE_CONSTANT) {
+        Ye = m.Ye_torus;
+    } else if (m.ye_profile == HADROS_MEV_YE_NEUTRON_RICH_TORUS) {
+        Ye = m.Ye_torus + 0.05 * std::exp(-std::pow((r - P.r0) / 4.0, 2.0));
+    } else if (m.ye_profile == HADROS_MEV_YE_FUNNEL_PROTON_RICH) {
+        Ye = m.Ye_torus * (1.0 - clamp_(polar, 0.0, 1.0)) + m.Ye_funnel * clamp_(polar, 0.0, 1.0);
+    } else if (m.ye_profile == HADROS_MEV_YE_TORUS_ENVELOPE_CONTRAST) {
+        Ye = envelope_like ? m.Ye_envelope : m.Ye_torus;
+        if (r < P.r_min && shape < 1.0e-2) Ye = m.Ye_funnel;
+    } else if (m.ye_profile == HADROS_MEV_YE_COLLAPSAR_NEUTRON_RICH) {
+        const double polar_weight = clamp_(polar, 0.0, 1.0);
+        const double dense_weight = std::pow(shape, 0.25);
+        const double radial_neutronization = std::exp(-std::pow((r - P.r0) / max_(1.2 * P.r0, 1.0e-6), 2.0));
+        const double torus_ye = m.Ye_torus - 0.05 * dense_weight * radial_neutronization;
+        Ye = torus_ye * (1.0 - polar_weight) + m.Ye_funnel * polar_weight;
+    }
+    return clamp_(Ye, max_(m.Ye_floor, 1.0e-6), min_(m.Ye_ceil, 1.0 - 1.0e-6));
+}
+
+double mev_thermal_shape(double E, double T)   // :338-358
+{
+    if (E <= 0.0 || T <= 0.0) return 0.0;
+    const double x = E / T;
+    if (x > 120.0) return 0.0;
+    const double T3 = max_(T * T * T, 1.0e-300);
+    return E * E / (std::exp(x) + 1.0) / T3;
+}
+
+double mev_fd_weight(double E, double T)   // :360-372
+{
+    if (E <= 0.0 || T <= 0.0) return 0.0;
+    const double x = E / T;
+    if (x > 120.0) return 0.0;
+    return E * E / (std::exp(x) + 1.0);
+}
+
+double mev_ne(double rho, double Ye) { return rho <= 0.0 ? 0.0 : rho * mev_clamp_ye(Ye) / M_U_G; }   // :374-385
+
+double mev_urca_deg(double rho, double Ye, double T, int flavor)   // :387-441
+{
+    if (flavor == HADROS_MEV_NU_X) return 1.0;
+    const double ne = mev_ne(rho, Ye);
+    const double pF = ne <= 0.0 ? 0.0 : MEV_HBAR_C * std::cbrt(3.0 * PI * PI * ne);
+    const double mu = std::sqrt(pF * pF + MEV_M_E * MEV_M_E);
+    const double eta = T <= 0.0 ? 0.0 : mu / T;
+    if (!std::isfinite(eta)) return 1.0;
+    const double shift = 0.25 * std::tanh((eta - 1.0) / 4.0);
+    const double factor = flavor == HADROS_MEV_NU_E ? (1.0 + shift) : (1.0 - shift);
+    return clamp_(factor, 0.5, 1.5);
+}
+
+double mev_emissivity(const MevP& P, double rho, double T, double Ye, double E)   // :443-593
+{
+    const hadros_mev& m = *P.m;
+    if (m.model == HADROS_MEV_MODEL_TOY) {
+        if (rho <= 0.0 || T <= 0.0 || E <= 0.0) return 0.0;
+        const double nb = rho / M_U_G;
+        const double neutron_fraction = clamp_(1.0 - Ye, 0.0, 1.0);
+        const double proton_fraction = clamp_(Ye, 0.0, 1.0);
+        const double ccw = neutron_fraction + 0.5 * proton_fraction;
+        const double thermal_spectrum = E * E / (std::exp(min_(E / T, 120.0)) + 1.0);
+        const double capture_like = nb * ccw * std::pow(T, 6.0);
+        const double pair_like = 0.05 * nb * std::pow(T, 9.0);
+        return m.norm * (capture_like + pair_like) * thermal_spectrum;
+    }
+    double total = 0.0;
+    if (P.has(HADROS_MEV_URCA)) {
+        double urca = 0.0;
+        if (!(rho <= 0.0 || T <= 0.0 || E <= 0.0) && m.flavor != HADROS_MEV_NU_X) {
+            const double ye = mev_clamp_ye(Ye);
+            const double ff = m.flavor == HADROS_MEV_NU_E ? ye : (1.0 - ye);
+            urca = 1.0e30 * mev_safe_pow(rho / 1.0e10, 1.0) * mev_safe_pow(T / 10.0, 6.0) * ff * mev_thermal_shape(E, T);
+        }
+        if (P.has(HADROS_MEV_DEGENERACY)) urca *= mev_urca_deg(rho, Ye, T, m.flavor);
+        total += urca;
+    }
+    if (P.has(HADROS_MEV_PAIR)) {
+        if (!(T <= 0.0 || E <= 0.0))
+            total += 3.0e28 * mev_safe_pow(T / 10.0, 9.0) * (m.flavor == HADROS_MEV_NU_X ? 1.0 : 0.7) *
+                     mev_thermal_shape(E, T);
+        else
+            total += 0.0;
+    }
+    if (P.has(HADROS_MEV_BREMS)) {
+        if (!(rho <= 0.0 || T <= 0.0 || E <= 0.0))
+            total += 1.0e27 * mev_safe_pow(rho / 1.0e10, 2.0) * mev_safe_pow(T / 10.0, 5.0) *
+                     (m.flavor == HADROS_MEV_NU_X ? 1.0 : 0.8) * mev_thermal_shape(E, T);
+        else
+            total += 0.0;
+    }
+    return m.norm * total;
+}
+
+double mev_opacity(const MevP& P, double rho, double Ye, double E)   // :595-769
+{
+    const hadros_mev& m = *P.m;
+    const bool absn = P.has(HADROS_MEV_ABSORPTION), scat = P.has(HADROS_MEV_SCATTERING);
+    double absorption, scattering;
+    if (m.model == HADROS_MEV_MODEL_TOY) {
+        absorption = 0.0;
+        if (!(!absn || rho <= 0.0 || E <= 0.0)) {
+            const double nb = rho / M_U_G;
+            const double target_fraction = clamp_(1.0 - Ye, 0.0, 1.0) + 0.25 * clamp_(Ye, 0.0, 1.0);
+            absorption = nb * m.sigma_abs0_cm2 * E * E * target_fraction;
+        }
+        scattering = 0.0;
+        if (!(!scat || rho <= 0.0 || E <= 0.0)) {
+            const double nb = rho / M_U_G;
+            const double composition_factor = 1.0 + 0.5 * clamp_(Ye, 0.0, 1.0);
+            scattering = nb * m.sigma_scat0_cm2 * E * E * composition_factor;
+        }
+        return absorption + scattering;
+    }
+    const bool ok = rho > 0.0 && E > 0.0;
+    const double nb = rho / M_U_G;
+    double an = 0.0, ap = 0.0, sn = 0.0, sp = 0.0, se = 0.0;
+    if (absn && P.has(HADROS_MEV_ABS_N) && m.flavor == HADROS_MEV_NU_E && ok)
+        an = nb * (1.0 - mev_clamp_ye(Ye)) * m.sigma_abs0_cm2 * E * E;
+    if (absn && P.has(HADROS_MEV_ABS_P) && m.flavor == HADROS_MEV_ANTI_NU_E && ok)
+        ap = nb * mev_clamp_ye(Ye) * m.sigma_abs0_cm2 * E * E;
+    if (scat && P.has(HADROS_MEV_SCAT_N) && ok) sn = nb * (1.0 - mev_clamp_ye(Ye)) * m.sigma_scat0_cm2 * E * E;
+    if (scat && P.has(HADROS_MEV_SCAT_P) && ok) sp = nb * mev_clamp_ye(Ye) * 0.5 * m.sigma_scat0_cm2 * E * E;
+    if (scat && P.has(HADROS_MEV_SCAT_E) && ok) se = mev_ne(rho, Ye) * m.sigma_scat_e0_cm2 * E * E;
+    absorption = an + ap;
+    scattering = sn + sp + se;
+    return absorption + scattering;
+}
+
+// kappa and j at one point (src/radiative_transfer.cpp:541-600)
+void mev_kappa_j(const MevP& P, double g, double rho, double T, double Ye, double& kappa, double& j)
+{
+    const hadros_mev& m = *P.m;
+    kappa = 0.0;
+    j = 0.0;
+    if (m.spectral_mode == HADROS_MEV_SPECTRAL_FERMI_DIRAC_BAND) {
+        const int n_bins = std::max(m.n_bins, 1);
+        const double e_min = max_(m.E_min_MeV, 1.0e-6);
+        const double e_max = max_(m.E_max_MeV, e_min);
+        const double dE = (e_max - e_min) / static_cast<double>(n_bins);
+        double weight_sum = 0.0, kappa_weighted = 0.0;
+        for (int ib = 0; ib < n_bins; ++ib) {
+            const double E_mid = e_min + (static_cast<double>(ib) + 0.5) * dE;
+            const double E_local = E_mid / max_(g, 1.0e-300);
+            const double w = mev_fd_weight(E_local, T) * dE;
+            if (w <= 0.0 || !std::isfinite(w)) continue;
+            const double local_kappa = mev_opacity(P, rho, Ye, E_local);
+            j += mev_emissivity(P, rho, T, Ye, E_local) * dE;
+            kappa_weighted += w * local_kappa;
+            weight_sum += w;
+        }
+        kappa = weight_sum > 0.0 ? kappa_weighted / weight_sum : 0.0;
+    } else {
+        const double E = m.enu_obs_MeV / max_(g, 1.0e-300);
+        kappa = mev_opacity(P, rho, Ye, E);
+        j = mev_emissivity(P, rho, T, Ye, E);
+    }
+}
+
+// The MeV accumulators of integrate_kerr_ray (:477-485, :500-640, :763-769).
+struct MevRay {
+    double tau = 0.0, I = 0.0, leak_sum = 0.0, leak_w = 0.0, r_ns = -1.0;
+    void point(const MevP& P, const Medium& med, double r_h_rt, double rg_cm, double r, double th, double dl, double z)
+    {
+        if (r < r_h_rt + 1.0e-3) return;
+        const double g = 1.0 / max_(z, 1.0e-300);
+        const double dl_cm = dl * rg_cm;
+        const double rho = med.rho(r, th);
+        const double T = mev_temperature(P, r, th, rho);
+        const double Ye = mev_ye(P, r, th, rho);
+        double kappa, j;
+        mev_kappa_j(P, g, rho, T, Ye, kappa, j);
+        const double dtau = kappa * dl_cm;
+        if (kappa > 0.0 && dtau > 1.0e-12) {
+            const double att = std::exp(-min_(dtau, 700.0));
+            I = I * att + std::pow(g, 3.0) * (j / kappa) * (1.0 - att);
+        } else {
+            I += std::pow(g, 3.0) * j * dl_cm;
+        }
+        leak_sum += j * dl_cm;
+        leak_w += j * dl_cm;
+        tau += dtau;
+        if (r_ns < 0.0 && tau >= P.m->neutrinosphere_tau) r_ns = r;
+    }
+    void finish(double out[5]) const
+    {
+        out[0] = tau;
+        out[1] = std::exp(-tau);
+        out[2] = I;
+        out[3] = r_ns;
+        out[4] = leak_w > 0.0 ? leak_sum / leak_w : 1.0;
+    }
+};
+
+MevP mev_params(const hadros_mev* m, const Medium& med) { return MevP{m, med.rho0, med.r0, med.rmin, med.rmax}; }
+
+}  // namespace
+
+extern "C" {
+
+void or_mev_point(const hadros_scene* s, const hadros_mev* m, double r, double th, double g, double out[4])
+{
+    const Medium med(&s->medium);
+    const MevP P = mev_params(m, med);
+    const double rho = med.rho(r, th);
+    const double T = mev_temperature(P, r, th, rho);
+    const double Ye = mev_ye(P, r, th, rho);
+    double kappa, j;
+    mev_kappa_j(P, g, rho, T, Ye, kappa, j);
+    out[0] = T;
+    out[1] = Ye;
+    out[2] = kappa;
+    out[3] = j;
+}
+
+int or_rt_from_cache_mev(const hadros_cache_view* c, const hadros_scene* s, const double* sig_e, const double* sig_s,
+                         int n_sig, const hadros_mev* m, int nthreads, double* mev_out)
+{
+    if (!c || !m || !mev_out) return HADROS_ERR_ARG;
+    const int rc = check_scene(s, n_sig);
+    if (rc) return rc;
+    const RtCtx ctx(s, sig_e, sig_s, n_sig, c->a_bh);
+    const MevP P = mev_params(m, ctx.med);
+    // integrate_kerr_ray_spectral takes the MeV fields from its first integrated
+    // bin (:741-760): none without one (or for an empty range)
+    const bool have = ctx.mono || ctx.nb > 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t k = 0; k < c->n_rays; ++k) {
+        MevRay ray;
+        if (have)
+            for (int64_t q = c->offset[k]; q < c->offset[k + 1]; ++q)
+                ray.point(P, ctx.med, ctx.r_h_rt, ctx.rg_cm, c->r[q], c->theta[q], c->dl[q], c->redshift[q]);
+        ray.finish(mev_out + 5 * k);
+    }
+    return HADROS_OK;
+}
+
 int or_rt_from_cache(const hadros_cache_view* c, const hadros_scene* s, const double* sig_e, const double* sig_s,
                      int n_sig, int nthreads, hadros_image_out* out)
 {

@@ -111,6 +111,15 @@ int or_rt_from_cache(const hadros_cache_view* c, const hadros_scene* s,
                      const double* sig_e, const double* sig_s, int n_sig,
                      int nthreads, hadros_image_out* out);
 
+/* The MeV channel (src/mev_neutrino_physics.cpp, src/radiative_transfer.cpp:500-640).
+   or_mev_point: T_MeV, Ye, kappa (cm^-1) and j at (r, theta) for a path point
+   with g = 1 / redshift.  or_rt_from_cache_mev: per ray of the cache the MeV
+   columns tau_mev, P_surv_mev, I_obs_mev, r_neutrinosphere_rg, leakage_mev
+   (mev_out[5 k + c]) as integrate_kerr_ray_spectral reports them. */
+void or_mev_point(const hadros_scene* s, const hadros_mev* m, double r, double th, double g, double out[4]);
+int or_rt_from_cache_mev(const hadros_cache_view* c, const hadros_scene* s, const double* sig_e, const double* sig_s,
+                         int n_sig, const hadros_mev* m, int nthreads, double* mev_out);
+
 /* apps/extract_opacity_surface.cpp:127-206: per polar angle theta_k =
    pi k / (ntheta - 1), the radial optical depth from r to R_max of a nu of
    energy E on the uniform grid r_i = R_min + i dr (nr points, trapezoidal

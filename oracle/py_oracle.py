@@ -60,6 +60,11 @@ def lib():
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.or_sigma.argtypes = [_dp, _dp, C.c_int, C.c_double]
         L.or_rho.argtypes = [C.POINTER(abi.Medium), C.c_double, C.c_double]
+        L.or_mev_point.argtypes = [C.POINTER(abi.Scene), C.POINTER(abi.Mev), C.c_double, C.c_double, C.c_double,
+                                   _dp]
+        L.or_mev_point.restype = None
+        L.or_rt_from_cache_mev.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Scene), _dp, _dp, C.c_int,
+                                           C.POINTER(abi.Mev), C.c_int, _dp]
         L.or_opacity_surface.argtypes = [C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double, C.c_double,
                                          C.c_double, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.or_spectral_weight.argtypes = [C.POINTER(abi.Scene), C.c_double]
@@ -101,6 +106,9 @@ def ref():
                                       C.c_double, C.c_double, C.c_int, C.c_int, _dp, C.c_long,
                                       C.POINTER(C.c_int), _dp]
         R.ref_sigma.argtypes = [C.c_char_p, _dp, _dp, C.c_long]
+        R.ref_mev_point.argtypes = [C.c_char_p, _dp, C.POINTER(C.c_char_p), _dp, _dp, _dp, _dp, _dp, C.c_long]
+        R.ref_integrate_mev.argtypes = [C.c_char_p, C.c_char_p, _dp, C.c_char_p, _dp, C.c_char_p, _dp, C.c_double,
+                                        C.c_double, C.c_double, C.POINTER(C.c_char_p), _dp, _dp, C.c_long, _dp]
         R.ref_rho.argtypes = [C.c_char_p, _dp, _dp, _dp, _dp, C.c_long]
         R.ref_emissivity.argtypes = [C.c_char_p, _dp, C.c_char_p, _dp, C.c_char_p, _dp, _dp, _dp, _dp,
                                      _dp, _dp, _dp, C.c_long]
@@ -196,6 +204,30 @@ def rt_from_cache(view, scene, sig_e, sig_s, nthreads=0):
         raise RuntimeError(f"or_rt_from_cache failed: {rc}")
     nb = bins(scene, sig_e)[0].size
     return img.trim_bins(nb)
+
+
+def rt_from_cache_mev(view, scene, sig_e, sig_s, mev, nthreads=0):
+    """Oracle MeV columns per ray of a cache: [n_rays x 5] (tau_mev, P_surv_mev,
+    I_obs_mev, r_neutrinosphere_rg, leakage_mev)."""
+    L = lib()
+    sig_e = np.ascontiguousarray(sig_e, np.float64)
+    sig_s = np.ascontiguousarray(sig_s, np.float64)
+    out = np.zeros((view.n_rays, 5))
+    rc = L.or_rt_from_cache_mev(C.byref(view), C.byref(scene), dptr(sig_e), dptr(sig_s), sig_e.size, C.byref(mev),
+                                nthreads, dptr(out))
+    if rc:
+        raise RuntimeError(f"or_rt_from_cache_mev failed: {rc}")
+    return out
+
+
+def mev_point(scene, mev, r, th, g):
+    """Oracle (T_MeV, Ye, kappa, j) per point, [n x 4]."""
+    L = lib()
+    r, th, g = (np.atleast_1d(np.asarray(x, np.float64)) for x in (r, th, g))
+    out = np.zeros((r.size, 4))
+    for k in range(r.size):
+        L.or_mev_point(C.byref(scene), C.byref(mev), r[k], th[k], g[k], dptr(out[k]))
+    return out
 
 
 def integrate_ray(scene, sig_e, sig_s, a_bh, pts):

@@ -10,6 +10,7 @@
 #include "kerr_camera.hpp"
 #include "kerr_geodesic.hpp"
 #include "kerr_metric.hpp"
+#include "mev_neutrino_physics.hpp"
 #include "radiative_transfer.hpp"
 #include "sigma_table.hpp"
 #include "torus_profile.hpp"
@@ -189,6 +190,111 @@ int ref_integrate(const char* sigma_path, const char* profile, const double* tp,
             p.dl_rg = o[5]; p.redshift_factor = o[6];
         }
         MeVThermalParams mev;
+        RTResult rt = radiative_transfer::integrate_kerr_ray_spectral(ray, enu, mbh, t, sig, so, mev, spec);
+        out8[0] = rt.tau; out8[1] = rt.P_surv; out8[2] = rt.I_obs; out8[3] = rt.tau_mev;
+        out8[4] = rt.P_surv_mev; out8[5] = rt.I_obs_mev; out8[6] = rt.r_neutrinosphere_rg;
+        out8[7] = rt.leakage_factor;
+    } catch (...) {
+        return 1;
+    }
+    return 0;
+}
+
+// MeV parameters: names of model, flavor, thermal profile, Ye profile and
+// spectral mode, then mp = enu, norm, sigma_abs0, sigma_scat0, sigma_scat_e0,
+// neutrinosphere_tau, T0, T_floor, T_power, Ye_torus, Ye_funnel, Ye_envelope,
+// Ye_floor, Ye_ceil, E_min, E_max, n_bins, process bits (HADROS_MEV_* order:
+// urca, pair, brems, absorption, scattering, degeneracy, abs_n, abs_p,
+// scat_n, scat_p, scat_e).
+static void fill_mev(MeVThermalParams& m, const char* const* names, const double* mp)
+{
+    m.model_name = names[0];
+    m.model = mev_neutrino::parse_mev_model(m.model_name);
+    m.flavor_name = names[1];
+    m.flavor = mev_neutrino::parse_mev_flavor(m.flavor_name);
+    m.thermal_profile_name = names[2];
+    m.thermal_profile = mev_neutrino::parse_mev_thermal_profile(m.thermal_profile_name);
+    m.ye_profile_name = names[3];
+    m.ye_profile = mev_neutrino::parse_mev_ye_profile(m.ye_profile_name);
+    m.spectral_mode_name = names[4];
+    m.spectral_mode = mev_neutrino::parse_mev_spectral_mode(m.spectral_mode_name);
+    m.Enu_obs_MeV = mp[0]; m.norm = mp[1]; m.sigma_abs0_cm2 = mp[2]; m.sigma_scat0_cm2 = mp[3];
+    m.neutrinosphere_tau = mp[5]; m.T0_MeV = mp[6]; m.T_floor_MeV = mp[7]; m.T_power = mp[8];
+    m.Ye_torus = mp[9]; m.Ye_funnel = mp[10]; m.Ye_envelope = mp[11]; m.Ye_floor = mp[12]; m.Ye_ceil = mp[13];
+    m.E_min_MeV = mp[14]; m.E_max_MeV = mp[15]; m.n_bins = int(mp[16]);
+    const unsigned bits = unsigned(mp[17]);
+    m.include_urca = bits & 1; m.include_pair = bits & 2; m.include_brems = bits & 4;
+    m.include_absorption = bits & 8; m.include_scattering = bits & 16; m.use_degeneracy_correction = bits & 32;
+    m.include_abs_n = bits & 64; m.include_abs_p = bits & 128; m.include_scat_n = bits & 256;
+    m.include_scat_p = bits & 512; m.include_scat_e = bits & 1024;
+}
+
+// the same parameters as the physics functions take them (field copy)
+static mev_neutrino::MeVPhysicsParams physics_params(const MeVThermalParams& m, double sigma_scat_e0)
+{
+    mev_neutrino::MeVPhysicsParams p;
+    p.model = m.model; p.flavor = m.flavor; p.include_urca = m.include_urca; p.include_pair = m.include_pair;
+    p.include_brems = m.include_brems; p.include_absorption = m.include_absorption;
+    p.include_scattering = m.include_scattering; p.use_degeneracy_correction = m.use_degeneracy_correction;
+    p.include_abs_n = m.include_abs_n; p.include_abs_p = m.include_abs_p; p.include_scat_n = m.include_scat_n;
+    p.include_scat_p = m.include_scat_p; p.include_scat_e = m.include_scat_e; p.norm = m.norm;
+    p.sigma_abs0_cm2 = m.sigma_abs0_cm2; p.sigma_scat0_cm2 = m.sigma_scat0_cm2; p.sigma_scat_e0_cm2 = sigma_scat_e0;
+    p.thermal_profile = m.thermal_profile; p.ye_profile = m.ye_profile; p.spectral_mode = m.spectral_mode;
+    p.T0_MeV = m.T0_MeV; p.T_floor_MeV = m.T_floor_MeV; p.T_power = m.T_power; p.Ye_torus = m.Ye_torus;
+    p.Ye_funnel = m.Ye_funnel; p.Ye_envelope = m.Ye_envelope; p.Ye_floor = m.Ye_floor; p.Ye_ceil = m.Ye_ceil;
+    p.E_min_MeV = m.E_min_MeV; p.E_max_MeV = m.E_max_MeV; p.n_bins = m.n_bins;
+    return p;
+}
+
+// per point: T, Ye, total opacity and total emissivity at local energy E[k]
+int ref_mev_point(const char* profile, const double* tp, const char* const* names, const double* mp,
+                  const double* r, const double* th, const double* E, double* out4, long n)
+{
+    try {
+        TorusProfile t = make_torus(profile, tp);
+        MeVThermalParams m;
+        fill_mev(m, names, mp);
+        const mev_neutrino::MeVPhysicsParams P = physics_params(m, mp[4]);
+        for (long k = 0; k < n; ++k) {
+            const double rho = t.rho(r[k], th[k]);
+            const double T = mev_neutrino::mev_temperature_profile_MeV(r[k], th[k], rho, t.rho0_gcm3(), t.r0_rg(),
+                                                                        t.r_min_rg(), t.r_max_rg(), P);
+            const double Ye = mev_neutrino::mev_ye_profile(r[k], th[k], rho, t.rho0_gcm3(), t.r0_rg(), t.r_min_rg(),
+                                                           t.r_max_rg(), P);
+            out4[4 * k] = T;
+            out4[4 * k + 1] = Ye;
+            out4[4 * k + 2] = mev_neutrino::mev_total_opacity_cm_inv(rho, Ye, E[k], P);
+            out4[4 * k + 3] = mev_neutrino::mev_total_emissivity(rho, T, Ye, E[k], P);
+        }
+    } catch (...) {
+        return 1;
+    }
+    return 0;
+}
+
+// integrate_kerr_ray_spectral with MeV parameters; out8 as ref_integrate
+int ref_integrate_mev(const char* sigma_path, const char* profile, const double* tp, const char* source,
+                      const double* sp, const char* spectral, const double* spp, double enu, double mbh, double a_bh,
+                      const char* const* mev_names, const double* mp, const double* pts, long n, double* out8)
+{
+    try {
+        SigmaTable sig(sigma_path);
+        TorusProfile t = make_torus(profile, tp);
+        UHESourceParams so;
+        fill_source(so, source, sp);
+        UHESpectralParams spec;
+        fill_spectral(spec, spectral, spp);
+        RayPath ray;
+        ray.a_bh = a_bh;
+        ray.points.resize(size_t(n));
+        for (long k = 0; k < n; ++k) {
+            const double* o = pts + 7 * k;
+            PathPoint& p = ray.points[k];
+            p.r_rg = o[0]; p.theta = o[1]; p.x_rg = o[2]; p.y_rg = o[3]; p.z_rg = o[4];
+            p.dl_rg = o[5]; p.redshift_factor = o[6];
+        }
+        MeVThermalParams mev;
+        fill_mev(mev, mev_names, mp);
         RTResult rt = radiative_transfer::integrate_kerr_ray_spectral(ray, enu, mbh, t, sig, so, mev, spec);
         out8[0] = rt.tau; out8[1] = rt.P_surv; out8[2] = rt.I_obs; out8[3] = rt.tau_mev;
         out8[4] = rt.P_surv_mev; out8[5] = rt.I_obs_mev; out8[6] = rt.r_neutrinosphere_rg;

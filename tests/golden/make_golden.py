@@ -302,7 +302,82 @@ def opacity_fixture(tmp):
     return d
 
 
+from mev_cases import MEV_BITS, MEV_CASES, MEV_MEDIA, mev_params, mev_scene  # noqa: E402
+
+
+def mev_fixture(R, tmp):
+    """Point-wise T, Ye, kappa, j and whole-ray MeV columns from the reference's
+    own mev_neutrino / integrate_kerr_ray_spectral, plus its CLI rows with the
+    MeV argv set."""
+    d = {}
+    rng = np.random.default_rng(41)
+    npt = 48
+    r = rng.uniform(1.5, 70.0, npt)
+    th = rng.uniform(0.0, np.pi, npt)
+    g = rng.uniform(0.4, 2.5, npt)
+    d["r"], d["th"], d["g"] = r, th, g
+    _, cache_path, rays = kgeo_fixture(tmp)
+    sig_path = os.path.join(tmp, "sigma_synthetic.dat")
+    from hadros_amd import workloads
+    workloads.write_sigma_file(sig_path, *workloads.synthetic_sigma())
+    for name, (names, over) in MEV_CASES.items():
+        mp, dd, flags, bits = mev_params(over)
+        cn = (C.c_char_p * 5)(*[x.encode() for x in names])
+        d[name + "_names"] = np.array(names)
+        d[name + "_mp"] = mp
+        E = dd["enu"] / np.maximum(g, 1e-300)
+        for mi, (prof, rho0) in enumerate(MEV_MEDIA):
+            sc = mev_scene(prof, rho0)
+            tp = np.array(abi.medium_params(sc.medium))
+            out = np.zeros((npt, 4))
+            assert R.ref_mev_point(prof.encode(), po.dptr(tp), cn, po.dptr(mp), po.dptr(r), po.dptr(th),
+                                   po.dptr(E), po.dptr(out), npt) == 0
+            d[f"{name}_point{mi}"] = out
+        # whole rays: a dense medium (the neutrinosphere is crossed) and a thin one
+        for si, (prof, rho0, spec) in enumerate((("gaussian", 1e12, "monochromatic"),
+                                                 ("powerlaw_funnel_envelope", 1e3, "powerlaw_cutoff"))):
+            sc = mev_scene(prof, rho0, "funnel_wall", spec)
+            tp = np.array(abi.medium_params(sc.medium))
+            sp = np.array(abi.source_params(sc.source))
+            spp = np.array(abi.spectrum_params(sc.spectrum))
+            res = np.zeros((len(rays), 8))
+            for k, ray in enumerate(rays):
+                pts = np.ascontiguousarray(ray[7])
+                assert R.ref_integrate_mev(sig_path.encode(), prof.encode(), po.dptr(tp), b"funnel_wall",
+                                           po.dptr(sp), spec.encode(), po.dptr(spp), 1e9, 3.0, 0.9, cn, po.dptr(mp),
+                                           po.dptr(pts), pts.shape[0], po.dptr(res[k])) == 0
+            d[f"{name}_rt{si}"] = res
+        # the CLI with argv 45-71 set, on the dense gaussian medium
+        argv = image_argv(dict(profile="gaussian", source="funnel_wall", spectrum="monochromatic", rho0="1e12"),
+                          cache_path)
+        argv = [sig_path if x == "SIGMA" else x for x in argv]
+        argv[13] = repr(dd["enu"])
+        argv[14] = repr(dd["norm"])
+        argv += [names[0], names[1]] + [str(flags[b]) for b in MEV_BITS[:5]] + [names[2], names[3]]
+        argv += [repr(dd[k]) for k in ("T0", "T_floor", "T_power", "Ye_torus", "Ye_funnel", "Ye_envelope",
+                                        "Ye_floor", "Ye_ceil")]
+        argv += [names[4], repr(dd["E_min"]), repr(dd["E_max"]), str(dd["n_bins"])]
+        argv += [str(flags[b]) for b in MEV_BITS[5:]]
+        os.makedirs(os.path.join(tmp, "output", "images"), exist_ok=True)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        out = subprocess.run([os.path.join(REF_BIN, "compute_kerr_image_from_cache")] + argv, cwd=tmp, check=True,
+                             capture_output=True, text=True, env=env).stdout
+        saved = re.search(r"Saved: (\S+)", out).group(1)
+        with open(os.path.join(tmp, saved)) as f:
+            lines = f.read().splitlines()
+        d[name + "_argv"] = np.array(["SIGMA" if x == sig_path else "CACHE" if x == cache_path else x
+                                      for x in argv])
+        d[name + "_rows"] = np.array([[float(x) for x in ln.split()] for ln in lines if not ln.startswith("#")])
+        d[name + "_header"] = np.array([ln for ln in lines if ln.startswith("#")])
+        d[name + "_filename"] = np.array([saved])
+    return d
+
+
 def main():
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "mev":
+        with tempfile.TemporaryDirectory() as tmp:
+            np.savez_compressed(os.path.join(HERE, "mev.npz"), **mev_fixture(po.ref(), tmp))
+        return
     if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "opacity":
         with tempfile.TemporaryDirectory() as tmp:
             np.savez_compressed(os.path.join(HERE, "opacity_surfaces.npz"), **opacity_fixture(tmp))
@@ -334,6 +409,8 @@ def main():
         imd.update(integrate_fixture(R, rays, sig_path))
         np.savez_compressed(os.path.join(HERE, "image_rows.npz"), **imd)
         np.savez_compressed(os.path.join(HERE, "opacity_surfaces.npz"), **opacity_fixture(tmp))
+    with tempfile.TemporaryDirectory() as tmp:
+        np.savez_compressed(os.path.join(HERE, "mev.npz"), **mev_fixture(R, tmp))
     for fn in sorted(os.listdir(HERE)):
         print(fn, os.path.getsize(os.path.join(HERE, fn)))
 

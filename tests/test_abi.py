@@ -36,6 +36,28 @@ def test_struct_sizes_match_header_layout():
     assert C.sizeof(abi.Spectrum) == 8 + 8 * 4
     assert C.sizeof(abi.ImageOut) == 8 * 11
     assert C.sizeof(abi.Stats) == 8 * 4 + 4 * 4
+    assert C.sizeof(abi.Mev) == 4 * 8 + 8 * 16
+    assert C.sizeof(abi.MevOut) == 8 * 5
+
+
+def test_mev_names_and_defaults_match_reference():
+    """parse_mev_* names and aliases (src/mev_neutrino_physics.cpp:36-233) and the
+    MeVThermalParams defaults (include/radiative_transfer.hpp:66-106) through the C-ABI."""
+    import ctypes as C
+
+    L = ffi.lib()
+    for fn, table in (("model", abi.MEV_MODELS), ("flavor", abi.MEV_FLAVORS), ("thermal", abi.MEV_THERMAL),
+                      ("ye", abi.MEV_YE), ("spectral", abi.MEV_SPECTRAL)):
+        for name, k in table.items():
+            v = C.c_int32(-1)
+            assert getattr(L, "hadros_parse_mev_" + fn)(name.encode(), C.byref(v)) == 0 and v.value == k
+        v = C.c_int32(-1)
+        assert getattr(L, "hadros_parse_mev_" + fn)(b"Physical_", C.byref(v)) == abi.ERR_KIND
+    m = abi.Mev()
+    L.hadros_mev_defaults(C.byref(m))
+    ref = abi.mev()
+    for f, _ in abi.Mev._fields_:
+        assert getattr(m, f) == getattr(ref, f), f
 
 
 def test_name_parsing_matches_reference_aliases():

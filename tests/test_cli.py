@@ -83,6 +83,28 @@ def test_stage2_unknown_names_and_bad_table(work):
     assert r.returncode == 1 and "Unknown UHE source model 'ringworld'" in r.stderr
 
 
+@pytest.mark.parametrize("pos,value,msg", [
+    (45, "quantum", "Unknown MEV_MODEL 'quantum'. Use toy or physical."),
+    (46, "nu_tau", "Unknown MEV_FLAVOR 'nu_tau'. Use nu_e, anti_nu_e, or nu_x."),
+    (52, "lukewarm", "Unknown MEV_THERMAL_PROFILE 'lukewarm'."),
+    (53, "salty", "Unknown MEV_YE_PROFILE 'salty'."),
+    (62, "rainbow", "Unknown MEV_SPECTRAL_MODE 'rainbow'. Use monochromatic or fermi_dirac_band."),
+])
+def test_stage2_unknown_mev_names(work, pos, value, msg):
+    """argv 45-71 name the MeV channel (apps/compute_kerr_image_from_cache.cpp:201-305); the
+    reference's parsers reject unknown names with these messages (src/mev_neutrino_physics.cpp:36-233)."""
+    cache = str(work / "c.bin")
+    kgeo.write(cache, 1, 2, 0.9, good_rays())
+    argv = image_args(cache) + ["inner_ring", "20", "1", "2", "0.1", "1", "-1", "0", "100", "monochromatic", "2",
+                                "1e12", "1e5", "1e12", "8"]
+    argv += ["physical", "anti_nu_e", "1", "1", "1", "1", "1", "inner_hot_torus", "neutron_rich_torus", "6", "0.1",
+             "0.2", "0.25", "0.55", "0.45", "0.01", "0.6", "monochromatic"]
+    assert len(argv) == 62
+    argv[pos - 1] = value
+    r = run("compute_kerr_image_from_cache", argv, work)
+    assert r.returncode == 1 and msg in r.stderr, (r.stdout, r.stderr)
+
+
 def test_cli_without_gpu_fails_loudly(work):
     from conftest import gpu_available
 

@@ -99,9 +99,11 @@ def test_stage2_matches_reference_image_files(work):
         rows = np.array([[float(v) for v in ln.split()] for ln in lines if not ln.startswith("#")])
         ref = im[name + "_rows"]
         assert np.array_equal(rows[:, [0, 1, 2, 3, 7]], ref[:, [0, 1, 2, 3, 7]])
-        for c in (4, 5, 6):
+        # UHE columns and the MeV columns (the reference's default MeV channel)
+        for c in (4, 5, 6, 8, 9, 10):
             scale = np.maximum(np.abs(ref[:, c]), 1e-300)
             assert np.all(np.abs(rows[:, c] - ref[:, c]) <= 2e-8 * scale + 1e-300), (name, c)
+        assert np.array_equal(rows[:, [11, 12]], ref[:, [11, 12]]), name
 
 
 def test_direct_mode_kimg_and_rows(work):
