@@ -25,6 +25,12 @@ namespace hd {
 constexpr double PI = 3.141592653589793238462643383279502884;
 constexpr double M_U_G = 1.66053906660e-24;   // atomic mass unit [g]
 
+// Products feeding a sum may fuse into FMAs (one rounding instead of two) in
+// the functions that open with HD_CONTRACT: the CFG integrator, the per-point
+// geometry and the transfer, all compared with the oracle to a tolerance.  The
+// REF-mode integrator keeps the written operation order.
+#define HD_CONTRACT _Pragma("clang fp contract(fast)")
+
 __host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __host__ __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
 __host__ __device__ __forceinline__ double dclamp(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
@@ -216,6 +222,7 @@ __device__ __forceinline__ Y5 rhs_ref(double a, const Y5& y, double pt, double p
 // (the contraction g^{mu nu} p_mu p_nu of src/kerr_metric.cpp:77-116 times Sigma).
 __device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double pph)
 {
+    HD_CONTRACT
     const double r = y.r;
     double s, c;
     sincos_m(y.th, &s, &c);
@@ -261,9 +268,50 @@ __device__ __forceinline__ Y5 axpy1(const Y5& y, double a1, const Y5& k1)
 
 // Fehlberg 4(5) attempt (tableau of src/kerr_geodesic.cpp:239-308), stage
 // sums left to right as the reference evaluates them.
+// the stage sums of rkf45 below, with FMA contraction allowed (CFG mode)
+__device__ __forceinline__ void rkf45_cfg(double a, const Y5& y, double pt, double pph, double h, Y5& y4, Y5& y5)
+{
+    HD_CONTRACT
+    const Y5 k1 = rhs_cfg(a, y, pt, pph);
+    const double a21 = h * 1.0 / 4.0;
+    const Y5 k2 = rhs_cfg(a, axpy1(y, a21, k1), pt, pph);
+    const double a31 = h * 3.0 / 32.0, a32 = h * 9.0 / 32.0;
+    Y5 t;
+#define HD_ST3(F) t.F = y.F + a31 * k1.F + a32 * k2.F
+    HD_ST3(r); HD_ST3(th); HD_ST3(ph); HD_ST3(pr); HD_ST3(pth);
+    const Y5 k3 = rhs_cfg(a, t, pt, pph);
+    const double a41 = h * 1932.0 / 2197.0, a42 = h * -7200.0 / 2197.0, a43 = h * 7296.0 / 2197.0;
+#define HD_ST4(F) t.F = y.F + a41 * k1.F + a42 * k2.F + a43 * k3.F
+    HD_ST4(r); HD_ST4(th); HD_ST4(ph); HD_ST4(pr); HD_ST4(pth);
+    const Y5 k4 = rhs_cfg(a, t, pt, pph);
+    const double a51 = h * 439.0 / 216.0, a52 = h * -8.0, a53 = h * 3680.0 / 513.0, a54 = h * -845.0 / 4104.0;
+#define HD_ST5(F) t.F = y.F + a51 * k1.F + a52 * k2.F + a53 * k3.F + a54 * k4.F
+    HD_ST5(r); HD_ST5(th); HD_ST5(ph); HD_ST5(pr); HD_ST5(pth);
+    const Y5 k5 = rhs_cfg(a, t, pt, pph);
+    const double a61 = h * -8.0 / 27.0, a62 = h * 2.0, a63 = h * -3544.0 / 2565.0, a64 = h * 1859.0 / 4104.0,
+                 a65 = h * -11.0 / 40.0;
+#define HD_ST6(F) t.F = y.F + a61 * k1.F + a62 * k2.F + a63 * k3.F + a64 * k4.F + a65 * k5.F
+    HD_ST6(r); HD_ST6(th); HD_ST6(ph); HD_ST6(pr); HD_ST6(pth);
+    const Y5 k6 = rhs_cfg(a, t, pt, pph);
+#define HD_Y4(F) y4.F = y.F + h * (25.0 / 216.0 * k1.F + 1408.0 / 2565.0 * k3.F + 2197.0 / 4104.0 * k4.F - 1.0 / 5.0 * k5.F)
+#define HD_Y5(F) y5.F = y.F + h * (16.0 / 135.0 * k1.F + 6656.0 / 12825.0 * k3.F + 28561.0 / 56430.0 * k4.F - 9.0 / 50.0 * k5.F + 2.0 / 55.0 * k6.F)
+    HD_Y4(r); HD_Y4(th); HD_Y4(ph); HD_Y4(pr); HD_Y4(pth);
+    HD_Y5(r); HD_Y5(th); HD_Y5(ph); HD_Y5(pr); HD_Y5(pth);
+#undef HD_ST3
+#undef HD_ST4
+#undef HD_ST5
+#undef HD_ST6
+#undef HD_Y4
+#undef HD_Y5
+}
+
 template <int MODE>
 __device__ __forceinline__ void rkf45(double a, const Y5& y, double pt, double pph, double h, Y5& y4, Y5& y5)
 {
+    if constexpr (MODE == HADROS_MODE_CFG) {
+        rkf45_cfg(a, y, pt, pph, h, y4, y5);
+        return;
+    }
     const Y5 k1 = rhs<MODE>(a, y, pt, pph);
     const double a21 = h * 1.0 / 4.0;
     const Y5 k2 = rhs<MODE>(a, axpy1(y, a21, k1), pt, pph);
@@ -328,6 +376,7 @@ __device__ __forceinline__ double err_ref(const Y5& a, const Y5& b)
 // so only the winning ratio is divided out.
 __device__ __forceinline__ double err_cfg(const Y5& y, const Y5& y4, const Y5& y5, double tol)
 {
+    HD_CONTRACT
     double nb = 0.0, db = 1.0;
     bool bad = false;
 #define HD_E(F)                                                                          \
@@ -368,6 +417,7 @@ __device__ __forceinline__ double inv_fifth_root(double x)
 // src/kerr_camera.cpp:14-29
 __device__ __forceinline__ double zamo_energy(double a, double r, double th, double pt, double pph)
 {
+    HD_CONTRACT
     double alpha, omega;
     lapse_omega(a, r, th, alpha, omega);
     return -(pt + omega * pph) / alpha;
@@ -385,6 +435,7 @@ __device__ __forceinline__ double wrapped_dphi(double pn, double po)
 // src/kerr_camera.cpp:46-76
 __device__ __forceinline__ double zamo_dl(double a, const Y5& cur, const Y5& prev)
 {
+    HD_CONTRACT
     const Metric5 g = metric(a, 0.5 * (cur.r + prev.r), 0.5 * (cur.th + prev.th));
     const double dr = cur.r - prev.r;
     const double dth = cur.th - prev.th;
@@ -415,6 +466,7 @@ __host__ __device__ __forceinline__ double pow_pos(double x, double y) { return 
 // reference's form of order |exponent| x 1e-16).
 __host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, double th)
 {
+    HD_CONTRACT
     double shape;
     const double lr = (m.powerlaw_disk || m.envelope) ? log(r * m.inv_r0) : 0.0;   // ln(r / r0)
     if (m.kind == HADROS_PROFILE_COLLAPSAR_NDAF_LIKE) {
@@ -463,6 +515,7 @@ __host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, dou
 // for w > 0, as two exponentials of summed exponents
 __device__ __forceinline__ double bipolar_times(double th, double th0, double inv_w, double base)
 {
+    HD_CONTRACT
     const double n = (th - th0) * inv_w;
     const double s = (th - (PI - th0)) * inv_w;
     return exp(base - n * n) + exp(base - s * s);
@@ -474,6 +527,7 @@ __device__ __forceinline__ double bipolar_times(double th, double th0, double in
 __device__ __forceinline__ double source_factor(const DevSource& so, const DevMedium& m, double r, double th,
                                                 double rho_here)
 {
+    HD_CONTRACT
     if (so.kind == HADROS_SOURCE_INNER_RING) {
         if (!so.ring_ok) return 0.0;
         const double dr = (r - so.rc) * so.inv_sr;
@@ -562,6 +616,7 @@ __device__ __forceinline__ double sigma_ln(const double* __restrict__ lnE, const
                                            const double* __restrict__ slope, int n, double lx, double lo0 = 0.0,
                                            double inv_d = 0.0)
 {
+    HD_CONTRACT
     const int lo = sigma_lower_bound(lnE, n, lx, lo0, inv_d);
     if (lo == 0) return lnS[0];
     if (lo >= n) return lnS[n - 1];
@@ -572,6 +627,7 @@ __device__ __forceinline__ float sigma_ln_f(const float* __restrict__ lnE, const
                                             const float* __restrict__ slope, int n, float lx, float lo0 = 0.0f,
                                             float inv_d = 0.0f)
 {
+    HD_CONTRACT
     const int lo = sigma_lower_bound(lnE, n, lx, lo0, inv_d);
     if (lo == 0) return lnS[0];
     if (lo >= n) return lnS[n - 1];
@@ -603,6 +659,7 @@ __device__ __forceinline__ double mev_shape(const DevMev& M, double rho)
 // mev_temperature_profile_MeV (src/mev_neutrino_physics.cpp:235-286)
 __device__ __forceinline__ double mev_temperature(const DevMev& M, double r, double th, double rho)
 {
+    HD_CONTRACT
     const double T0 = dmax(M.T0, 1.0e-12);
     const double floor_ = dmax(M.T_floor, 1.0e-12);
     const double shape = mev_shape(M, rho);
@@ -632,6 +689,7 @@ __device__ __forceinline__ double mev_temperature(const DevMev& M, double r, dou
 // mev_ye_profile (src/mev_neutrino_physics.cpp:288-336)
 __device__ __forceinline__ double mev_ye(const DevMev& M, double r, double th, double rho)
 {
+    HD_CONTRACT
     const double shape = mev_shape(M, rho);
     double Ye = M.Ye_torus;
     if (M.ye == HADROS_MEV_YE_NEUTRON_RICH_TORUS) {
@@ -669,6 +727,7 @@ __device__ __forceinline__ double mev_fd(double E, double T)
 // brems terms and the legacy toy branch)
 __device__ __forceinline__ double mev_emissivity(const DevMev& M, double rho, double T, double Ye, double E)
 {
+    HD_CONTRACT
     if (M.model == HADROS_MEV_MODEL_TOY) {
         if (rho <= 0.0 || T <= 0.0 || E <= 0.0) return 0.0;
         const double nb = rho / M_U_G;
@@ -716,6 +775,7 @@ __device__ __forceinline__ double mev_emissivity(const DevMev& M, double rho, do
 // (src/mev_neutrino_physics.cpp:595-769)
 __device__ __forceinline__ double mev_opacity(const DevMev& M, double rho, double Ye, double E)
 {
+    HD_CONTRACT
     if (rho <= 0.0 || E <= 0.0) return 0.0;
     const double nb = rho / M_U_G;
     if (M.model == HADROS_MEV_MODEL_TOY) {
@@ -746,6 +806,7 @@ __device__ __forceinline__ double mev_opacity(const DevMev& M, double rho, doubl
 __device__ __forceinline__ void mev_kappa_j(const DevMev& M, double g, double rho, double T, double Ye, double& kappa,
                                             double& j)
 {
+    HD_CONTRACT
     const double gc = dmax(g, 1.0e-300);
     if (M.spectral == HADROS_MEV_SPECTRAL_FERMI_DIRAC_BAND) {
         const int nbins = M.n_bins > 1 ? M.n_bins : 1;
@@ -782,6 +843,7 @@ struct PointQ {
 
 __device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, double dl, double z, PointQ& q)
 {
+    HD_CONTRACT
     if (r < P.r_h_rt + 1.0e-3) return false;
     const double zc = dmax(z, 1.0e-300);
     const double g = 1.0 / zc;

@@ -50,6 +50,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
                                                          const double* slope, const float* lnEf, const float* lnSf,
                                                          const float* slopef)
 {
+    HD_CONTRACT
     const int n = P.n_sigma;
     const int nb = P.nb;
     // points of this ray: [b0, b1) of the SoA arrays, or n_st points in linked chunks
