@@ -1,0 +1,69 @@
+// fastmath.h -- table-driven FP64 exponential for the transfer kernels.
+//
+// e^x by Tang's table-driven method (P. T. P. Tang, "Table-driven
+// implementation of the exponential function in IEEE floating-point
+// arithmetic", ACM TOMS 15(2):144-157, 1989) with a 64-entry table:
+//
+//   x = (64 m + j) ln2/64 + r,   |r| <= ln2/128,   e^x = 2^m 2^(j/64) e^r,
+//
+// e^r by its degree-5 Taylor polynomial (truncation below 3.6e-17 relative
+// for |r| <= ln2/128).  The reduction x - k ln2/64 uses a two-part ln2/64
+// whose high part has 33 significant bits, so k L1 is exact for |k| < 2^17
+// (all |x| < 745).  Error: the table entry (0.5 ulp), the polynomial's last
+// rounding (0.5 ulp) and the product (0.5 ulp): under 2 ulp.  About 18 VALU
+// instructions and one LDS read, against about 22-31 for the general exp,
+// whose degree-11 polynomial covers |r| <= ln2/2.
+//
+// Domain: finite x (x > 709.78 gives +inf) and x = -inf; every x below
+// -745.2 gives +0.  The kernels stage EXP_TAB into LDS and pass it as T.
+// The same code compiles for the host, where the unit test checks it
+// against a long-double exponential.
+#pragma once
+
+#if defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define HD_FM_QUAL __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define HD_FM_QUAL inline
+#endif
+
+namespace hd {
+
+// 2^(j/64), j = 0..63, each correctly rounded (from 60-digit decimal arithmetic)
+constexpr double EXP_TAB[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+};
+
+constexpr double EXP_INV_L = 0x1.71547652b82fep+6;   // 64 / ln 2
+constexpr double EXP_L1 = 0x1.62e42fef00000p-7;      // ln2/64, high 33 bits
+constexpr double EXP_L2 = 0x1.473de6af278edp-40;     // ln2/64 - EXP_L1
+constexpr double EXP_ZERO_BELOW = -745.2;            // e^x < 2^-1075 below this
+
+HD_FM_QUAL double exp_tab(double x, const double* T)
+{
+    const double k = rint(x * EXP_INV_L);
+    const double r = fma(-k, EXP_L2, fma(-k, EXP_L1, x));
+    const int ki = int(k);
+    const double p =
+        fma(r, fma(r, fma(r, fma(r, fma(r, 1.0 / 120.0, 1.0 / 24.0), 1.0 / 6.0), 0.5), 1.0), 1.0);
+    const double v = ldexp(T[ki & 63] * p, ki >> 6);
+    return x < EXP_ZERO_BELOW ? 0.0 : v;
+}
+
+}  // namespace hd

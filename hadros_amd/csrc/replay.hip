@@ -15,6 +15,7 @@
 // the chunk pool written by the split tracer (k_trace_points): 64-point
 // chunks of 32-byte (r, theta, dl, z) records, the first at the ray's own
 // index, the rest linked through next[].
+#include "fastmath.h"
 #include "kernels.h"
 #include "wave.h"
 
@@ -25,32 +26,66 @@ namespace hd {
 #endif
 
 constexpr int REPLAY_BLOCK = 256;
+
+// A scene block read through the constant address space: wave-uniform scalar loads.
+typedef const __attribute__((address_space(4))) DevRt* DevRtC;
 constexpr int REPLAY_WAVES = REPLAY_BLOCK / 64;
 
 // Per-wavefront, per-bin running sums of the current ray, kept in LDS so that
 // the bin loop stays rolled (few registers, more resident wavefronts).  Lane
 // 63 holds each chunk's inclusive totals and alone updates them; LDS
 // operations of one wavefront complete in order, so the next bin/chunk reads
-// see the update.
+// see the update.  The FP64 intensity is not summed per chunk: each lane adds
+// its own points' emission terms to its slot em[b * 64 + lane] (dynamic LDS),
+// and the 64 partial sums of a bin are added once, at the end of the ray.
 struct ReplayAcc64 {
-    double tau[HADROS_MAX_BINS], I[HADROS_MAX_BINS];
+    double tau[HADROS_MAX_BINS];
 };
+static_assert(HADROS_MAX_BINS <= 16, "the ray-end reduction keeps one bin per lane of a 16-lane row");
 struct ReplayAcc32 {
     Kahan tau[HADROS_MAX_BINS];
     ScaledKahan I[HADROS_MAX_BINS];
 };
 
+// The scene block for one chunk's per-point quantities.  The block is
+// invariant over the ray, but held in SGPRs across the chunk loop its medium
+// and source fields (every profile kind's) exceed the SGPR file and spill
+// into VGPR lanes, one VALU read-back per use.  Hiding the invariance from
+// the optimiser (an empty asm on the pointer) leaves them in memory, re-read
+// by scalar loads where the chunk uses them.
+__device__ __forceinline__ const DevRt& scene_per_chunk(DevRtC Pc)
+{
+    asm volatile("" : "+s"(Pc));
+    return *(const DevRt*)Pc;
+}
+
+// Doubles of dynamic LDS per wavefront for the FP64 emission partial sums:
+// one per lane and bin.
+__host__ __device__ constexpr int replay_emission_slots(int precision, int nb)
+{
+    return precision == HADROS_PREC_FP64 ? 64 * nb : 0;
+}
+
+// The batch's camera start table is staged in LDS when it is short and the
+// dynamic LDS stays within 56 KiB (64 KiB per workgroup with the static part).
+__host__ __device__ constexpr bool replay_pix0_in_lds(int precision, int nb, int n_sigma, int n_cams)
+{
+    return n_cams <= BATCH_LDS_CAMS &&
+           (long(REPLAY_WAVES) * replay_emission_slots(precision, nb) + 3L * n_sigma + n_cams + 1) * 8 <= 56L * 1024;
+}
+
 // One ray of the replay: points from the ray's source, transfer with the
 // scene's parameters P, outputs at global ray index g (bins nb_out apart).
 template <int PREC, int SRC, bool MEV>
-__device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, const DevRt& P, long ray, long g,
+__device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, DevRtC Pc, long ray, long g,
                                                          int nb_out, int lane, ReplayAcc64& a64, ReplayAcc32& a32,
                                                          double* bin_e, double* bin_lne, double* bin_w,
                                                          double* bin_de, const double* lnE, const double* lnS,
                                                          const double* slope, const float* lnEf, const float* lnSf,
-                                                         const float* slopef)
+                                                         const float* slopef, const double* exp_t, double* em)
 {
     HD_CONTRACT
+    const DevRt& P = *(const DevRt*)Pc;
     const int n = P.n_sigma;
     const int nb = P.nb;
     // points of this ray: [b0, b1) of the SoA arrays, or n_st points in linked chunks
@@ -62,9 +97,12 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
     } else {
         b1 = A.n_st[ray];
     }
+    if constexpr (PREC == HADROS_PREC_FP64) {
+#pragma unroll 1
+        for (int b = 0; b < nb; ++b) em[b * 64 + lane] = 0.0;
+    }
     if (lane < nb) {
         a64.tau[lane] = 0.0;
-        a64.I[lane] = 0.0;
         a32.tau[lane] = {0.0f, 0.0f};
         a32.I[lane] = {{0.0f, 0.0f}, 0.0f, 0};
         if constexpr (SRC == REPLAY_SRC_BATCH) {   // this ray's scene
@@ -91,7 +129,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
                 pr = A.r[q];
                 pth = A.th[q];
                 pdl = A.dl[q];
-                ok = point_q(P, pr, pth, pdl, A.z[q], pq);
+                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq);
             }
         } else {
             if (base + lane < b1) {
@@ -100,7 +138,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
                 pr = rt.x;
                 pth = rt.y;
                 pdl = dz.x;
-                ok = point_q(P, rt.x, rt.y, dz.x, dz.y, pq);
+                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq);
             }
             if (base + 64 < b1) chunk = A.next[chunk];
         }
@@ -152,7 +190,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
                 const double lnEb = bin_lne[b] + pq.lnz;
                 double dt = 0.0;
                 if (ok && pq.rho > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                    dt = pq.nbar_dl * exp(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d));
+                    dt = pq.nbar_dl * exp_tab(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d), exp_t);
                 double inc[1] = {dt};
                 wave_scan_add(inc);
                 const double tau_b = a64.tau[b];
@@ -161,17 +199,10 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
                 const double arg = -P.gam * lnEb - E * P.invcut - before;
                 // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
                 // every term is zero (behind an opaque column) adds nothing
-                double em[1] = {0.0};
                 const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
-                if (live) {
-                    if (emits) em[0] = pq.g3Adl * exp(arg);
-                    wave_scan_add(em);
-                }
+                if (live && emits) em[b * 64 + lane] += pq.g3Adl * exp_tab(arg, exp_t);
                 __builtin_amdgcn_wave_barrier();
-                if (lane == 63) {
-                    a64.tau[b] = tau_b + inc[0];
-                    if (live) a64.I[b] += em[0];
-                }
+                if (lane == 63) a64.tau[b] = tau_b + inc[0];
                 __builtin_amdgcn_wave_barrier();
             }
         } else {
@@ -212,37 +243,50 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, co
             }
         }
     }
-    if (lane == 0) {
-        double t0 = 0.0, p0 = 1.0, i0 = 0.0;
-        double ws = 0.0, ts = 0.0, ps = 0.0, is = 0.0;
-#pragma unroll 1
-        for (int b = 0; b < nb_out; ++b) {
-            double bt = 0.0, bi = 0.0, bp = 1.0;   // bins beyond the scene's own: tau 0, P 1, I 0
-            if (b < nb) {
-                if (PREC == HADROS_PREC_FP64) {
-                    bt = a64.tau[b];
-                    bi = a64.I[b];
-                } else {
-                    const Kahan kt = a32.tau[b];
-                    const ScaledKahan ki = a32.I[b];
-                    bt = double(kt.s) - double(kt.c);
-                    bi = ki.set ? (double(ki.k.s) - double(ki.k.c)) * exp(double(ki.off)) : 0.0;
-                }
-                bp = exp(-bt);
-                if (b == 0) {
-                    t0 = bt;
-                    p0 = bp;
-                    i0 = bi;
-                }
-                ws += bin_w[b];
-                ts += bin_w[b] * bt;
-                ps += bin_w[b] * bp;
-                is += bi * bin_de[b];
-            }
-            if (A.btau) A.btau[g * nb_out + b] = bt;
-            if (A.bp) A.bp[g * nb_out + b] = bp;
-            if (A.bi) A.bi[g * nb_out + b] = bi;
+    // Ray-end outputs, one lane per bin (lanes 0..nb_out-1 of the first row).
+    double bi = 0.0;
+    if constexpr (PREC == HADROS_PREC_FP64) {
+        // I_b: lanes b, b+16, b+32, b+48 each add 16 of the bin's 64 partial
+        // sums, then two cross-row steps add the four
+        const int bb = lane & 15;
+        if (bb < nb) {
+            const double* src = em + bb * 64 + (lane >> 4) * 16;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) bi += src[i];
         }
+        bi += __shfl_xor(bi, 16);
+        bi += __shfl_xor(bi, 32);
+    }
+    double bt = 0.0, bp = 1.0;   // bins beyond the scene's own: tau 0, P 1, I 0
+    double sums[4] = {0.0, 0.0, 0.0, 0.0};   // sum over bins of w, w tau, w P, I dE
+    if (lane < nb) {
+        if constexpr (PREC == HADROS_PREC_FP64) {
+            bt = a64.tau[lane];
+        } else {
+            const Kahan kt = a32.tau[lane];
+            const ScaledKahan ki = a32.I[lane];
+            bt = double(kt.s) - double(kt.c);
+            bi = ki.set ? (double(ki.k.s) - double(ki.k.c)) * exp(double(ki.off)) : 0.0;
+        }
+        bp = exp(-bt);
+        const double w = bin_w[lane];
+        sums[0] = w;
+        sums[1] = w * bt;
+        sums[2] = w * bp;
+        sums[3] = bi * bin_de[lane];
+    } else {
+        bi = 0.0;
+    }
+    if (lane < nb_out) {
+        if (A.btau) A.btau[g * nb_out + lane] = bt;
+        if (A.bp) A.bp[g * nb_out + lane] = bp;
+        if (A.bi) A.bi[g * nb_out + lane] = bi;
+    }
+    row_scan_add(sums);
+    const double ws = read_lane(sums[0], 15), ts = read_lane(sums[1], 15);
+    const double ps = read_lane(sums[2], 15), is = read_lane(sums[3], 15);
+    const double t0 = read_lane(bt, 0), p0 = read_lane(bp, 0), i0 = read_lane(bi, 0);
+    if (lane == 0) {
         double agg_t = 0.0, agg_p = 1.0, agg_i = 0.0;
         if (b1 > b0) {
             if (P.mono) {
@@ -283,6 +327,8 @@ k_replay(ReplayArgs A)
     // per-wavefront bin constants (the batch source changes scene per ray),
     // indexed by the rolled bin loop
     __shared__ double bin_tab[REPLAY_WAVES][4][HADROS_MAX_BINS];
+    __shared__ double exp_t[64];   // 2^(j/64) for exp_tab
+    if (threadIdx.x < 64) exp_t[threadIdx.x] = EXP_TAB[threadIdx.x];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double* bin_e = bin_tab[wv][0];
@@ -298,22 +344,27 @@ k_replay(ReplayArgs A)
             bin_de[lane] = P.de[lane];
         }
     }
-    // sigma table (shared by every scene), then the batch's camera start table
+    // dynamic LDS: the FP64 emission partial sums (replay_emission_slots per
+    // wavefront), the sigma table (shared by every scene), then the batch's
+    // camera start table
+    const int n_em = replay_emission_slots(PREC, SRC == REPLAY_SRC_BATCH ? A.nb_out : A.rt.nb);
+    double* em = smem + wv * n_em;
+    double* sig = smem + REPLAY_WAVES * n_em;
     const int n = A.rt.n_sigma;
-    const double* lnE = smem;
-    const double* lnS = smem + n;
-    const double* slope = smem + 2 * n;
-    const float* lnEf = reinterpret_cast<const float*>(smem);
+    const double* lnE = sig;
+    const double* lnS = sig + n;
+    const double* slope = sig + 2 * n;
+    const float* lnEf = reinterpret_cast<const float*>(sig);
     const float* lnSf = lnEf + n;
     const float* slopef = lnEf + 2 * n;
     if constexpr (PREC == HADROS_PREC_FP64) {
-        for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) smem[t] = A.sig[t];
+        for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) sig[t] = A.sig[t];
     } else {
-        float* sf = reinterpret_cast<float*>(smem);
+        float* sf = reinterpret_cast<float*>(sig);
         for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) sf[t] = A.sigf[t];
     }
-    long* pix0_lds = reinterpret_cast<long*>(smem + 3 * n);
-    const bool pix_in_lds = SRC == REPLAY_SRC_BATCH && A.n_cams <= BATCH_LDS_CAMS;
+    long* pix0_lds = reinterpret_cast<long*>(sig + 3 * n);
+    const bool pix_in_lds = SRC == REPLAY_SRC_BATCH && replay_pix0_in_lds(PREC, A.nb_out, n, A.n_cams);
     const long* pix0 = pix_in_lds ? pix0_lds : A.pix0;
     if (pix_in_lds)
         for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0_lds[t] = A.pix0[t];
@@ -321,6 +372,10 @@ k_replay(ReplayArgs A)
 
     ReplayAcc64& a64 = acc64[wv];
     ReplayAcc32& a32 = acc32[wv];
+    // A.rt where it lies in the kernel argument segment (the kernel's only
+    // explicit argument starts at offset 0)
+    typedef const __attribute__((address_space(4))) char* KernargC;
+    const DevRtC rt_c = (DevRtC)((KernargC)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(ReplayArgs, rt));
     const long nwaves = long(gridDim.x) * REPLAY_WAVES;
     unsigned long long pts_done = 0;
     for (long ray = long(blockIdx.x) * REPLAY_WAVES + wv; ray < A.n_rays; ray += nwaves) {
@@ -334,12 +389,11 @@ k_replay(ReplayArgs A)
             int p, q;
             batch_pixel(C, gq - pix0[c], p, q);
             const long g = pix0[c] + long(p) * C.sub_ny + q;
-            const DevRt* P = (const DevRt*)(const __attribute__((address_space(4))) DevRt*)(A.rts + C.pad_rt);
-            pts_done += replay_ray<PREC, SRC, MEV>(A, *P, ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef);
+            pts_done += replay_ray<PREC, SRC, MEV>(A, (DevRtC)(A.rts + C.pad_rt), ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w,
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, em);
         } else {
-            pts_done += replay_ray<PREC, SRC, MEV>(A, A.rt, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef);
+            pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, em);
         }
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);
@@ -366,8 +420,10 @@ hipError_t launch_replay(const ReplayArgs& A, int precision, hipStream_t s)
     if (grid < 1) grid = 1;
     const bool batch = A.rts != nullptr;
     // the FP32 sigma table takes the first half of the FP64 one's space
-    size_t lds = size_t(3) * A.rt.n_sigma * sizeof(double);
-    if (batch && A.n_cams <= BATCH_LDS_CAMS) lds += size_t(A.n_cams + 1) * sizeof(long);
+    const int n_em = replay_emission_slots(precision, batch ? A.nb_out : A.rt.nb);
+    size_t lds = (size_t(REPLAY_WAVES) * n_em + size_t(3) * A.rt.n_sigma) * sizeof(double);
+    if (batch && replay_pix0_in_lds(precision, A.nb_out, A.rt.n_sigma, A.n_cams))
+        lds += size_t(A.n_cams + 1) * sizeof(long);
     const bool chunks = A.pool != nullptr;
     if (precision == HADROS_PREC_FP64) {
         if (batch) launch_p<HADROS_PREC_FP64, REPLAY_SRC_BATCH>(A, int(grid), lds, s);

@@ -74,6 +74,19 @@ __device__ __forceinline__ void wave_scan_add(float (&x)[N])
 #undef HD_SCAN_STEP
 }
 
+// inclusive prefix sums within each row of 16 lanes: lane 15 of a row ends
+// with the row's total
+template <int N>
+__device__ __forceinline__ void row_scan_add(double (&x)[N])
+{
+#define HD_SHIFT_STEP(CTRL) _Pragma("unroll") for (int b = 0; b < N; ++b) x[b] += dpp_shift<CTRL>(x[b]);
+    HD_SHIFT_STEP(0x111)
+    HD_SHIFT_STEP(0x112)
+    HD_SHIFT_STEP(0x114)
+    HD_SHIFT_STEP(0x118)
+#undef HD_SHIFT_STEP
+}
+
 // inclusive prefix maxima (identity -inf)
 template <int N>
 __device__ __forceinline__ void wave_scan_max(float (&x)[N])
