@@ -24,13 +24,32 @@ namespace hd {
 
 struct BatchLane {
     long ray;        // global ray index in queue order, -1 = idle
-    long out;        // global output index of the ray's pixel
     int cam;
     int step, npts, captured, exhausted, att_in_step;
     long attempts;
-    double pt, pph, e_obs_c, h, r_stop;
+    double pt, pph, inv_e_obs, h;   // inv_e_obs: redshift factor of every point
     Y5 y, yp;
 };
+
+// the capture test of src/kerr_camera.cpp: r below the camera's stopping
+// radius (read from the camera when tested rather than held in registers)
+template <int MODE>
+__device__ __forceinline__ bool batch_inside_stop(const DevCam& C, double r)
+{
+    if constexpr (MODE == HADROS_MODE_CFG) return r < 1.01 * C.r_h;
+    else return r <= C.r_h + 1.0e-3;
+}
+
+// global output index of queue ray `ray` of camera `cam` (recomputed when the
+// ray finishes rather than held in registers while it is traced)
+__device__ __forceinline__ long batch_out_index(const DevCam* __restrict__ cams, const long* __restrict__ pix0, int cam,
+                                                long ray)
+{
+    const DevCam& C = cams[cam];
+    int p, q;
+    batch_pixel(C, ray - pix0[cam], p, q);
+    return pix0[cam] + long(p) * C.sub_ny + q;
+}
 
 template <int MODE>
 __device__ __forceinline__ void batch_init_ray(const DevCam* __restrict__ cams, int n_cams,
@@ -42,7 +61,6 @@ __device__ __forceinline__ void batch_init_ray(const DevCam* __restrict__ cams, 
     const DevCam& C = cams[lo];
     int p, q;   // lattice pixel of the camera's k-th ray in queue order
     batch_pixel(C, ray - pix0[lo], p, q);
-    L.out = pix0[lo] + long(p) * C.sub_ny + q;
     const int i = C.sub_offset + C.sub_stride * p;
     const int j = C.sub_offset + C.sub_stride * q;
     // src/kerr_camera.cpp:110-186 with the observer metric from the host
@@ -58,8 +76,7 @@ __device__ __forceinline__ void batch_init_ray(const DevCam* __restrict__ cams, 
     L.pph = C.g_tph * pt_c + C.g_phph * pph_c;
     L.y = Y5{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
     L.yp = L.y;
-    L.e_obs_c = dmax(-(L.pt + C.omega * L.pph) / C.alpha, 1.0e-300);
-    L.r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
+    L.inv_e_obs = 1.0 / dmax(-(L.pt + C.omega * L.pph) / C.alpha, 1.0e-300);
     L.h = C.h;
     L.step = 0;
     L.npts = 0;
@@ -130,7 +147,7 @@ k_trace_batch(BatchArgs A)
                             ktau[b] = {0.0f, 0.0f};
                             kI[b] = {{0.0f, 0.0f}, 0.0f, 0};
                         }
-                        const bool cap0 = (MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop);
+                        const bool cap0 = batch_inside_stop<MODE>(A.cams[L.cam], L.y.r);
                         if (cap0) L.captured = 1;
                         else L.npts = 1;
                     }
@@ -193,7 +210,7 @@ k_trace_batch(BatchArgs A)
                 if (L.step >= C.max_steps) {
                     L.exhausted = 1;
                     finished = true;
-                } else if ((MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop)) {
+                } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
                     L.captured = 1;
                     finished = true;
                 } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
@@ -201,7 +218,7 @@ k_trace_batch(BatchArgs A)
                 } else {
                     ++L.npts;
                     const double dl = zamo_dl(a, L.y, L.yp);
-                    const double z = zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) / L.e_obs_c;
+                    const double z = zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs;
                     const DevRt& P = A.rts[C.pad_rt];
                     PointQ q;
                     if (point_q(P, L.y.r, L.y.th, dl, z, q)) {
@@ -220,7 +237,7 @@ k_trace_batch(BatchArgs A)
         if (finished) {
             // ---- outputs of ray L.ray (aggregation of src/radiative_transfer.cpp:711-769)
             const DevRt& P = A.rts[A.cams[L.cam].pad_rt];
-            const long k = L.out;
+            const long k = batch_out_index(A.cams, pix0, L.cam, L.ray);
             double bt[NBMAX], bp[NBMAX], bi[NBMAX];
 #pragma unroll
             for (int b = 0; b < NBMAX; ++b) {
@@ -282,7 +299,7 @@ k_trace_batch(BatchArgs A)
 }
 
 #ifndef HD_BATCH_POINTS_WAVES_PER_EU
-#define HD_BATCH_POINTS_WAVES_PER_EU 3
+#define HD_BATCH_POINTS_WAVES_PER_EU 4
 #endif
 
 // Geodesic half of the split batch (the transfer runs afterwards in k_replay
@@ -309,7 +326,12 @@ k_batch_points(BatchPointsArgs A)
     ChunkWriter w;
     w.begin(-1);
     bool drained = false;   // wave-uniform: the queue is empty
-    unsigned long long c_pts = 0, c_att = 0, c_cap = 0, c_ex = 0;
+    // per-thread points / attempts / captured / exhausted totals, in LDS: they
+    // change once per ray, and in registers they would cost the 8 VGPRs that
+    // keep the kernel at 4 wavefronts per SIMD
+    __shared__ unsigned long long cnt[256][4];
+    unsigned long long* my_cnt = cnt[threadIdx.x];
+    my_cnt[0] = my_cnt[1] = my_cnt[2] = my_cnt[3] = 0ull;
     int since_refill = A.steps_per_chunk;
 
     while (true) {
@@ -330,7 +352,7 @@ k_batch_points(BatchPointsArgs A)
                     if (my < A.n_rays) {
                         batch_init_ray<MODE>(A.cams, A.n_cams, pix0, L, A.ray0 + my);
                         w.begin(my);
-                        const bool cap0 = (MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop);
+                        const bool cap0 = batch_inside_stop<MODE>(A.cams[L.cam], L.y.r);
                         if (cap0) L.captured = 1;
                         else L.npts = 1;
                     }
@@ -393,7 +415,7 @@ k_batch_points(BatchPointsArgs A)
                 if (L.step >= C.max_steps) {
                     L.exhausted = 1;
                     finished = true;
-                } else if ((MODE == HADROS_MODE_CFG) ? (L.y.r < L.r_stop) : (L.y.r <= L.r_stop)) {
+                } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
                     L.captured = 1;
                     finished = true;
                 } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
@@ -402,7 +424,7 @@ k_batch_points(BatchPointsArgs A)
                     // ---- path point L.step (src/kerr_camera.cpp:236-273)
                     ++L.npts;
                     w.put(A.cp, L.y.r, L.y.th, zamo_dl(a, L.y, L.yp),
-                          zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) / L.e_obs_c);
+                          zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs);
                 }
             }
         } else if (L.ray >= 0) {
@@ -410,21 +432,21 @@ k_batch_points(BatchPointsArgs A)
         }
 
         if (finished) {
-            const long g = L.out;
+            const long g = batch_out_index(A.cams, pix0, L.cam, L.ray);
             w.finish(A.cp, L.npts - 1);   // every point after the observer's was put()
             A.n_st[L.ray - A.ray0] = w.stored;
             const TraceOut& O = A.out;
             if (O.captured) O.captured[g] = L.captured;
             if (O.npoints) O.npoints[g] = L.npts;
             if (O.flags) O.flags[g] = L.exhausted ? HADROS_FLAG_EXHAUSTED : 0;
-            c_pts += (unsigned long long)L.npts;
-            c_att += (unsigned long long)L.attempts;
-            c_cap += (unsigned long long)L.captured;
-            c_ex += (unsigned long long)L.exhausted;
+            my_cnt[0] += (unsigned long long)L.npts;
+            my_cnt[1] += (unsigned long long)L.attempts;
+            my_cnt[2] += (unsigned long long)L.captured;
+            my_cnt[3] += (unsigned long long)L.exhausted;
             L.ray = -1;
         }
     }
-    wave_add4(A.out.counters, c_pts, c_att, c_cap, c_ex);
+    wave_add4(A.out.counters, my_cnt[0], my_cnt[1], my_cnt[2], my_cnt[3]);
 }
 
 static size_t batch_points_lds_bytes(int n_cams)

@@ -66,4 +66,32 @@ HD_FM_QUAL double exp_tab(double x, const double* T)
     return x < EXP_ZERO_BELOW ? 0.0 : v;
 }
 
+#if defined(__HIP__)
+// 1/x for finite normal x whose reciprocal is normal (no denormal or
+// overflow scaling, unlike the IEEE division sequence): the hardware estimate
+// refined by two Newton-Raphson steps r <- r + r (1 - x r), each of which
+// doubles the correct bits; within an ulp of the correctly rounded value.
+// x = 0 gives +-inf, as the division does.
+__device__ __forceinline__ double fast_rcp(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// 1/sqrt(x) for finite normal x > 0: the hardware estimate refined by two
+// Newton-Raphson steps y <- y + y (1 - x y^2) / 2.  x < 0 gives NaN.
+__device__ __forceinline__ double fast_rsqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double e = fma(-x * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+    }
+    return y;
+}
+#endif
+
 }  // namespace hd

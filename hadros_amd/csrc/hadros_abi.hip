@@ -1412,7 +1412,12 @@ constexpr long BATCH_SUB_MAX = 1L << 30;
 // the split form only on request: with the queue refilling idle lanes the
 // fused kernel keeps its lanes as busy as the replay does, and on the
 // 256-camera workload it is the faster of the two (508 vs 516 ms)
-bool batch_want_split(const hadros_batch* b) { return b->path == HADROS_PATH_SPLIT; }
+// HADROS_PATH_AUTO: split in CFG mode (the sub-batches bound the pool), fused
+// for REF-mode rays, whose ~1e4 points each would leave few rays per sub-batch
+bool batch_want_split(const hadros_batch* b)
+{
+    return b->path == HADROS_PATH_SPLIT || (b->path == HADROS_PATH_AUTO && b->mode == HADROS_MODE_CFG);
+}
 
 double batch_pool_bound(const hadros_batch* b) { return b->pool_max > 0.0 ? b->pool_max : BATCH_POOL_MAX_BYTES; }
 
@@ -1602,37 +1607,58 @@ int hadros_batch_last_ms(hadros_batch* b, double* ms)
 }
 
 // Times every (block, steps_per_chunk) pair on the first eighth of the rays
-// (at least 2^18) and keeps the fastest.
+// (at least 2^18), after one untimed run, then times the three fastest twice
+// more and keeps the pair with the least total: a single timing per pair
+// picked a pair 13 % slower than the best now and then.
 int hadros_batch_autotune(hadros_batch* b, int* block_out, int* spc_out)
 {
     if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
     const long limit = std::min(b->n_rays, std::max(b->n_rays / 8, 1L << 18));
     const int blocks[] = {64, 128, 256};
     const int spcs[] = {1, 4, 16, 64};
-    double best = 1e300;
-    int bb = b->block, bs = b->spc;
+    struct Cand {
+        int blk, spc;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    auto time_pair = [&](int blk, int spc, int runs, bool warm, double* total) {
+        b->block = blk;
+        b->args.steps_per_chunk = spc;
+        double ms = 0.0;
+        int rc = 0;
+        if (warm) {
+            rc = batch_launch(b, nullptr, limit);
+            if (!rc) rc = hadros_batch_last_ms(b, &ms);
+        }
+        *total = 0.0;
+        for (int k = 0; k < runs && !rc; ++k) {
+            rc = batch_launch(b, nullptr, limit);
+            if (!rc) rc = hadros_batch_last_ms(b, &ms);
+            *total += ms;
+        }
+        return rc;
+    };
     for (int blk : blocks)
         for (int spc : spcs) {
-            b->block = blk;
-            b->args.steps_per_chunk = spc;
             double ms = 0.0;
-            int rc = batch_launch(b, nullptr, limit);
-            if (!rc) rc = hadros_batch_last_ms(b, &ms);
-            if (!rc) rc = batch_launch(b, nullptr, limit);
-            if (!rc) rc = hadros_batch_last_ms(b, &ms);
-            if (rc) return rc;
-            if (ms < best) {
-                best = ms;
-                bb = blk;
-                bs = spc;
-            }
+            if (const int rc = time_pair(blk, spc, 1, true, &ms)) return rc;
+            cands.push_back({blk, spc, ms});
         }
-    b->block = bb;
-    b->spc = bs;
-    b->args.steps_per_chunk = bs;
+    std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+    cands.resize(3);
+    for (Cand& c : cands) {
+        double ms = 0.0;
+        if (const int rc = time_pair(c.blk, c.spc, 2, false, &ms)) return rc;
+        c.ms += ms;
+    }
+    const Cand best = *std::min_element(cands.begin(), cands.end(),
+                                        [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+    b->block = best.blk;
+    b->spc = best.spc;
+    b->args.steps_per_chunk = best.spc;
     b->ran = false;
-    if (block_out) *block_out = bb;
-    if (spc_out) *spc_out = bs;
+    if (block_out) *block_out = best.blk;
+    if (spc_out) *spc_out = best.spc;
     return HADROS_OK;
 }
 

@@ -18,6 +18,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fastmath.h"
+
 #include "../../include/hadros.h"
 
 namespace hd {
@@ -233,7 +235,7 @@ __device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double p
     const double W = ra * pt + a * pph;
     // one reciprocal for 1/s, 1/Sigma, 1/Delta
     const double sd = sig * del;
-    const double inv_all = 1.0 / (s * sd);
+    const double inv_all = fast_rcp(s * sd);
     const double inv_s = sd * inv_all;
     const double inv_sig = s * del * inv_all;
     const double V = pph * inv_s + a * s * pt;
@@ -268,28 +270,30 @@ __device__ __forceinline__ Y5 axpy1(const Y5& y, double a1, const Y5& k1)
 
 // Fehlberg 4(5) attempt (tableau of src/kerr_geodesic.cpp:239-308), stage
 // sums left to right as the reference evaluates them.
-// the stage sums of rkf45 below, with FMA contraction allowed (CFG mode)
+// the stage sums of rkf45 below, with FMA contraction allowed (CFG mode) and
+// the tableau's fractions as rounded constants: h times a constant instead
+// of the reference's (h x numerator) / denominator, a run-time division each
 __device__ __forceinline__ void rkf45_cfg(double a, const Y5& y, double pt, double pph, double h, Y5& y4, Y5& y5)
 {
     HD_CONTRACT
     const Y5 k1 = rhs_cfg(a, y, pt, pph);
-    const double a21 = h * 1.0 / 4.0;
+    const double a21 = h * 0.25;
     const Y5 k2 = rhs_cfg(a, axpy1(y, a21, k1), pt, pph);
-    const double a31 = h * 3.0 / 32.0, a32 = h * 9.0 / 32.0;
+    const double a31 = h * (3.0 / 32.0), a32 = h * (9.0 / 32.0);
     Y5 t;
 #define HD_ST3(F) t.F = y.F + a31 * k1.F + a32 * k2.F
     HD_ST3(r); HD_ST3(th); HD_ST3(ph); HD_ST3(pr); HD_ST3(pth);
     const Y5 k3 = rhs_cfg(a, t, pt, pph);
-    const double a41 = h * 1932.0 / 2197.0, a42 = h * -7200.0 / 2197.0, a43 = h * 7296.0 / 2197.0;
+    const double a41 = h * (1932.0 / 2197.0), a42 = h * (-7200.0 / 2197.0), a43 = h * (7296.0 / 2197.0);
 #define HD_ST4(F) t.F = y.F + a41 * k1.F + a42 * k2.F + a43 * k3.F
     HD_ST4(r); HD_ST4(th); HD_ST4(ph); HD_ST4(pr); HD_ST4(pth);
     const Y5 k4 = rhs_cfg(a, t, pt, pph);
-    const double a51 = h * 439.0 / 216.0, a52 = h * -8.0, a53 = h * 3680.0 / 513.0, a54 = h * -845.0 / 4104.0;
+    const double a51 = h * (439.0 / 216.0), a52 = h * -8.0, a53 = h * (3680.0 / 513.0), a54 = h * (-845.0 / 4104.0);
 #define HD_ST5(F) t.F = y.F + a51 * k1.F + a52 * k2.F + a53 * k3.F + a54 * k4.F
     HD_ST5(r); HD_ST5(th); HD_ST5(ph); HD_ST5(pr); HD_ST5(pth);
     const Y5 k5 = rhs_cfg(a, t, pt, pph);
-    const double a61 = h * -8.0 / 27.0, a62 = h * 2.0, a63 = h * -3544.0 / 2565.0, a64 = h * 1859.0 / 4104.0,
-                 a65 = h * -11.0 / 40.0;
+    const double a61 = h * (-8.0 / 27.0), a62 = h * 2.0, a63 = h * (-3544.0 / 2565.0), a64 = h * (1859.0 / 4104.0),
+                 a65 = h * (-11.0 / 40.0);
 #define HD_ST6(F) t.F = y.F + a61 * k1.F + a62 * k2.F + a63 * k3.F + a64 * k4.F + a65 * k5.F
     HD_ST6(r); HD_ST6(th); HD_ST6(ph); HD_ST6(pr); HD_ST6(pth);
     const Y5 k6 = rhs_cfg(a, t, pt, pph);
@@ -391,7 +395,7 @@ __device__ __forceinline__ double err_cfg(const Y5& y, const Y5& y4, const Y5& y
     }
     HD_E(r); HD_E(th); HD_E(ph); HD_E(pr); HD_E(pth);
 #undef HD_E
-    return bad ? __builtin_nan("") : nb / db;
+    return bad ? __builtin_nan("") : nb * fast_rcp(db);   // db >= tol > 0
 }
 
 constexpr double CFG_H_MIN = 1.0e-9;
@@ -414,13 +418,33 @@ __device__ __forceinline__ double inv_fifth_root(double x)
 }
 
 // ------------------------------------------------------------------ camera
-// src/kerr_camera.cpp:14-29
+// The per-point geometry below fixes its fused multiply-adds by hand (no
+// HD_CONTRACT): the compiler's choice of which products to fuse depends on the
+// surrounding code, and every kernel that traces a ray must round its points
+// alike (the batch and plain tracers are bitwise equal).
+
+// Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2 r + a^2
+__device__ __forceinline__ void sigma_delta(double a2, double r, double c, double& sig, double& del)
+{
+    sig = fma(a2 * c, c, r * r);
+    del = fma(r, r - 2.0, a2);
+}
+
+// src/kerr_camera.cpp:14-29 with the lapse and frame drag of
+// src/kerr_metric.cpp:118-144: E = -(p_t + omega p_phi) / alpha, alpha^2 =
+// Sigma Delta / A, omega = 2 a r / A, by one reciprocal and one reciprocal
+// square root
 __device__ __forceinline__ double zamo_energy(double a, double r, double th, double pt, double pph)
 {
-    HD_CONTRACT
-    double alpha, omega;
-    lapse_omega(a, r, th, alpha, omega);
-    return -(pt + omega * pph) / alpha;
+    double s, c;
+    sincos_m(th, &s, &c);
+    const double a2 = a * a;
+    double sig, del;
+    sigma_delta(a2, r, c, sig, del);
+    const double ra = fma(r, r, a2);
+    const double inv_A = fast_rcp(fma(ra, ra, -(a2 * del) * (s * s)));
+    const double omega = 2.0 * a * r * inv_A;
+    return -fma(omega, pph, pt) * fast_rsqrt(sig * del * inv_A);
 }
 
 // src/kerr_camera.cpp:31-44 (the reference's while loops, bounded)
@@ -432,15 +456,25 @@ __device__ __forceinline__ double wrapped_dphi(double pn, double po)
     return d;
 }
 
-// src/kerr_camera.cpp:46-76
+// src/kerr_camera.cpp:46-76 with the spatial metric of src/kerr_metric.cpp:
+// 34-75 at the midpoint: g_rr = Sigma / Delta, g_thth = Sigma, g_phph =
+// (r^2 + a^2 + 2 a^2 r s^2 / Sigma) s^2, by one reciprocal of Sigma Delta
 __device__ __forceinline__ double zamo_dl(double a, const Y5& cur, const Y5& prev)
 {
-    HD_CONTRACT
-    const Metric5 g = metric(a, 0.5 * (cur.r + prev.r), 0.5 * (cur.th + prev.th));
+    const double r = 0.5 * (cur.r + prev.r);
+    double s, c;
+    sincos_m(0.5 * (cur.th + prev.th), &s, &c);
+    const double a2 = a * a;
+    double sig, del;
+    sigma_delta(a2, r, c, sig, del);
+    const double s2 = s * s;
+    const double inv_sd = fast_rcp(sig * del);
+    const double g_rr = sig * sig * inv_sd;
+    const double g_pp = fma(2.0 * a2 * r * s2, del * inv_sd, fma(r, r, a2)) * s2;
     const double dr = cur.r - prev.r;
     const double dth = cur.th - prev.th;
     const double dph = wrapped_dphi(cur.ph, prev.ph);
-    const double dl2 = g.rr * dr * dr + g.thth * dth * dth + g.phph * dph * dph;
+    const double dl2 = fma(g_rr * dr, dr, fma(sig * dth, dth, g_pp * dph * dph));
     return sqrt(dmax(dl2, 0.0));
 }
 

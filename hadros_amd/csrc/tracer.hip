@@ -60,7 +60,7 @@ k_trace_rt(TraceArgs A)
     const double pph = C.g_tph * pt_c + C.g_phph * pph_c;
     Y5 y{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
     const double e_obs = -(pt + C.omega * pph) / C.alpha;
-    const double e_obs_c = dmax(e_obs, 1.0e-300);
+    const double inv_e_obs = 1.0 / dmax(e_obs, 1.0e-300);   // redshift factor of every point
 
     const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
 
@@ -150,7 +150,7 @@ k_trace_rt(TraceArgs A)
         // ---- emit path point `step` (src/kerr_camera.cpp:236-273) and integrate it
         ++npts;
         const double dl = zamo_dl(a, y, yp);
-        const double z = zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c;
+        const double z = zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs;
         // The scene block is re-read through scalar loads at every point: an
         // empty asm on the pointer keeps the compiler from hoisting ~100 uniform
         // doubles out of the loop into vector registers.
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
     const double pt = C.g_tt * pt_c + C.g_tph * pph_c;
     const double pph = C.g_tph * pt_c + C.g_phph * pph_c;
     Y5 y{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
-    const double e_obs_c = dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);
+    const double inv_e_obs = 1.0 / dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);   // redshift factor of every point
     const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
     double* out = nullptr;
     long cap = 0;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
         captured = 1;
         done = true;
     } else {
-        emit(npts++, y, 0.0, zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+        emit(npts++, y, 0.0, zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs);
     }
     Y5 yp = y;
     double h = C.h;
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
         if ((MODE == HADROS_MODE_CFG) ? (y.r > C.r_max) : (y.r >= C.r_max)) {
             if (step > 10) break;
         }
-        emit(npts++, y, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+        emit(npts++, y, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs);
     }
     if constexpr (!WRITE) {
         A.counts[kk] = npts;
@@ -354,13 +354,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 // ---- split form, geodesic half: the same trace as k_trace_rt, with every
 // emitted point's (r, theta, dl, redshift) stored in the chunk pool for
 // k_replay instead of being integrated here.  Without the per-point transfer
-// in the loop the kernel needs 131 VGPRs instead of k_trace_rt's 256 (with
-// spills), so 3 wavefronts per SIMD stay resident instead of 2.  Pixel k's first chunk is chunk k;
+// in the loop the kernel fits 128 VGPRs instead of k_trace_rt's 256 (with
+// spills), so 4 wavefronts per SIMD stay resident instead of 2.  Pixel k's first chunk is chunk k;
 // further chunks come from a global counter (beyond nx * ny) and are linked
 // through next[].  A ray whose chunk lies beyond the pool stops storing
 // (the host sees the counter, grows the pool and runs again).
 #ifndef HD_POINTS_WAVES_PER_EU
-#define HD_POINTS_WAVES_PER_EU 3
+#define HD_POINTS_WAVES_PER_EU 4
 #endif
 
 template <int MODE>
@@ -387,7 +387,7 @@ k_trace_points(PointsArgs A)
     const double pt = C.g_tt * pt_c + C.g_tph * pph_c;
     const double pph = C.g_tph * pt_c + C.g_phph * pph_c;
     Y5 y{C.r_obs, C.th_obs, 0.0, C.g_rr * pr_c, C.g_thth * pth_c};
-    const double e_obs_c = dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);
+    const double inv_e_obs = 1.0 / dmax(-(pt + C.omega * pph) / C.alpha, 1.0e-300);   // redshift factor of every point
     const double r_stop = (MODE == HADROS_MODE_CFG) ? 1.01 * C.r_h : C.r_h + 1.0e-3;
     const long k = long(i) * C.ny + j;
     // the camera-plane coordinates are stored now, so they hold no registers through the loop
@@ -469,7 +469,7 @@ k_trace_points(PointsArgs A)
         }
         // ---- emit path point `step` (src/kerr_camera.cpp:236-273)
         ++npts;
-        w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) / e_obs_c);
+        w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs);
     }
 
     const unsigned long long c_pts = valid ? (unsigned long long)npts : 0ull;

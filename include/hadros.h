@@ -436,13 +436,15 @@ int hadros_batch_fetch_all_device(hadros_batch* b, const hadros_image_out* out, 
 /* workgroup size (64..256, multiple of 64) and RKF45 attempts between refills */
 int hadros_batch_set_tuning(hadros_batch* b, int block, int steps_per_chunk);
 /* times every (block in {64,128,256}) x (steps_per_chunk in {1,4,16,64}) on
-   the first eighth of the rays and keeps the fastest pair */
+   the first eighth of the rays, re-times the three fastest twice more and
+   keeps the pair with the least total time */
 int hadros_batch_autotune(hadros_batch* b, int* block, int* steps_per_chunk);
 /* How the batch runs (no reference counterpart): HADROS_PATH_FUSED, the
    persistent kernel integrating the transfer at every point; HADROS_PATH_SPLIT,
    sub-batches of rays whose persistent geodesic kernel stores the path points
    in a device chunk pool, each followed by the cache-replay kernel over those
-   rays; HADROS_PATH_AUTO (default): fused.  The split form makes
+   rays; HADROS_PATH_AUTO (default): split in CFG mode, fused in REF mode.
+   The split form makes
    hadros_batch_run's results final only once a fetch, stats or last_ms call
    has checked the pool (it grows the pool and runs again when rays ran out of
    chunks). */
