@@ -489,16 +489,33 @@ __host__ __device__ __forceinline__ double cos_theta(double th)
     return c;
 }
 
+// The exponential of the density and source functions: the math library's
+// (the default), or exp_tab over the 2^(j/64) table a kernel staged in LDS
+// (the transfer kernels, where these exponentials run once per path point).
+// Their arguments are at most moderate positive values, inside exp_tab's domain.
+struct ExpLib {
+    __host__ __device__ double operator()(double x) const { return exp(x); }
+};
+struct ExpTab {
+    const double* T;
+    __device__ double operator()(double x) const { return exp_tab(x, T); }
+};
+
 // x^y for x > 0 as exp(y ln x): relative error about |y ln x| ulp, 1e-15 for
 // the density radii (r / r0 < 1e3, |y| < 10), and a fraction of pow's cost
-__host__ __device__ __forceinline__ double pow_pos(double x, double y) { return exp(y * log(x)); }
+template <class Exp = ExpLib>
+__host__ __device__ __forceinline__ double pow_pos(double x, double y, Exp ex = Exp{})
+{
+    return ex(y * log(x));
+}
 
 // TorusProfile::rho (src/torus_profile.cpp:150-272); pow(x, 2.0) is x*x.
 // Products of exponentials are one exponential of the summed exponent, the
 // power laws share ln(r / r0), and divisions by profile parameters are
 // products with host-side reciprocals (relative differences from the
 // reference's form of order |exponent| x 1e-16).
-__host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, double th)
+template <class Exp = ExpLib>
+__host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, double th, Exp ex = Exp{})
 {
     HD_CONTRACT
     double shape;
@@ -508,73 +525,75 @@ __host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, dou
             shape = 0.0;
         } else {
             const double mu = cos_theta(th);
-            const double vertical = exp(-sq(mu / dmax(m.H, 1.0e-6)));
-            const double inner_taper = 1.0 / (1.0 + exp(-(r - m.rmin) / dmax(0.35 * m.sr, 1.0e-6)));
-            const double outer_taper = 1.0 / (1.0 + exp((r - m.rmax) / dmax(0.60 * m.sr, 1.0e-6)));
-            const double inner_core = 0.45 * exp(-sq((r - 0.75 * m.r0) / dmax(0.75 * m.sr, 1.0e-6)));
-            const double radial = pow_pos(dmax(r / m.r0, 1.0e-300), -dmax(m.p, 0.0));
-            const double floor_tail = 0.08 * exp(-sq((r - m.r0) / dmax(1.8 * m.sr, 1.0e-6)));
+            const double vertical = ex(-sq(mu / dmax(m.H, 1.0e-6)));
+            const double inner_taper = 1.0 / (1.0 + ex(-(r - m.rmin) / dmax(0.35 * m.sr, 1.0e-6)));
+            const double outer_taper = 1.0 / (1.0 + ex((r - m.rmax) / dmax(0.60 * m.sr, 1.0e-6)));
+            const double inner_core = 0.45 * ex(-sq((r - 0.75 * m.r0) / dmax(0.75 * m.sr, 1.0e-6)));
+            const double radial = pow_pos(dmax(r / m.r0, 1.0e-300), -dmax(m.p, 0.0), ex);
+            const double floor_tail = 0.08 * ex(-sq((r - m.r0) / dmax(1.8 * m.sr, 1.0e-6)));
             shape = (radial + inner_core + floor_tail) * vertical * inner_taper * outer_taper;
         }
     } else if (m.powerlaw_disk) {
         if (!(r >= m.rmin && r <= m.rmax) || r <= 0.0 || m.r0 <= 0.0) {
             shape = 0.0;
         } else {
-            // radial (r/r0)^-p x vertical exp(-(cos th / H)^2) x outer exp(-(r/rmax)^4)
+            // radial (r/r0)^-p x vertical ex(-(cos th / H)^2) x outer ex(-(r/rmax)^4)
             const double mu = cos_theta(th) * m.inv_H;
             const double q = r * m.inv_rmax;
-            const double inner_taper = 1.0 - exp(-sq((r - m.rmin) * m.inv_sr));
-            shape = exp(-m.p * lr - mu * mu - (q * q) * (q * q)) * inner_taper;
+            const double inner_taper = 1.0 - ex(-sq((r - m.rmin) * m.inv_sr));
+            shape = ex(-m.p * lr - mu * mu - (q * q) * (q * q)) * inner_taper;
         }
     } else {
         const double delta = th - 0.5 * PI;
         if (r <= 4.0 || r >= 18.0 || fabs(delta) >= 0.45) {
             shape = 0.0;
         } else {
-            shape = exp(-sq((r - m.r0) * m.inv_sr) - sq(delta * m.inv_H));
+            shape = ex(-sq((r - m.r0) * m.inv_sr) - sq(delta * m.inv_H));
         }
     }
     double F = 1.0;
     if (m.funnel) {
-        const double north = exp(-sq(th * m.inv_ft));
-        const double south = exp(-sq((PI - th) * m.inv_ft));
+        const double north = ex(-sq(th * m.inv_ft));
+        const double south = ex(-sq((PI - th) * m.inv_ft));
         F = dclamp(1.0 - m.fd * (north + south), 0.0, 1.0);
     }
     double env = 0.0;
-    if (m.envelope && !(m.env0 <= 0.0 || r < m.r0 || r > m.rmax)) env = m.env0 * exp(-m.ealpha * lr);
+    if (m.envelope && !(m.env0 <= 0.0 || r < m.r0 || r > m.rmax)) env = m.env0 * ex(-m.ealpha * lr);
     return dmax(m.rho0 * shape * F + env, m.floor_);
 }
 
 // exp(base) x bipolar_gaussian(th, th0, w) (src/radiative_transfer.cpp:64-76)
 // for w > 0, as two exponentials of summed exponents
-__device__ __forceinline__ double bipolar_times(double th, double th0, double inv_w, double base)
+template <class Exp>
+__device__ __forceinline__ double bipolar_times(double th, double th0, double inv_w, double base, Exp ex)
 {
     HD_CONTRACT
     const double n = (th - th0) * inv_w;
     const double s = (th - (PI - th0)) * inv_w;
-    return exp(base - n * n) + exp(base - s * s);
+    return ex(base - n * n) + ex(base - s * s);
 }
 
 // norm x spatial factor of emissivity_uhe (src/radiative_transfer.cpp:209-395):
 // bin-independent, so evaluated once per path point.
 // rho_here: the density at (r, th), which the caller has already evaluated
+template <class Exp = ExpLib>
 __device__ __forceinline__ double source_factor(const DevSource& so, const DevMedium& m, double r, double th,
-                                                double rho_here)
+                                                double rho_here, Exp ex = Exp{})
 {
     HD_CONTRACT
     if (so.kind == HADROS_SOURCE_INNER_RING) {
         if (!so.ring_ok) return 0.0;
         const double dr = (r - so.rc) * so.inv_sr;
         const double dt = (th - 0.5 * PI) * so.inv_tw;
-        return so.norm * exp(-dr * dr - dt * dt);
+        return so.norm * ex(-dr * dr - dt * dt);
     }
     if (r <= 0.0 || so.sr <= 0.0 || so.tw <= 0.0) return 0.0;
     double W = 0.0;
     if (so.kind == HADROS_SOURCE_FUNNEL_WALL) {
         const double dr = (r - so.rc) * so.inv_sr;
-        W = bipolar_times(th, so.ft, so.inv_tw, -dr * dr);
+        W = bipolar_times(th, so.ft, so.inv_tw, -dr * dr, ex);
     } else if (so.kind == HADROS_SOURCE_JET_BASE) {
-        W = bipolar_times(th, 0.0, so.inv_tw, -sq(r * so.inv_rc));
+        W = bipolar_times(th, 0.0, so.inv_tw, -sq(r * so.inv_rc), ex);
     } else if (so.kind == HADROS_SOURCE_DENSITY_WEIGHTED) {
         const double rh = rho_here;
         const double rho_norm = dmax(so.rho_ref > 0.0 ? so.rho_ref : so.rho_norm_c, 1.0e-300);
@@ -591,7 +610,7 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
         for (int k = 0; k < 4; ++k) {
             const double rr = (k == 0) ? r + dr : (k == 1) ? dmax(r - dr, 1.0e-6) : r;
             const double tt = (k == 2) ? dclamp(th + dth, 0.0, PI) : (k == 3) ? dclamp(th - dth, 0.0, PI) : th;
-            const double v = rho(m, rr, tt);
+            const double v = rho(m, rr, tt, ex);
             if (k == 0) sr = v;
             else if (k == 1) sr = sr - v;
             else if (k == 2) st = v;
@@ -601,7 +620,7 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
         const double d_r = sr / (2.0 * dr);
         const double d_t = st / (2.0 * dth * dmax(r, 1.0e-6));
         const double grad = sqrt(d_r * d_r + d_t * d_t) / rho_norm;
-        const double taper = exp(-sq(r / dmax(so.rc, 1.0e-300)));
+        const double taper = ex(-sq(r / dmax(so.rc, 1.0e-300)));
         W = dclamp(grad * taper, dmax(so.cmin, 0.0), dmax(so.cmax, dmax(so.cmin, 0.0)));
     }
     return so.norm * W;
@@ -875,19 +894,21 @@ struct PointQ {
     double z, lnz, g3Adl, nbar_dl, rho;
 };
 
-__device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, double dl, double z, PointQ& q)
+template <class Exp = ExpLib>
+__device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, double dl, double z, PointQ& q,
+                                        Exp ex = Exp{})
 {
     HD_CONTRACT
     if (r < P.r_h_rt + 1.0e-3) return false;
     const double zc = dmax(z, 1.0e-300);
-    const double g = 1.0 / zc;
+    const double g = fast_rcp(zc);
     const double dl_cm = dl * P.rg_cm;
-    q.rho = rho(P.med, r, th);
-    const double A = source_factor(P.src, P.med, r, th, q.rho);
+    q.rho = rho(P.med, r, th, ex);
+    const double A = source_factor(P.src, P.med, r, th, q.rho, ex);
     q.z = zc;
     q.lnz = log(zc);
     q.g3Adl = g * g * g * A * dl_cm;
-    q.nbar_dl = q.rho / M_U_G * dl_cm;
+    q.nbar_dl = q.rho * (1.0 / M_U_G) * dl_cm;
     return true;
 }
 

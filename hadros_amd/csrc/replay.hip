@@ -129,7 +129,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 pr = A.r[q];
                 pth = A.th[q];
                 pdl = A.dl[q];
-                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq);
+                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq, ExpTab{exp_t});
             }
         } else {
             if (base + lane < b1) {
@@ -138,7 +138,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 pr = rt.x;
                 pth = rt.y;
                 pdl = dz.x;
-                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq);
+                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq, ExpTab{exp_t});
             }
             if (base + 64 < b1) chunk = A.next[chunk];
         }
