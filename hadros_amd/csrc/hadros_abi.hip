@@ -1608,8 +1608,7 @@ int hadros_batch_last_ms(hadros_batch* b, double* ms)
 
 // Times every (block, steps_per_chunk) pair on the first eighth of the rays
 // (at least 2^18), after one untimed run, then times the three fastest twice
-// more and keeps the pair with the least total: a single timing per pair
-// picked a pair 13 % slower than the best now and then.
+// more and keeps the pair with the least total.
 int hadros_batch_autotune(hadros_batch* b, int* block_out, int* spc_out)
 {
     if (!b) return fail(HADROS_ERR_ARG, "NULL batch");
@@ -1623,7 +1622,8 @@ int hadros_batch_autotune(hadros_batch* b, int* block_out, int* spc_out)
     std::vector<Cand> cands;
     auto time_pair = [&](int blk, int spc, int runs, bool warm, double* total) {
         b->block = blk;
-        b->args.steps_per_chunk = spc;
+        b->spc = spc;   // the split form's points kernel
+        b->args.steps_per_chunk = spc;   // the fused kernel
         double ms = 0.0;
         int rc = 0;
         if (warm) {
