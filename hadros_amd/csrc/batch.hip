@@ -299,7 +299,7 @@ k_trace_batch(BatchArgs A)
 }
 
 #ifndef HD_BATCH_POINTS_WAVES_PER_EU
-#define HD_BATCH_POINTS_WAVES_PER_EU 4
+#define HD_BATCH_POINTS_WAVES_PER_EU 3
 #endif
 
 // Geodesic half of the split batch (the transfer runs afterwards in k_replay

@@ -44,7 +44,11 @@ __host__ __device__ __forceinline__ double dclamp(double v, double lo, double hi
 // n * P1 and n * P2 are exact), then the Taylor series of sin and cos on
 // [-pi/4, pi/4] to degree 17 / 18 (truncation < 1e-19 relative).  About 1 ulp,
 // and far fewer instructions and registers than a general double sincos with
-// a large-argument reduction path.
+// a large-argument reduction path.  FUSED (the default) evaluates the
+// reduction and the Horner steps as fused multiply-adds, 20 % fewer
+// instructions; the written multiply/add form is kept for the density's cos
+// theta, where the fused form's schedule made the transfer kernel spill.
+template <bool FUSED = true>
 __host__ __device__ __forceinline__ void sincos_m(double x, double* sp, double* cp)
 {
     constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
@@ -52,20 +56,47 @@ __host__ __device__ __forceinline__ void sincos_m(double x, double* sp, double* 
     constexpr double P2 = 0x1.0b4611a600000p-34;
     constexpr double P3 = 0x1.3198a2e037073p-69;
     const double fn = rint(x * TWO_OVER_PI);
-    const double y = ((x - fn * P1) - fn * P2) - fn * P3;
-    const double z = y * y;
-    // sin y = y + y z (-1/3! + z (1/5! - z (1/7! ...)))
-    const double ps = -1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 + z * (
-                      -1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 + z * (
-                      1.0 / 355687428096000.0)))))));
-    const double sn = y + y * z * ps;
-    // cos y = 1 - z/2 + z^2 (1/4! - z (1/6! ...))
-    const double pc = 1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 + z * (-1.0 / 3628800.0 + z * (
-                      1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 + z * (1.0 / 20922789888000.0 + z * (
-                      -1.0 / 6402373705728000.0)))))));
-    const double hz = 0.5 * z;
-    const double w = 1.0 - hz;
-    const double cs = w + (((1.0 - w) - hz) + z * z * pc);
+    double y, sn, cs;
+    if constexpr (FUSED) {
+        y = fma(-fn, P3, fma(-fn, P2, fma(-fn, P1, x)));   // n P1 and n P2 exact
+        const double z = y * y;
+        // Horner steps as fused multiply-adds (one rounding each)
+        // sin y = y + y z (-1/3! + z (1/5! - z (1/7! ...)))
+        double ps = 1.0 / 355687428096000.0;
+        ps = fma(z, ps, -1.0 / 1307674368000.0);
+        ps = fma(z, ps, 1.0 / 6227020800.0);
+        ps = fma(z, ps, -1.0 / 39916800.0);
+        ps = fma(z, ps, 1.0 / 362880.0);
+        ps = fma(z, ps, -1.0 / 5040.0);
+        ps = fma(z, ps, 1.0 / 120.0);
+        ps = fma(z, ps, -1.0 / 6.0);
+        sn = fma(y * z, ps, y);
+        // cos y = 1 - z/2 + z^2 (1/4! - z (1/6! ...))
+        double pc = -1.0 / 6402373705728000.0;
+        pc = fma(z, pc, 1.0 / 20922789888000.0);
+        pc = fma(z, pc, -1.0 / 87178291200.0);
+        pc = fma(z, pc, 1.0 / 479001600.0);
+        pc = fma(z, pc, -1.0 / 3628800.0);
+        pc = fma(z, pc, 1.0 / 40320.0);
+        pc = fma(z, pc, -1.0 / 720.0);
+        pc = fma(z, pc, 1.0 / 24.0);
+        const double hz = 0.5 * z;
+        const double w = 1.0 - hz;
+        cs = w + fma(z * z, pc, (1.0 - w) - hz);
+    } else {
+        y = ((x - fn * P1) - fn * P2) - fn * P3;
+        const double z = y * y;
+        const double ps = -1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 + z * (
+                          -1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 + z * (
+                          1.0 / 355687428096000.0)))))));
+        sn = y + y * z * ps;
+        const double pc = 1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 + z * (-1.0 / 3628800.0 + z * (
+                          1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 + z * (1.0 / 20922789888000.0 + z * (
+                          -1.0 / 6402373705728000.0)))))));
+        const double hz = 0.5 * z;
+        const double w = 1.0 - hz;
+        cs = w + (((1.0 - w) - hz) + z * z * pc);
+    }
     const int q = int(static_cast<long long>(fn) & 3);
     const double s0 = (q & 1) ? cs : sn;
     const double c0 = (q & 1) ? sn : cs;
@@ -485,7 +516,7 @@ __host__ __device__ __forceinline__ double sq(double x) { return x * x; }
 __host__ __device__ __forceinline__ double cos_theta(double th)
 {
     double s, c;
-    sincos_m(th, &s, &c);
+    sincos_m<false>(th, &s, &c);
     return c;
 }
 
