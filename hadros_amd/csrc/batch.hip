@@ -120,6 +120,7 @@ k_trace_batch(BatchArgs A)
     Kahan ktau[NBMAX];
     ScaledKahan kI[NBMAX];
     bool drained = false;   // wave-uniform: the queue is empty
+    bool direct = false;    // redo the lane's attempt in rkf45_cfg's form (rkf45_try)
     unsigned long long c_pts = 0, c_att = 0, c_cap = 0, c_ex = 0;
     int since_refill = A.steps_per_chunk;
 
@@ -166,66 +167,68 @@ k_trace_batch(BatchArgs A)
             const DevCam& C = A.cams[L.cam];
             const double a = C.a;
             Y5 y4, y5;
-            rkf45<MODE>(a, L.y, L.pt, L.pph, L.h, y4, y5);
-            ++L.attempts;
-            bool acc;
-            if constexpr (MODE == HADROS_MODE_REF) {
-                const double h_min = C.h * 1.0e-5;
-                const double err = err_ref(y4, y5);
-                acc = (err < C.tol) || (L.h <= h_min);
-                if (!acc) {
-                    const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
-                    L.h *= dclamp(factor, 0.1, 5.0);
-                    if (L.h < h_min) L.h = h_min;
-                    if (++L.att_in_step == 50) {
-                        y5 = rk4_ref(a, L.y, L.pt, L.pph, C.h);
-                        acc = true;
+            // a discarded angle-addition attempt is redone on the next pass (rkf45_try)
+            if (rkf45_try<MODE>(a, L.y, L.pt, L.pph, L.h, direct, y4, y5)) {
+                ++L.attempts;
+                bool acc;
+                if constexpr (MODE == HADROS_MODE_REF) {
+                    const double h_min = C.h * 1.0e-5;
+                    const double err = err_ref(y4, y5);
+                    acc = (err < C.tol) || (L.h <= h_min);
+                    if (!acc) {
+                        const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
+                        L.h *= dclamp(factor, 0.1, 5.0);
+                        if (L.h < h_min) L.h = h_min;
+                        if (++L.att_in_step == 50) {
+                            y5 = rk4_ref(a, L.y, L.pt, L.pph, C.h);
+                            acc = true;
+                        }
+                    }
+                    if (acc) {
+                        L.h = C.h;
+                        L.att_in_step = 0;
+                    }
+                } else {
+                    const double err = err_cfg(L.y, y4, y5, C.tol);
+                    acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
+                    if (acc) {
+                        double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
+                        fac = fmin(5.0, fmax(0.2, fac));
+                        L.h = fmax(L.h * fac, CFG_H_MIN);
+                    } else {
+                        double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
+                        fac = fmin(0.9, fmax(0.1, fac));
+                        L.h = fmax(L.h * fac, CFG_H_MIN);
+                        if (L.attempts >= 8L * C.max_steps + 64) {
+                            L.exhausted = 1;
+                            finished = true;
+                        }
                     }
                 }
                 if (acc) {
-                    L.h = C.h;
-                    L.att_in_step = 0;
-                }
-            } else {
-                const double err = err_cfg(L.y, y4, y5, C.tol);
-                acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
-                if (acc) {
-                    double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
-                    fac = fmin(5.0, fmax(0.2, fac));
-                    L.h = fmax(L.h * fac, CFG_H_MIN);
-                } else {
-                    double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
-                    fac = fmin(0.9, fmax(0.1, fac));
-                    L.h = fmax(L.h * fac, CFG_H_MIN);
-                    if (L.attempts >= 8L * C.max_steps + 64) {
+                    L.yp = L.y;
+                    L.y = y5;
+                    ++L.step;
+                    if (L.step >= C.max_steps) {
                         L.exhausted = 1;
                         finished = true;
-                    }
-                }
-            }
-            if (acc) {
-                L.yp = L.y;
-                L.y = y5;
-                ++L.step;
-                if (L.step >= C.max_steps) {
-                    L.exhausted = 1;
-                    finished = true;
-                } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
-                    L.captured = 1;
-                    finished = true;
-                } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
-                    finished = true;
-                } else {
-                    ++L.npts;
-                    const double dl = zamo_dl(a, L.y, L.yp);
-                    const double z = zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs;
-                    const DevRt& P = A.rts[C.pad_rt];
-                    PointQ q;
-                    if (point_q(P, L.y.r, L.y.th, dl, z, q)) {
-                        if constexpr (PREC == HADROS_PREC_FP64) {
-                            point_rt64<NBMAX>(P, q, lnE, lnS, slope, tau, I);
-                        } else {
-                            point_rt32<NBMAX>(P, q, lnEf, lnSf, slopef, n, ktau, kI);
+                    } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
+                        L.captured = 1;
+                        finished = true;
+                    } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
+                        finished = true;
+                    } else {
+                        ++L.npts;
+                        const double dl = zamo_dl(a, L.y, L.yp);
+                        const double z = zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs;
+                        const DevRt& P = A.rts[C.pad_rt];
+                        PointQ q;
+                        if (point_q(P, L.y.r, L.y.th, dl, z, q)) {
+                            if constexpr (PREC == HADROS_PREC_FP64) {
+                                point_rt64<NBMAX>(P, q, lnE, lnS, slope, tau, I);
+                            } else {
+                                point_rt32<NBMAX>(P, q, lnEf, lnSf, slopef, n, ktau, kI);
+                            }
                         }
                     }
                 }
@@ -326,6 +329,7 @@ k_batch_points(BatchPointsArgs A)
     ChunkWriter w;
     w.begin(-1);
     bool drained = false;   // wave-uniform: the queue is empty
+    bool direct = false;    // redo the lane's attempt in rkf45_cfg's form (rkf45_try)
     // per-thread points / attempts / captured / exhausted totals, in LDS: they
     // change once per ray, and in registers they would cost the 8 VGPRs that
     // keep the kernel at 4 wavefronts per SIMD
@@ -371,60 +375,62 @@ k_batch_points(BatchPointsArgs A)
             const DevCam& C = A.cams[L.cam];
             const double a = C.a;
             Y5 y4, y5;
-            rkf45<MODE>(a, L.y, L.pt, L.pph, L.h, y4, y5);
-            ++L.attempts;
-            bool acc;
-            if constexpr (MODE == HADROS_MODE_REF) {
-                const double h_min = C.h * 1.0e-5;
-                const double err = err_ref(y4, y5);
-                acc = (err < C.tol) || (L.h <= h_min);
-                if (!acc) {
-                    const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
-                    L.h *= dclamp(factor, 0.1, 5.0);
-                    if (L.h < h_min) L.h = h_min;
-                    if (++L.att_in_step == 50) {
-                        y5 = rk4_ref(a, L.y, L.pt, L.pph, C.h);
-                        acc = true;
+            // a discarded angle-addition attempt is redone on the next pass (rkf45_try)
+            if (rkf45_try<MODE>(a, L.y, L.pt, L.pph, L.h, direct, y4, y5)) {
+                ++L.attempts;
+                bool acc;
+                if constexpr (MODE == HADROS_MODE_REF) {
+                    const double h_min = C.h * 1.0e-5;
+                    const double err = err_ref(y4, y5);
+                    acc = (err < C.tol) || (L.h <= h_min);
+                    if (!acc) {
+                        const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
+                        L.h *= dclamp(factor, 0.1, 5.0);
+                        if (L.h < h_min) L.h = h_min;
+                        if (++L.att_in_step == 50) {
+                            y5 = rk4_ref(a, L.y, L.pt, L.pph, C.h);
+                            acc = true;
+                        }
+                    }
+                    if (acc) {
+                        L.h = C.h;
+                        L.att_in_step = 0;
+                    }
+                } else {
+                    const double err = err_cfg(L.y, y4, y5, C.tol);
+                    acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
+                    if (acc) {
+                        double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
+                        fac = fmin(5.0, fmax(0.2, fac));
+                        L.h = fmax(L.h * fac, CFG_H_MIN);
+                    } else {
+                        double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
+                        fac = fmin(0.9, fmax(0.1, fac));
+                        L.h = fmax(L.h * fac, CFG_H_MIN);
+                        if (L.attempts >= 8L * C.max_steps + 64) {
+                            L.exhausted = 1;
+                            finished = true;
+                        }
                     }
                 }
                 if (acc) {
-                    L.h = C.h;
-                    L.att_in_step = 0;
-                }
-            } else {
-                const double err = err_cfg(L.y, y4, y5, C.tol);
-                acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
-                if (acc) {
-                    double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
-                    fac = fmin(5.0, fmax(0.2, fac));
-                    L.h = fmax(L.h * fac, CFG_H_MIN);
-                } else {
-                    double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
-                    fac = fmin(0.9, fmax(0.1, fac));
-                    L.h = fmax(L.h * fac, CFG_H_MIN);
-                    if (L.attempts >= 8L * C.max_steps + 64) {
+                    L.yp = L.y;
+                    L.y = y5;
+                    ++L.step;
+                    if (L.step >= C.max_steps) {
                         L.exhausted = 1;
                         finished = true;
+                    } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
+                        L.captured = 1;
+                        finished = true;
+                    } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
+                        finished = true;
+                    } else {
+                        // ---- path point L.step (src/kerr_camera.cpp:236-273)
+                        ++L.npts;
+                        w.put(A.cp, L.y.r, L.y.th, zamo_dl(a, L.y, L.yp),
+                              zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs);
                     }
-                }
-            }
-            if (acc) {
-                L.yp = L.y;
-                L.y = y5;
-                ++L.step;
-                if (L.step >= C.max_steps) {
-                    L.exhausted = 1;
-                    finished = true;
-                } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
-                    L.captured = 1;
-                    finished = true;
-                } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
-                    finished = true;
-                } else {
-                    // ---- path point L.step (src/kerr_camera.cpp:236-273)
-                    ++L.npts;
-                    w.put(A.cp, L.y.r, L.y.th, zamo_dl(a, L.y, L.yp),
-                          zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs);
                 }
             }
         } else if (L.ray >= 0) {

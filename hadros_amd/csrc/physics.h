@@ -104,6 +104,32 @@ __host__ __device__ __forceinline__ void sincos_m(double x, double* sp, double* 
     *cp = ((q + 1) & 2) ? -c0 : c0;
 }
 
+// sin and cos of x + d from s0 = sin x, c0 = cos x, for |d| <= 1/4: the
+// Taylor series of sin d to degree 11 and of cos d to degree 12 (truncation
+// below 1e-17 relative), then the angle-addition formulas; about 20
+// instructions against sincos_m's 35.  Returns false for |d| > 1/4 or NaN,
+// whose results are to be discarded.
+__device__ __forceinline__ bool sincos_shift(double s0, double c0, double d, double* sp, double* cp)
+{
+    const double z = d * d;
+    double ps = -1.0 / 39916800.0;
+    ps = fma(z, ps, 1.0 / 362880.0);
+    ps = fma(z, ps, -1.0 / 5040.0);
+    ps = fma(z, ps, 1.0 / 120.0);
+    ps = fma(z, ps, -1.0 / 6.0);
+    const double sd = fma(d * z, ps, d);
+    double pc = 1.0 / 479001600.0;
+    pc = fma(z, pc, -1.0 / 3628800.0);
+    pc = fma(z, pc, 1.0 / 40320.0);
+    pc = fma(z, pc, -1.0 / 720.0);
+    pc = fma(z, pc, 1.0 / 24.0);
+    pc = fma(z, pc, -0.5);
+    const double cdm1 = z * pc;   // cos d - 1
+    *sp = fma(s0, cdm1, fma(c0, sd, s0));
+    *cp = fma(c0, cdm1, fma(-s0, sd, c0));
+    return fabs(d) <= 0.25;
+}
+
 // ------------------------------------------------------------------ params
 // Normalised TorusProfile (constructor clamps of src/torus_profile.cpp:12-42
 // applied on the host).
@@ -253,12 +279,11 @@ __device__ __forceinline__ Y5 rhs_ref(double a, const Y5& y, double pt, double p
 // Analytic Hamiltonian derivative (CFG mode).  With W = (r^2+a^2) p_t + a p_phi
 // and V = p_phi/sin + a sin p_t, 2 Sigma H = N = Delta p_r^2 + p_th^2 - W^2/Delta + V^2
 // (the contraction g^{mu nu} p_mu p_nu of src/kerr_metric.cpp:77-116 times Sigma).
-__device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double pph)
+// Takes sin and cos of y.th (y.th itself is not read); rhs_cfg computes them.
+__device__ __forceinline__ Y5 rhs_cfg_sc(double a, const Y5& y, double s, double c, double pt, double pph)
 {
     HD_CONTRACT
     const double r = y.r;
-    double s, c;
-    sincos_m(y.th, &s, &c);
     const double r2 = r * r, a2 = a * a;
     const double sig = r2 + a2 * c * c;
     const double del = r2 - 2.0 * r + a2;
@@ -285,6 +310,13 @@ __device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double p
     d.pr = -(dNr * h_is - n_over * (2.0 * r));
     d.pth = -(dNth * h_is - n_over * (-2.0 * a2 * c * s));
     return d;
+}
+
+__device__ __forceinline__ Y5 rhs_cfg(double a, const Y5& y, double pt, double pph)
+{
+    double s, c;
+    sincos_m(y.th, &s, &c);
+    return rhs_cfg_sc(a, y, s, c, pt, pph);
 }
 
 template <int MODE>
@@ -340,6 +372,78 @@ __device__ __forceinline__ void rkf45_cfg(double a, const Y5& y, double pt, doub
 #undef HD_Y5
 }
 
+// The same attempt with the five later stages' sin/cos by angle addition
+// from the attempt's (sincos_shift, theta_j = y.th + d_j with d_j the stage's
+// theta offset) and phi's 4th- and 5th-order sums accumulated as the stages
+// come (phi feeds no stage).  Returns false when a stage offset leaves
+// sincos_shift's range: the attempt is then discarded and redone by
+// rkf45_cfg (rkf45_try).
+__device__ __forceinline__ bool rkf45_cfg_shift(double a, const Y5& y, double pt, double pph, double h, Y5& y4,
+                                                Y5& y5)
+{
+    HD_CONTRACT
+    double s0, c0, sj, cj;
+    bool ok = true;
+    sincos_m(y.th, &s0, &c0);
+    const Y5 k1 = rhs_cfg_sc(a, y, s0, c0, pt, pph);
+    double ph4 = (25.0 / 216.0) * k1.ph, ph5 = (16.0 / 135.0) * k1.ph;
+    const double a21 = h * 0.25;
+    ok &= sincos_shift(s0, c0, a21 * k1.th, &sj, &cj);
+    const Y5 k2 = rhs_cfg_sc(a, axpy1(y, a21, k1), sj, cj, pt, pph);
+    const double a31 = h * (3.0 / 32.0), a32 = h * (9.0 / 32.0);
+    Y5 t;
+#define HD_ST3(F) t.F = y.F + a31 * k1.F + a32 * k2.F
+    HD_ST3(r); HD_ST3(pr); HD_ST3(pth);
+    ok &= sincos_shift(s0, c0, fma(a32, k2.th, a31 * k1.th), &sj, &cj);
+    const Y5 k3 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
+    ph4 = fma(1408.0 / 2565.0, k3.ph, ph4);
+    ph5 = fma(6656.0 / 12825.0, k3.ph, ph5);
+    const double a41 = h * (1932.0 / 2197.0), a42 = h * (-7200.0 / 2197.0), a43 = h * (7296.0 / 2197.0);
+#define HD_ST4(F) t.F = y.F + a41 * k1.F + a42 * k2.F + a43 * k3.F
+    HD_ST4(r); HD_ST4(pr); HD_ST4(pth);
+    ok &= sincos_shift(s0, c0, fma(a43, k3.th, fma(a42, k2.th, a41 * k1.th)), &sj, &cj);
+    const Y5 k4 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
+    ph4 = fma(2197.0 / 4104.0, k4.ph, ph4);
+    ph5 = fma(28561.0 / 56430.0, k4.ph, ph5);
+    const double a51 = h * (439.0 / 216.0), a52 = h * -8.0, a53 = h * (3680.0 / 513.0), a54 = h * (-845.0 / 4104.0);
+#define HD_ST5(F) t.F = y.F + a51 * k1.F + a52 * k2.F + a53 * k3.F + a54 * k4.F
+    HD_ST5(r); HD_ST5(pr); HD_ST5(pth);
+    ok &= sincos_shift(s0, c0, fma(a54, k4.th, fma(a53, k3.th, fma(a52, k2.th, a51 * k1.th))), &sj, &cj);
+    const Y5 k5 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
+    ph4 = fma(-1.0 / 5.0, k5.ph, ph4);
+    ph5 = fma(-9.0 / 50.0, k5.ph, ph5);
+    const double a61 = h * (-8.0 / 27.0), a62 = h * 2.0, a63 = h * (-3544.0 / 2565.0), a64 = h * (1859.0 / 4104.0),
+                 a65 = h * (-11.0 / 40.0);
+#define HD_ST6(F) t.F = y.F + a61 * k1.F + a62 * k2.F + a63 * k3.F + a64 * k4.F + a65 * k5.F
+    HD_ST6(r); HD_ST6(pr); HD_ST6(pth);
+    ok &= sincos_shift(s0, c0, fma(a65, k5.th, fma(a64, k4.th, fma(a63, k3.th, fma(a62, k2.th, a61 * k1.th)))),
+                       &sj, &cj);
+    const Y5 k6 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
+    ph5 = fma(2.0 / 55.0, k6.ph, ph5);
+#define HD_Y4(F) y4.F = y.F + h * (25.0 / 216.0 * k1.F + 1408.0 / 2565.0 * k3.F + 2197.0 / 4104.0 * k4.F - 1.0 / 5.0 * k5.F)
+#define HD_Y5(F) y5.F = y.F + h * (16.0 / 135.0 * k1.F + 6656.0 / 12825.0 * k3.F + 28561.0 / 56430.0 * k4.F - 9.0 / 50.0 * k5.F + 2.0 / 55.0 * k6.F)
+    HD_Y4(r); HD_Y4(th); HD_Y4(pr); HD_Y4(pth);
+    HD_Y5(r); HD_Y5(th); HD_Y5(pr); HD_Y5(pth);
+    y4.ph = fma(h, ph4, y.ph);
+    y5.ph = fma(h, ph5, y.ph);
+#undef HD_ST3
+#undef HD_ST4
+#undef HD_ST5
+#undef HD_ST6
+#undef HD_Y4
+#undef HD_Y5
+    return ok;
+}
+
+// One RKF45 attempt of a lane.  CFG mode takes the angle-addition form unless
+// `direct` is set; when a stage offset leaves its range it sets `direct` and
+// returns false (no attempt made): the caller's next pass redoes the attempt
+// with rkf45_cfg, which clears `direct`.  Which form a ray's attempt uses thus
+// depends on that ray alone, so every kernel traces a ray to the same bits.
+template <int MODE>
+__device__ __forceinline__ bool rkf45_try(double a, const Y5& y, double pt, double pph, double h, bool& direct,
+                                          Y5& y4, Y5& y5);
+
 template <int MODE>
 __device__ __forceinline__ void rkf45(double a, const Y5& y, double pt, double pph, double h, Y5& y4, Y5& y5)
 {
@@ -378,6 +482,25 @@ __device__ __forceinline__ void rkf45(double a, const Y5& y, double pt, double p
 #undef HD_ST6
 #undef HD_Y4
 #undef HD_Y5
+}
+
+template <int MODE>
+__device__ __forceinline__ bool rkf45_try(double a, const Y5& y, double pt, double pph, double h, bool& direct,
+                                          Y5& y4, Y5& y5)
+{
+    if constexpr (MODE == HADROS_MODE_CFG) {
+        if (direct) {
+            rkf45_cfg(a, y, pt, pph, h, y4, y5);
+            direct = false;
+            return true;
+        }
+        if (rkf45_cfg_shift(a, y, pt, pph, h, y4, y5)) return true;
+        direct = true;
+        return false;
+    } else {
+        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        return true;
+    }
 }
 
 // KerrGeodesic::step_rk4 at the base step (src/kerr_geodesic.cpp:137-192).

@@ -309,6 +309,8 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             if (A.p_m) A.p_m[g] = exp(-m_tau);
             if (A.I_m) A.I_m[g] = m_I;
             if (A.rns_m) A.rns_m[g] = m_rns;
+            // the reference's leakage factor divides the summed j dl by the same sum
+            // (src/radiative_transfer.cpp:630-631, 681-682): 1 wherever it is positive
             if (A.leak_m) A.leak_m[g] = m_leak > 0.0 ? m_leak / m_leak : 1.0;
         }
     }

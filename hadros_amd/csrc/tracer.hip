@@ -93,9 +93,10 @@ k_trace_rt(TraceArgs A)
     int att_in_step = 0;
     const long max_attempts = 8L * C.max_steps + 64;
 
+    bool direct = false;   // redo the attempt in rkf45_cfg's form (rkf45_try)
     while (!done) {
         Y5 y4, y5;
-        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        if (!rkf45_try<MODE>(a, y, pt, pph, h, direct, y4, y5)) continue;
         ++attempts;
         bool acc;
         if constexpr (MODE == HADROS_MODE_REF) {
@@ -291,9 +292,10 @@ __global__ void __launch_bounds__(256) k_trace_cache(CacheArgs A)
     int att_in_step = 0;
     long attempts = 0;
     const long max_attempts = 8L * C.max_steps + 64;
+    bool direct = false;   // redo the attempt in rkf45_cfg's form (rkf45_try)
     while (!done) {
         Y5 y4, y5;
-        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        if (!rkf45_try<MODE>(a, y, pt, pph, h, direct, y4, y5)) continue;
         ++attempts;
         bool acc;
         if constexpr (MODE == HADROS_MODE_REF) {
@@ -354,13 +356,15 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 // ---- split form, geodesic half: the same trace as k_trace_rt, with every
 // emitted point's (r, theta, dl, redshift) stored in the chunk pool for
 // k_replay instead of being integrated here.  Without the per-point transfer
-// in the loop the kernel fits 128 VGPRs instead of k_trace_rt's 256 (with
-// spills), so 4 wavefronts per SIMD stay resident instead of 2.  Pixel k's first chunk is chunk k;
+// in the loop the kernel fits 163 VGPRs instead of k_trace_rt's 256 (with
+// spills), so 3 wavefronts per SIMD stay resident instead of 2 (the
+// angle-addition attempt of rkf45_try costs the registers of a fourth, and
+// measured 5 % faster).  Pixel k's first chunk is chunk k;
 // further chunks come from a global counter (beyond nx * ny) and are linked
 // through next[].  A ray whose chunk lies beyond the pool stops storing
 // (the host sees the counter, grows the pool and runs again).
 #ifndef HD_POINTS_WAVES_PER_EU
-#define HD_POINTS_WAVES_PER_EU 4
+#define HD_POINTS_WAVES_PER_EU 3
 #endif
 
 template <int MODE>
@@ -413,10 +417,11 @@ k_trace_points(PointsArgs A)
     const double h_min = C.h * 1.0e-5;
     int att_in_step = 0;
     const long max_attempts = 8L * C.max_steps + 64;
+    bool direct = false;   // redo the attempt in rkf45_cfg's form (rkf45_try)
 
     while (!done) {
         Y5 y4, y5;
-        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        if (!rkf45_try<MODE>(a, y, pt, pph, h, direct, y4, y5)) continue;
         ++attempts;
         bool acc;
         if constexpr (MODE == HADROS_MODE_REF) {
