@@ -52,6 +52,36 @@ def transfer_flops(points, nbins):
     return points * (ALGO["F_PT_RT"] + nbins * ALGO["F_BIN"])
 
 
+# HBM bytes a transfer pass must move: 32 B per point read, and per ray its
+# point count or range (8 B) and outputs: tau, P, I and the per-bin tau, P, I
+def transfer_bytes(points, rays, nbins):
+    return 32.0 * points + rays * (8.0 + 8.0 * (3 + 3 * max(nbins, 1)))
+
+
+# geodesic kernel of the split form: the 32-byte points written, and per ray
+# its outputs (alpha, beta, captured, npoints, flags) and stored-point count
+def geodesic_bytes(points, rays):
+    return 32.0 * points + 32.0 * rays
+
+
+RIDGE_FLOP_PER_BYTE = PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
+
+
+def kernel_roofline(name, ms, flop, nbytes):
+    """The roofline of one kernel: bound by FP64 issue when its algorithmic
+    intensity (FLOP per HBM byte) is above the MI355X ridge, else by HBM."""
+    sec = ms * 1e-3
+    intensity = flop / nbytes
+    out = {"kernel": name, "ms": float(ms), "flop": flop, "algorithmic_bytes": nbytes,
+           "intensity_flop_per_byte": intensity, "tflops": flop / sec / 1e12, "gbs": nbytes / sec / 1e9}
+    if intensity >= RIDGE_FLOP_PER_BYTE:
+        out.update(bound="mfma", achieved=out["tflops"], peak=PEAK_FP64_TFLOPS, unit="TFLOP/s")
+    else:
+        out.update(bound="hbm", achieved=out["gbs"], peak=PEAK_HBM_GBS, unit="GB/s")
+    out["frac"] = out["achieved"] / out["peak"]
+    return out
+
+
 def image_errors(img, ref):
     """Per-image error of a GPU image against the FP64 oracle's: rel(tau)
     where tau_ref > 0, |dP|, rel(P) where P_ref > 1e-300, rel(sum I)."""
@@ -173,8 +203,7 @@ def replay_leg(cam, sc, e, s, steps=5):
         _, st = rp.fetch()
         rp.close()
         kms = float(np.mean(ms))
-        nb = max(st.n_bins, 1)
-        alg = 32.0 * pts + rays * (8.0 + 8.0 * (3 + 3 * nb))
+        alg = transfer_bytes(pts, rays, st.n_bins)
         out[name] = {"kernel_ms": kms, "rays_per_s": rays / (kms * 1e-3), "achieved": alg / (kms * 1e-3) / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      "bound": "hbm", "algorithmic_bytes": alg}
@@ -344,22 +373,20 @@ def main():
     # per kernel of the split form (HIP events between the two launches)
     kernels = {}
     if rt_ms > 0:
-        gf = geodesic_flops(st.total_attempts, st.total_points)
-        rf = transfer_flops(st.total_points, st.n_bins)
         kernels = {
-            "geodesic": {"kernel": "hd::k_trace_points<CFG>", "ms": float(geo_ms), "flop": gf,
-                         "achieved": gf / (geo_ms * 1e-3) / 1e12},
-            "transfer": {"kernel": "hd::k_replay<FP64, chunks>", "ms": float(rt_ms), "flop": rf,
-                         "achieved": rf / (rt_ms * 1e-3) / 1e12},
+            "geodesic": kernel_roofline("hd::k_trace_points<CFG>", geo_ms,
+                                        geodesic_flops(st.total_attempts, st.total_points),
+                                        geodesic_bytes(st.total_points, npix)),
+            "transfer": kernel_roofline("hd::k_replay<FP64, chunks>", rt_ms,
+                                        transfer_flops(st.total_points, st.n_bins),
+                                        transfer_bytes(st.total_points, npix, st.n_bins)),
         }
-        for k in kernels.values():
-            k["frac"] = k["achieved"] / PEAK_FP64_TFLOPS
         dom = max(kernels.values(), key=lambda k: k["ms"])
-        flops, dom_ms, dom_name = dom["flop"], dom["ms"], dom["kernel"]
     else:
-        flops, dom_ms, dom_name = tracer_flops(st.total_attempts, st.total_points, st.n_bins), kavg, \
-            "hd::k_trace_rt<CFG, FP64, 8>"
-    achieved = flops / (dom_ms * 1e-3) / 1e12
+        flops = tracer_flops(st.total_attempts, st.total_points, st.n_bins)
+        dom = kernel_roofline("hd::k_trace_rt<CFG, FP64, 8>", kavg, flops,
+                              geodesic_bytes(st.total_points, npix))
+    dom_name = dom["kernel"]
 
     wall32, kern32, _, img32, st32 = timed(abi.PREC_FP32K)
     # the same camera through a medium 1e-8 as dense (rho0 100 g/cm^3): the
@@ -392,11 +419,13 @@ def main():
         "config": {"workload": f"plain FP64 tracer, {cam.nx}x{cam.ny} camera, spin 0.99, incl 80 deg, "
                                "powerlaw_funnel_envelope, powerlaw_cutoff 8 bins, funnel_wall, CFG mode rtol 1e-8",
                    "rays_per_step": npix, "device": dev_name, "block": st.block_size},
-        "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix peak)", "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": None, "kernel": dom_name, "kernel_ms": dom_ms, "tracer_ms": kavg,
-                     "algorithmic_flop_per_launch": flops, "points_per_launch": st.total_points,
-                     "attempts_per_launch": st.total_attempts, "kernels": kernels},
+        "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+                     "frac": dom["frac"], "traffic": None, "kernel": dom_name, "kernel_ms": dom["ms"],
+                     "peak_kind": "fp64 (vector = matrix peak)" if dom["bound"] == "mfma" else "hbm3e",
+                     "ridge_flop_per_byte": RIDGE_FLOP_PER_BYTE, "tracer_ms": kavg,
+                     "algorithmic_flop_per_launch": dom["flop"], "algorithmic_bytes_per_launch": dom["algorithmic_bytes"],
+                     "points_per_launch": st.total_points, "attempts_per_launch": st.total_attempts,
+                     "kernels": kernels},
         "precision_form": {"value": args.steps * npix / wall32, "unit": "rays/s", "dtype": "f64 geodesic + f32 Kahan",
                            "kernel_ms": float(np.mean(kern32)), "max_rel_tau_vs_fp64": err_tau,
                            "max_abs_p_vs_fp64": err_p, "rel_sum_i_vs_fp64": err_i},
