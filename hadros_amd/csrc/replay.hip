@@ -15,6 +15,9 @@
 // the chunk pool written by the split tracer (k_trace_points): 64-point
 // chunks of 32-byte (r, theta, dl, z) records, the first at the ray's own
 // index, the rest linked through next[].
+#include <cstddef>
+#include <type_traits>
+
 #include "fastmath.h"
 #include "kernels.h"
 #include "wave.h"
@@ -375,7 +378,8 @@ k_replay(ReplayArgs A)
     ReplayAcc64& a64 = acc64[wv];
     ReplayAcc32& a32 = acc32[wv];
     // A.rt where it lies in the kernel argument segment (the kernel's only
-    // explicit argument starts at offset 0)
+    // explicit argument starts at offset 0; offsetof needs a standard layout)
+    static_assert(std::is_standard_layout<ReplayArgs>::value, "ReplayArgs is read at its kernarg offsets");
     typedef const __attribute__((address_space(4))) char* KernargC;
     const DevRtC rt_c = (DevRtC)((KernargC)__builtin_amdgcn_kernarg_segment_ptr() + offsetof(ReplayArgs, rt));
     const long nwaves = long(gridDim.x) * REPLAY_WAVES;
