@@ -145,6 +145,22 @@ __global__ void k_sum_parts(const float* p, int parts, long n, float* out)
     out[i] = s;
 }
 
+// The same for few outputs and many parts (the bias gradients' row-chunk
+// partials): one workgroup per output, each thread adding a strided share of
+// the parts, then a fixed tree over the 256 threads (deterministic).
+__global__ void __launch_bounds__(256) k_sum_parts_wide(const float* p, int parts, long n, float* out)
+{
+    __shared__ float wsum[4];
+    const long i = blockIdx.x;
+    float s = 0.0f;
+    for (int q = threadIdx.x; q < parts; q += 256) s += p[q * n + i];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[i] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+}
+
 // column sums of a bf16 [M x N] matrix in row chunks -> partials [chunks x N]
 __global__ void k_colsum_bf16(const __bf16* x, int M, int N, int rows_per_chunk, float* part)
 {
@@ -239,7 +255,13 @@ static inline int nblk(long n) { return int((n + 255) / 256); }
 
 hipError_t mlp_sum_parts(const float* p, int parts, long n, float* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_sum_parts, dim3(nblk(n)), dim3(256), 0, s, p, parts, n, out);
+    // one thread per output when there are enough outputs to fill the chip;
+    // a workgroup per output for the bias gradients (<= 256 outputs, 256 parts),
+    // where a thread per output ran a 256-long dependent chain (about 50 us)
+    if (n <= 4096 && parts >= 64)
+        hipLaunchKernelGGL(k_sum_parts_wide, dim3(unsigned(n)), dim3(256), 0, s, p, parts, n, out);
+    else
+        hipLaunchKernelGGL(k_sum_parts, dim3(nblk(n)), dim3(256), 0, s, p, parts, n, out);
     return hipGetLastError();
 }
 hipError_t mlp_colsum(const __bf16* x, int M, int N, int rows_per_chunk, float* part, hipStream_t s)
