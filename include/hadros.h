@@ -450,7 +450,7 @@ int hadros_batch_autotune(hadros_batch* b, int* block, int* steps_per_chunk);
    chunks). */
 int hadros_batch_set_path(hadros_batch* b, int path);
 /* Bound (bytes) of the split form's chunk pool; the rays per sub-batch follow
-   from it.  0 = the default, 24 GB. */
+   from it.  0 = the default, 64 GB. */
 int hadros_batch_set_pool(hadros_batch* b, double max_bytes);
 int hadros_batch_run(hadros_batch* b, void* stream);
 int hadros_batch_last_ms(hadros_batch* b, double* ms);
