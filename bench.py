@@ -310,9 +310,10 @@ def main():
     ap.add_argument("--no-replay", action="store_true", help="skip the cache-replay leg")
     ap.add_argument("--no-opacity", action="store_true", help="skip the opacity-surface leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--pmc-target", nargs="?", const="tracer", choices=["tracer", "replay"],
-                    help="for rocprofv3 --pmc passes: run the plain tracer (or the cache replay) twice, then copy "
-                         "a 1 GiB device buffer twice (calibration of FETCH_SIZE / WRITE_SIZE), print nothing else")
+    ap.add_argument("--pmc-target", nargs="?", const="tracer", choices=["tracer", "replay", "batch"],
+                    help="for rocprofv3 passes: run the plain tracer (or the cache replay, or the 256-camera batch "
+                         "at block 128 and 64 steps per refill, without the autotuner) twice, then copy a 1 GiB "
+                         "device buffer twice (calibration of FETCH_SIZE / WRITE_SIZE), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the per-ray work has no "
@@ -333,8 +334,13 @@ def main():
     if args.pmc_target:
         if args.pmc_target == "tracer":
             t = ffi.Tracer(cam, sc, e, s, abi.PREC_FP64, args.block)
-        else:
+        elif args.pmc_target == "replay":
             t = ffi.Replay(ffi.trace_cache(cam), sc, e, s, abi.PREC_FP64)
+        else:
+            man = workloads.c3_manifest()
+            t = ffi.Batch([workloads.manifest_camera(m) for m in man], [workloads.manifest_scene(m) for m in man],
+                          e, s, abi.PREC_FP64)
+            t.set_tuning(128, 64)
         for _ in range(2):
             t.run()
             t.last_ms()
