@@ -287,13 +287,14 @@ __device__ __forceinline__ void rkf45_cfg(double a, const Y5& y, double pt, doub
 // come (phi feeds no stage).  Returns false when a stage offset leaves
 // sincos_shift's range: the attempt is then discarded and redone by
 // rkf45_cfg (rkf45_try).
-__device__ __forceinline__ bool rkf45_cfg_shift(double a, const Y5& y, double pt, double pph, double h, Y5& y4,
-                                                Y5& y5)
+// s0, c0: sin and cos of y.th (sincos_m), which a caller that already has
+// them passes in.
+__device__ __forceinline__ bool rkf45_cfg_shift_sc(double a, const Y5& y, double s0, double c0, double pt, double pph,
+                                                   double h, Y5& y4, Y5& y5)
 {
     HD_CONTRACT
-    double s0, c0, sj, cj;
+    double sj, cj;
     bool ok = true;
-    sincos_m(y.th, &s0, &c0);
     const Y5 k1 = rhs_cfg_sc(a, y, s0, c0, pt, pph);
     double ph4 = (25.0 / 216.0) * k1.ph, ph5 = (16.0 / 135.0) * k1.ph;
     const double a21 = h * 0.25;
@@ -342,6 +343,14 @@ __device__ __forceinline__ bool rkf45_cfg_shift(double a, const Y5& y, double pt
 #undef HD_Y4
 #undef HD_Y5
     return ok;
+}
+
+__device__ __forceinline__ bool rkf45_cfg_shift(double a, const Y5& y, double pt, double pph, double h, Y5& y4,
+                                                Y5& y5)
+{
+    double s0, c0;
+    sincos_m(y.th, &s0, &c0);
+    return rkf45_cfg_shift_sc(a, y, s0, c0, pt, pph, h, y4, y5);
 }
 
 // One RKF45 attempt of a lane.  CFG mode takes the angle-addition form unless
@@ -404,6 +413,27 @@ __device__ __forceinline__ bool rkf45_try(double a, const Y5& y, double pt, doub
             return true;
         }
         if (rkf45_cfg_shift(a, y, pt, pph, h, y4, y5)) return true;
+        direct = true;
+        return false;
+    } else {
+        rkf45<MODE>(a, y, pt, pph, h, y4, y5);
+        return true;
+    }
+}
+
+// The same with sin and cos of y.th passed in (sincos_m's, so the results are
+// the same bits): a kernel that keeps them from the last accepted point.
+template <int MODE>
+__device__ __forceinline__ bool rkf45_try_sc(double a, const Y5& y, double s0, double c0, double pt, double pph,
+                                             double h, bool& direct, Y5& y4, Y5& y5)
+{
+    if constexpr (MODE == HADROS_MODE_CFG) {
+        if (direct) {
+            rkf45_cfg(a, y, pt, pph, h, y4, y5);
+            direct = false;
+            return true;
+        }
+        if (rkf45_cfg_shift_sc(a, y, s0, c0, pt, pph, h, y4, y5)) return true;
         direct = true;
         return false;
     } else {
@@ -497,10 +527,8 @@ __device__ __forceinline__ void sigma_delta(double a2, double r, double c, doubl
 // src/kerr_metric.cpp:118-144: E = -(p_t + omega p_phi) / alpha, alpha^2 =
 // Sigma Delta / A, omega = 2 a r / A, by one reciprocal and one reciprocal
 // square root
-__device__ __forceinline__ double zamo_energy(double a, double r, double th, double pt, double pph)
+__device__ __forceinline__ double zamo_energy_sc(double a, double r, double s, double c, double pt, double pph)
 {
-    double s, c;
-    sincos_m(th, &s, &c);
     const double a2 = a * a;
     double sig, del;
     sigma_delta(a2, r, c, sig, del);
@@ -508,6 +536,14 @@ __device__ __forceinline__ double zamo_energy(double a, double r, double th, dou
     const double inv_A = fast_rcp(fma(ra, ra, -(a2 * del) * (s * s)));
     const double omega = 2.0 * a * r * inv_A;
     return -fma(omega, pph, pt) * fast_rsqrt(sig * del * inv_A);
+}
+
+// with s, c = sincos_m(th)
+__device__ __forceinline__ double zamo_energy(double a, double r, double th, double pt, double pph)
+{
+    double s, c;
+    sincos_m(th, &s, &c);
+    return zamo_energy_sc(a, r, s, c, pt, pph);
 }
 
 // src/kerr_camera.cpp:31-44 (the reference's while loops, bounded)

@@ -418,10 +418,14 @@ k_trace_points(PointsArgs A)
     int att_in_step = 0;
     const long max_attempts = 8L * C.max_steps + 64;
     bool direct = false;   // redo the attempt in rkf45_cfg's form (rkf45_try)
+    // sin and cos of y.th, kept from the last accepted point for the next
+    // attempt's first stage (CFG mode; the same bits as computing them there)
+    double sy = 0.0, cy = 1.0;
+    if constexpr (MODE == HADROS_MODE_CFG) sincos_m(y.th, &sy, &cy);
 
     while (!done) {
         Y5 y4, y5;
-        if (!rkf45_try<MODE>(a, y, pt, pph, h, direct, y4, y5)) continue;
+        if (!rkf45_try_sc<MODE>(a, y, sy, cy, pt, pph, h, direct, y4, y5)) continue;
         ++attempts;
         bool acc;
         if constexpr (MODE == HADROS_MODE_REF) {
@@ -474,7 +478,14 @@ k_trace_points(PointsArgs A)
         }
         // ---- emit path point `step` (src/kerr_camera.cpp:236-273)
         ++npts;
-        w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs);
+        double z;
+        if constexpr (MODE == HADROS_MODE_CFG) {
+            sincos_m(y.th, &sy, &cy);
+            z = zamo_energy_sc(a, y.r, sy, cy, pt, pph) * inv_e_obs;
+        } else {
+            z = zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs;
+        }
+        w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), z);
     }
 
     const unsigned long long c_pts = valid ? (unsigned long long)npts : 0ull;
