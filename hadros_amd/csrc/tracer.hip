@@ -356,7 +356,7 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 // ---- split form, geodesic half: the same trace as k_trace_rt, with every
 // emitted point's (r, theta, dl, redshift) stored in the chunk pool for
 // k_replay instead of being integrated here.  Without the per-point transfer
-// in the loop the kernel fits 163 VGPRs instead of k_trace_rt's 256 (with
+// in the loop the kernel fits 165 VGPRs instead of k_trace_rt's 256 (with
 // spills), so 3 wavefronts per SIMD stay resident instead of 2 (the
 // angle-addition attempt of rkf45_try costs the registers of a fourth, and
 // measured 5 % faster).  Pixel k's first chunk is chunk k;
