@@ -5,9 +5,9 @@
 // else the CPU cache, :353-360), the same header checks and error exits
 // (:365-406, :672-700), the same output file name (:408-500), metadata header
 // (:512-597) and rows in cache order (:635-654), and the same stdout lines.
-// The UHE columns come from the GPU replay kernel (hadros_rt_from_cache); the
-// MeV columns are outside this path and hold the RTResult defaults
-// (tau 0, P 1, I 0, r_ns -1, leakage 1), flagged in the header.
+// The UHE columns and the five MeV columns (argv 14-15 and 45-71, the model of
+// src/radiative_transfer.cpp:513-638) both come from the GPU replay kernel
+// (hadros_replay_* with hadros_replay_set_mev).
 #include "cli_common.h"
 
 #include <chrono>

@@ -447,6 +447,8 @@ struct hadros_tracer {
     DevRt* d_rt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evm = nullptr;   // split form: between the geodesic and the transfer kernel
+    hipStream_t stream = nullptr;   // stream of the last run (a pool regrowth re-runs on it)
+    bool pool_too_big = false;  // AUTO: the pool the rays need exceeds SPLIT_POOL_MAX_BYTES, run fused
     bool ran = false;
     MevState mev;
     bool ran_mev = false;       // the last run computed the MeV columns
@@ -863,6 +865,7 @@ bool want_split(const hadros_tracer* t)
 {
     if (t->path == HADROS_PATH_FUSED) return false;
     if (t->path == HADROS_PATH_SPLIT || t->mev.on) return true;   // the MeV channel lives in the transfer kernel
+    if (t->pool_too_big) return false;
     return t->args.cam.mode == HADROS_MODE_CFG &&
            3.0 * double(t->npix) * CHUNK_DOUBLES * sizeof(double) <= SPLIT_POOL_MAX_BYTES;
 }
@@ -872,6 +875,7 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
     if (t->mev.on && t->path == HADROS_PATH_FUSED)
         return fail(HADROS_ERR_ARG, "the MeV channel runs in the split form: HADROS_PATH_FUSED cannot compute it");
     HIPC(launch_zero_u64(t->d_cnt, 4, s));
+    t->stream = s;
     t->split = want_split(t);
     mev_apply(t->mev, t->rargs);
     t->ran_mev = t->split && t->mev.on;
@@ -911,12 +915,14 @@ int tracer_settle(hadros_tracer* t)
         if (t->path == HADROS_PATH_SPLIT || t->mev.on || bytes <= SPLIT_POOL_MAX_BYTES) {
             if ((rc = tracer_pool(t, grown - t->npix))) return rc;
         } else {
+            // AUTO only: this camera runs fused from now on; the caller's path
+            // setting stays as it was (a later MeV run still takes the split form)
             dfree(t->d_pool);
             dfree(t->d_next);
             t->capacity = 0;
-            t->path = HADROS_PATH_FUSED;
+            t->pool_too_big = true;
         }
-        if ((rc = tracer_launch(t, nullptr))) return rc;
+        if ((rc = tracer_launch(t, t->stream))) return rc;
         HIPC(hipEventSynchronize(t->ev1));
     }
     if (t->split) {
