@@ -31,6 +31,7 @@ PATH_AUTO, PATH_FUSED, PATH_SPLIT = 0, 1, 2
 
 MAX_BINS = 16
 MAX_SIGMA = 1024
+NOALLOC_SCRATCH_BYTES = 32768
 FLAG_EXHAUSTED = 1
 
 # names accepted by the reference parsers (src/torus_profile.cpp:44-91,

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <initializer_list>
+#include <mutex>
 #include <sstream>
 #include <utility>
 #include <string>
@@ -232,16 +233,25 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
     return HADROS_OK;
 }
 
+// Entry k of the device sigma table (3 n doubles): ln E, then ln sigma, then
+// the log-log slope of each interval ending at node m (0 for m = 0).
+double sigma_table_entry(const double* se, const double* ss, int n, int k)
+{
+    if (k < n) return std::log(se[k]);
+    if (k < 2 * n) return std::log(ss[k - n]);
+    const int m = k - 2 * n;
+    if (m == 0) return 0.0;
+    return (std::log(ss[m]) - std::log(ss[m - 1])) / (std::log(se[m]) - std::log(se[m - 1]));
+}
+
 void sigma_tables(const double* se, const double* ss, int n, std::vector<double>& d, std::vector<float>& f)
 {
-    d.assign(3 * size_t(n), 0.0);
-    f.assign(3 * size_t(n), 0.0f);
-    for (int k = 0; k < n; ++k) {
-        d[k] = std::log(se[k]);
-        d[n + k] = std::log(ss[k]);
+    d.resize(3 * size_t(n));
+    f.resize(3 * size_t(n));
+    for (int k = 0; k < 3 * n; ++k) {
+        d[size_t(k)] = sigma_table_entry(se, ss, n, k);
+        f[size_t(k)] = float(d[size_t(k)]);
     }
-    for (int k = 1; k < n; ++k) d[2 * n + k] = (d[n + k] - d[n + k - 1]) / (d[k] - d[k - 1]);
-    for (size_t k = 0; k < d.size(); ++k) f[k] = float(d[k]);
 }
 
 // KerrMetric on the host (src/kerr_metric.cpp:10-144) for the observer constants.
@@ -1085,6 +1095,138 @@ int hadros_trace_rt(const hadros_camera* cam, const hadros_scene* scene, const d
     if (!rc && stats)
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- allocation-free fused trace
+// Device scratch of hadros_trace_rt_noalloc, one copy per device, held until
+// the process ends.  Layout (bytes): [0, 32) the four counters when the caller
+// passes none; [NOALLOC_RT_OFF, +sizeof(DevRt)) the scene block the fused
+// kernel re-reads per path point; [NOALLOC_TAB_OFF, ...) the sigma table,
+// 3 n doubles (FP64) or 3 n floats (FP32K).  Host code never names the array:
+// it uses the address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) unsigned char scratch[HADROS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t NOALLOC_RT_OFF = 256;
+constexpr size_t NOALLOC_TAB_OFF = NOALLOC_RT_OFF + (sizeof(DevRt) + 255) / 256 * 256;
+static_assert(sizeof(DevRt) % 8 == 0, "the scene block is staged in 8-byte words");
+static_assert(NOALLOC_TAB_OFF + 3 * size_t(HADROS_MAX_SIGMA) * sizeof(double) <= HADROS_NOALLOC_SCRATCH_BYTES,
+              "the scratch holds the largest sigma table");
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[MAX_DEVICES] = {};
+
+// this device's copy of the scratch array (looked up once per device, with it current)
+int noalloc_scratch(unsigned char** out)
+{
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(HADROS_ERR_DEVICE, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(HADROS_ERR_DEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[dev];
+    return HADROS_OK;
+}
+
+// Host data reaches the scratch as kernel arguments (2 KiB per launch), so the
+// call needs no staging buffer and no host-to-device copy.
+constexpr int STAGE_WORDS = 256;
+struct StageBlock {
+    unsigned long long w[STAGE_WORDS];
+    unsigned long long* dst;
+    int n;
+};
+
+__global__ void k_stage(StageBlock b)
+{
+    for (int t = threadIdx.x; t < b.n; t += blockDim.x) b.dst[t] = b.w[t];
+}
+
+// fill(word_index) -> 64-bit word; writes n_words words at dst on stream s
+template <class Fill>
+int stage_words(unsigned long long* dst, long n_words, Fill fill, hipStream_t s)
+{
+    StageBlock b;
+    for (long w0 = 0; w0 < n_words; w0 += STAGE_WORDS) {
+        b.n = int(std::min<long>(STAGE_WORDS, n_words - w0));
+        b.dst = dst + w0;
+        for (int t = 0; t < b.n; ++t) b.w[t] = fill(w0 + t);
+        hipLaunchKernelGGL(k_stage, dim3(1), dim3(256), 0, s, b);
+        HIPC(hipGetLastError());
+    }
+    return HADROS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hadros_trace_rt_noalloc(const hadros_camera* cam, const hadros_scene* scene, const double* se, const double* ss,
+                            int n_sigma, int precision, const hadros_image_out* out, int64_t* counters, void* stream)
+{
+    if (!cam || !scene || !out) return fail(HADROS_ERR_ARG, "NULL argument");
+    if (precision != HADROS_PREC_FP64 && precision != HADROS_PREC_FP32K) return fail(HADROS_ERR_ARG, "unknown precision");
+    int rc = ensure_device();
+    if (rc) return rc;
+    TraceArgs A{};
+    if ((rc = make_cam(*cam, A.cam)) || (rc = make_rt(*scene, se, ss, n_sigma, cam->a_spin, A.rt))) return rc;
+    const bool fp64 = precision == HADROS_PREC_FP64;
+    const size_t tab_bytes = 3 * size_t(n_sigma) * (fp64 ? sizeof(double) : sizeof(float));
+    if (NOALLOC_TAB_OFF + tab_bytes > size_t(HADROS_NOALLOC_SCRATCH_BYTES))
+        return fail(HADROS_ERR_ARG, "sigma table larger than the allocation-free scratch (HADROS_NOALLOC_SCRATCH_BYTES)");
+    unsigned char* base = nullptr;
+    if ((rc = noalloc_scratch(&base))) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+
+    // the counters must start at zero: a fill kernel, not a memset node
+    unsigned long long* ctr =
+        counters ? reinterpret_cast<unsigned long long*>(counters) : reinterpret_cast<unsigned long long*>(base);
+    HIPC(launch_zero_u64(ctr, 4, s));
+    const DevRt rt = A.rt;
+    auto rt_word = [&rt](long w) {
+        unsigned long long v;
+        std::memcpy(&v, reinterpret_cast<const unsigned char*>(&rt) + 8 * w, 8);
+        return v;
+    };
+    if ((rc = stage_words(reinterpret_cast<unsigned long long*>(base + NOALLOC_RT_OFF), long(sizeof(DevRt) / 8), rt_word,
+                          s)))
+        return rc;
+    unsigned long long* tab = reinterpret_cast<unsigned long long*>(base + NOALLOC_TAB_OFF);
+    if (fp64) {
+        auto word = [&](long w) {
+            const double d = sigma_table_entry(se, ss, n_sigma, int(w));
+            unsigned long long v;
+            std::memcpy(&v, &d, 8);
+            return v;
+        };
+        if ((rc = stage_words(tab, 3L * n_sigma, word, s))) return rc;
+    } else {
+        const long nf = 3L * n_sigma;   // two floats per word, the last one padded with 0
+        auto word = [&](long w) {
+            float f[2] = {float(sigma_table_entry(se, ss, n_sigma, int(2 * w))), 0.0f};
+            if (2 * w + 1 < nf) f[1] = float(sigma_table_entry(se, ss, n_sigma, int(2 * w + 1)));
+            unsigned long long v;
+            std::memcpy(&v, f, 8);
+            return v;
+        };
+        if ((rc = stage_words(tab, (nf + 1) / 2, word, s))) return rc;
+    }
+    A.rtg = reinterpret_cast<const DevRt*>(base + NOALLOC_RT_OFF);
+    A.sig = fp64 ? reinterpret_cast<const double*>(tab) : nullptr;
+    A.sigf = fp64 ? nullptr : reinterpret_cast<const float*>(tab);
+    A.out = TraceOut{out->tau, out->p_surv, out->intensity, out->alpha, out->beta, out->bin_tau, out->bin_p,
+                     out->bin_i, out->captured, out->npoints, out->flags, ctr};
+    HIPC(launch_trace_rt(A, precision, 128, s));
+    return HADROS_OK;
 }
 
 // ---------------------------------------------------------------- replay plan

@@ -26,7 +26,7 @@ EXPORTS = [
     "hadros_last_error", "hadros_abi_version", "hadros_device_info", "hadros_camera_defaults",
     "hadros_scene_defaults", "hadros_parse_profile", "hadros_parse_source", "hadros_parse_spectrum",
     "hadros_profile_name", "hadros_source_name", "hadros_spectrum_name", "hadros_sigma_load",
-    "hadros_sigma_check", "hadros_trace_rt", "hadros_tracer_create", "hadros_tracer_set_block", "hadros_tracer_set_path",
+    "hadros_sigma_check", "hadros_trace_rt", "hadros_trace_rt_noalloc", "hadros_tracer_create", "hadros_tracer_set_block", "hadros_tracer_set_path",
     "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_kernel_ms", "hadros_tracer_fetch",
     "hadros_tracer_destroy",
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
@@ -80,6 +80,8 @@ def lib():
     L.hadros_sigma_check.argtypes = [_dp, _dp, C.c_int]
     L.hadros_trace_rt.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
                                   C.POINTER(abi.ImageOut), C.POINTER(abi.Stats)]
+    L.hadros_trace_rt_noalloc.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
+                                          C.POINTER(abi.ImageOut), C.c_void_p, C.c_void_p]
     L.hadros_tracer_create.argtypes = [C.POINTER(abi.Camera), C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_int,
                                        C.POINTER(C.c_void_p)]
     L.hadros_tracer_set_block.argtypes = [C.c_void_p, C.c_int]

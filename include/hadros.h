@@ -328,6 +328,32 @@ int hadros_trace_rt(const hadros_camera* cam, const hadros_scene* scene,
                     const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
                     int precision, hadros_image_out* out, hadros_stats* stats);
 
+/* Allocation-free variant of hadros_trace_rt.  It has no counterpart in the
+   reference: it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   It runs the fused kernel (the kernels of hadros_tracer_run with
+   HADROS_PATH_FUSED, so its results equal that path's bit for bit) and calls
+   no hipMalloc, hipFree or host-to-device copy: the scene block, the sigma
+   table and the counters live in a static device array of
+   HADROS_NOALLOC_SCRATCH_BYTES per GPU, kept until the process ends, and
+   reach it as kernel arguments.
+   - Every non-NULL member of `out` is DEVICE memory of the current device,
+     sized as for hadros_trace_rt (nx*ny per pixel column, nx*ny*n_bins per
+     bin column, where n_bins is the number of bins the scene integrates).
+   - `counters`: NULL, or 4 int64 of device memory that receive the points,
+     attempts, captured and exhausted counts (cleared by the call).
+   - Asynchronous on `stream` (a hipStream_t or NULL); no host synchronisation.
+   - Calls that overlap on one GPU (on different streams or threads) share
+     the scratch and race: order them on one stream.
+   - Host inputs are read before the call returns.
+   - The scratch holds tables of up to HADROS_MAX_SIGMA rows; a larger
+     layout is rejected with HADROS_ERR_ARG. */
+#define HADROS_NOALLOC_SCRATCH_BYTES 32768
+int hadros_trace_rt_noalloc(const hadros_camera* cam, const hadros_scene* scene,
+                            const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                            int precision, const hadros_image_out* out, int64_t* counters,
+                            void* stream);
+
 /* Re-usable plan of hadros_trace_rt: device buffers are allocated once,
    run() only launches (asynchronously on `stream`, a hipStream_t or NULL),
    fetch() synchronises and copies results to host. */
