@@ -92,7 +92,45 @@ __global__ void k_eval_metric(double a, const double* r, const double* th, doubl
     o[5] = gi.tt; o[6] = gi.tph; o[7] = gi.rr; o[8] = gi.thth; o[9] = gi.phph;
 }
 
+// One integrator step of each form the kernels take, on states y (8 doubles
+// each): out[11 k ..] = y4 (r, theta, phi, p_r, p_theta), y5 (same), ok.
+//   EVAL_STEP_RK4        KerrGeodesic::step_rk4 (rk4_ref): y4 = y5 = result
+//   EVAL_STEP_RKF45_REF  a REF-mode Fehlberg attempt (rkf45<REF>)
+//   EVAL_STEP_RKF45_CFG  a CFG attempt with sincos_m at every stage (rkf45_cfg)
+//   EVAL_STEP_RKF45_CFG_SHIFT  the CFG attempt with the later stages' sin/cos by
+//                        angle addition (rkf45_cfg_shift); ok = 0 when a stage
+//                        offset leaves sincos_shift's range (the kernels then
+//                        redo the attempt with rkf45_cfg)
+__global__ void k_eval_step(double a, int kind, double h, const double* y, double* out, long n)
+{
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double* s = y + 8 * k;
+    const Y5 yy{s[1], s[2], s[3], s[5], s[6]};
+    Y5 y4{}, y5{};
+    bool ok = true;
+    if (kind == EVAL_STEP_RK4) {
+        y4 = y5 = rk4_ref(a, yy, s[4], s[7], h);
+    } else if (kind == EVAL_STEP_RKF45_REF) {
+        rkf45<HADROS_MODE_REF>(a, yy, s[4], s[7], h, y4, y5);
+    } else if (kind == EVAL_STEP_RKF45_CFG) {
+        rkf45_cfg(a, yy, s[4], s[7], h, y4, y5);
+    } else {
+        ok = rkf45_cfg_shift(a, yy, s[4], s[7], h, y4, y5);
+    }
+    double* o = out + 11 * k;
+    o[0] = y4.r; o[1] = y4.th; o[2] = y4.ph; o[3] = y4.pr; o[4] = y4.pth;
+    o[5] = y5.r; o[6] = y5.th; o[7] = y5.ph; o[8] = y5.pr; o[9] = y5.pth;
+    o[10] = ok ? 1.0 : 0.0;
+}
+
 static inline int blocks(long n) { return int((n + 255) / 256); }
+
+hipError_t launch_eval_step(double a, int kind, double h, const double* y, double* out, long n)
+{
+    hipLaunchKernelGGL(k_eval_step, dim3(blocks(n)), dim3(256), 0, 0, a, kind, h, y, out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_eval_rho(const DevMedium& m, const double* r, const double* th, double* out, long n)
 {

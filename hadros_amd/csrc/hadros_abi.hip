@@ -1229,6 +1229,35 @@ int hadros_trace_rt_noalloc(const hadros_camera* cam, const hadros_scene* scene,
     return HADROS_OK;
 }
 
+// ------------------------------------------------------- legacy optical depth
+// optical_depth::tau_along_ray (src/optical_depth.cpp:56-91) is the transfer
+// kernel's tau with a scene of its own: one bin at E_inf, the black-hole mass
+// fixed at 3 M_sun (:83-85), no horizon cut, and no emission.
+int hadros_tau_along_ray(const hadros_cache_view* c, const hadros_medium* m, const double* se, const double* ss,
+                         int n_sigma, double enu_inf, double* tau, double* p_surv)
+{
+    if (!c || !m) return fail(HADROS_ERR_ARG, "NULL argument");
+    hadros_scene sc;
+    hadros_scene_defaults(&sc);
+    sc.medium = *m;
+    sc.enu_obs_GeV = enu_inf;
+    sc.m_bh_msun = 3.0;
+    sc.spectrum.kind = HADROS_SPECTRUM_MONOCHROMATIC;
+    sc.source.kind = HADROS_SOURCE_INNER_RING;
+    sc.source.emax_GeV = 0.0;   // emissivity_collapsar_ring returns 0: no intensity
+    hadros_replay* p = nullptr;
+    int rc = hadros_replay_create(c, &sc, se, ss, n_sigma, HADROS_PREC_FP64, &p);
+    if (rc) return rc;
+    p->args.rt.r_h_rt = -HUGE_VAL;   // every point counts, inside the horizon too
+    rc = hadros_replay_run(p, nullptr);
+    hadros_image_out o{};
+    o.tau = tau;
+    o.p_surv = p_surv;
+    if (!rc) rc = hadros_replay_fetch(p, &o, nullptr);
+    hadros_replay_destroy(p);
+    return rc;
+}
+
 // ---------------------------------------------------------------- replay plan
 int hadros_replay_create(const hadros_cache_view* c, const hadros_scene* scene, const double* se, const double* ss,
                          int n_sigma, int precision, hadros_replay** out)
@@ -2151,6 +2180,13 @@ int hadros_eval_mev(const hadros_scene* scene, const hadros_mev* mev, const doub
     const DevMev M = make_mev(*mev, med);
     return run_eval({{r, size_t(n)}, {th, size_t(n)}, {g, size_t(n)}}, out, size_t(n) * 4,
                     [&](std::vector<double*>& d, double* o) { return launch_eval_mev(med, M, d[0], d[1], d[2], o, long(n)); });
+}
+
+int hadros_eval_step(double a, int kind, double h, const double* y, double* out, int64_t n)
+{
+    if (kind < EVAL_STEP_RK4 || kind > EVAL_STEP_RKF45_CFG_SHIFT) return fail(HADROS_ERR_ARG, "unknown step kind");
+    return run_eval({{y, size_t(n) * 8}}, out, size_t(n) * 11,
+                    [&](std::vector<double*>& d, double* o) { return launch_eval_step(a, kind, h, d[0], o, long(n)); });
 }
 
 int hadros_eval_rhs(double a, int mode, const double* y, double* dy, int64_t n)

@@ -238,5 +238,7 @@ hipError_t launch_eval_rhs(double a, int mode, const double* y, double* dy, long
 hipError_t launch_eval_mev(const DevMedium& med, const DevMev& M, const double* r, const double* th, const double* g,
                            double* out, long n);
 hipError_t launch_eval_metric(double a, const double* r, const double* th, double* out, long n);
+enum { EVAL_STEP_RK4 = 0, EVAL_STEP_RKF45_REF = 1, EVAL_STEP_RKF45_CFG = 2, EVAL_STEP_RKF45_CFG_SHIFT = 3 };
+hipError_t launch_eval_step(double a, int kind, double h, const double* y, double* out, long n);
 
 }  // namespace hd

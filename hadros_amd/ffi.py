@@ -32,6 +32,7 @@ EXPORTS = [
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric", "hadros_eval_mev",
+    "hadros_eval_step", "hadros_tau_along_ray",
     "hadros_tracer_set_mev", "hadros_tracer_fetch_mev", "hadros_replay_set_mev", "hadros_replay_fetch_mev",
     "hadros_mev_defaults", "hadros_parse_mev_model", "hadros_parse_mev_flavor", "hadros_parse_mev_thermal",
     "hadros_parse_mev_ye", "hadros_parse_mev_spectral",
@@ -145,6 +146,9 @@ def lib():
     for f in ("model", "flavor", "thermal", "ye", "spectral"):
         getattr(L, "hadros_parse_mev_" + f).argtypes = [C.c_char_p, C.POINTER(C.c_int32)]
     L.hadros_eval_rhs.argtypes = [C.c_double, C.c_int, _dp, _dp, C.c_int64]
+    L.hadros_eval_step.argtypes = [C.c_double, C.c_int, C.c_double, _dp, _dp, C.c_int64]
+    L.hadros_tau_along_ray.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double,
+                                       _dp, _dp]
     L.hadros_eval_metric.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int64]
     _lib = L
     return L
@@ -684,6 +688,29 @@ def eval_rhs(a, mode, y):
     out = np.zeros_like(y)
     check(lib().hadros_eval_rhs(a, mode, dptr(y), dptr(out), y.shape[0]))
     return out
+
+
+STEP_RK4, STEP_RKF45_REF, STEP_RKF45_CFG, STEP_RKF45_CFG_SHIFT = 0, 1, 2, 3
+
+
+def eval_step(a, kind, h, y):
+    """One integrator step of each state (hadros_eval_step): (y4 [n x 5], y5 [n x 5], ok [n])
+    with components r, theta, phi, p_r, p_theta."""
+    y = np.ascontiguousarray(y, np.float64).reshape(-1, 8)
+    out = np.zeros((y.shape[0], 11))
+    check(lib().hadros_eval_step(a, kind, h, dptr(y), dptr(out), y.shape[0]))
+    return out[:, :5], out[:, 5:10], out[:, 10] != 0.0
+
+
+def tau_along_ray(cache, medium, sig_e, sig_s, enu_inf):
+    """The legacy per-ray optical depth and survival of a cache (hadros_tau_along_ray)."""
+    se, ss = _sig(sig_e, sig_s)
+    tau = np.zeros(cache.n_rays)
+    p = np.zeros(cache.n_rays)
+    v = cache.view()
+    check(lib().hadros_tau_along_ray(C.byref(v), C.byref(medium), dptr(se), dptr(ss), se.size, float(enu_inf),
+                                     dptr(tau), dptr(p)))
+    return tau, p
 
 
 def eval_metric(a, r, th):

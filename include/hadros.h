@@ -392,6 +392,16 @@ int hadros_tracer_set_mev(hadros_tracer* t, const hadros_mev* mev);
 int hadros_tracer_fetch_mev(hadros_tracer* t, hadros_mev_out* out);
 int hadros_tracer_destroy(hadros_tracer* t);
 
+/* optical_depth::tau_along_ray and survival_probability
+   (src/optical_depth.cpp:50-91) for every ray of a cache: tau = sum over the
+   ray's points of rho/m_u sigma(E_inf z) dl r_g, with r_g of 3 M_sun whatever
+   the black hole (:83-85), every point counted (no horizon cut) and points
+   whose local energy E_inf z leaves the table skipped; P = exp(-tau).  tau
+   and p_surv receive n_rays values each (either may be NULL). */
+int hadros_tau_along_ray(const hadros_cache_view* cache, const hadros_medium* medium,
+                         const double* sigma_e_gev, const double* sigma_cm2, int n_sigma,
+                         double enu_inf_GeV, double* tau, double* p_surv);
+
 /* Cache replay: radiative_transfer::integrate_kerr_ray_spectral over every ray
    of a KGEO cache (apps/compute_kerr_image_from_cache.cpp:605-658;
    apps/compute_kerr_image_cuda.cu:1293-1335).  Host arrays in, host results
@@ -563,6 +573,16 @@ int hadros_eval_mev(const hadros_scene* scene, const hadros_mev* mev, const doub
    differences) or the analytic derivative for mode CFG; y and dy are n x 8
    (t, r, theta, phi, p_t, p_r, p_theta, p_phi). */
 int hadros_eval_rhs(double a_spin, int mode, const double* y, double* dy, int64_t n);
+/* One step of the integrator forms on n states y (n x 8): out is n x 11, the
+   4th- and 5th-order results (r, theta, phi, p_r, p_theta each) and a flag.
+   kind 0: KerrGeodesic::step_rk4 (src/kerr_geodesic.cpp:137-192), both
+   results equal; 1: one Fehlberg attempt of step_adaptive
+   (src/kerr_geodesic.cpp:239-308) on the REF right-hand side; 2: the same
+   attempt on the CFG right-hand side; 3: the CFG attempt with the later
+   stages' sin/cos by angle addition, as the tracing kernels take it (flag 0
+   when a stage's theta offset leaves that form's range; the kernels then
+   redo the attempt as kind 2). */
+int hadros_eval_step(double a_spin, int kind, double h, const double* y, double* out, int64_t n);
 /* KerrMetric::metric / inverse_metric (src/kerr_metric.cpp:34-116):
    5 distinct components each (tt, tphi, rr, thth, phph) per point, n x 10. */
 int hadros_eval_metric(double a_spin, const double* r, const double* th,

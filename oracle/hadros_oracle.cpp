@@ -660,7 +660,7 @@ struct RtCtx {
 
 // ---------------------------------------------------------------- tracing
 // The loop of KerrCamera::trace_pixel (src/kerr_camera.cpp:188-283); `emit`
-// receives every PathPoint.  REF: stop at r <= r_+ + 1e-3; CFG: r < 1.01 r_+.
+// receives every PathPoint and the state it was taken from.  REF: stop at r <= r_+ + 1e-3; CFG: r < 1.01 r_+.
 template <class Emit>
 void trace(const hadros_camera* c, int i, int j, Emit&& emit, int& captured, int& exhausted, int64_t& attempts)
 {
@@ -701,7 +701,7 @@ void trace(const hadros_camera* c, int i, int j, Emit&& emit, int& captured, int
         pt[4] = y[1] * cth;
         pt[5] = (step == 0) ? 0.0 : zamo_dl(a, y, yp);
         pt[6] = zamo_energy(a, y[1], y[2], y[4], y[7]) / max_(energy_obs, 1.0e-300);
-        emit(pt);
+        emit(pt, y);
         std::memcpy(yp, y, sizeof(y));
         if (cfg) {
             bool ok = false;
@@ -753,6 +753,30 @@ double or_hamiltonian(double a, const double y[8])
 
 void or_step_ref(double a, double h, double tol, double y[8]) { step_ref(a, h, tol, y); }
 
+void or_step_rk4(double a, double h, double y[8]) { rk4_ref(a, h, y); }
+
+void or_rkf45_attempt(double a, int mode, double h, const double y[8], double y4[8], double y5[8])
+{
+    RKF o;
+    rkf45(a, mode, y, h, o);
+    std::memcpy(y4, o.y4, sizeof(o.y4));
+    std::memcpy(y5, o.y5, sizeof(o.y5));
+}
+
+int64_t or_trace_states(const hadros_camera* cam, int i, int j, double* states, int64_t cap)
+{
+    int64_t n = 0;
+    int cp = 0, ex = 0;
+    int64_t at = 0;
+    trace(cam, i, j,
+          [&](const double*, const double* y) {
+              if (states && n < cap) std::memcpy(states + 8 * n, y, 8 * sizeof(double));
+              ++n;
+          },
+          cp, ex, at);
+    return n;
+}
+
 void or_initial_state(const hadros_camera* cam, int i, int j, double y[8]) { initial_state(cam, i, j, y); }
 
 int64_t or_trace_pixel(const hadros_camera* cam, int i, int j, double* pts, int64_t cap, int* captured,
@@ -762,7 +786,7 @@ int64_t or_trace_pixel(const hadros_camera* cam, int i, int j, double* pts, int6
     int cp = 0, ex = 0;
     int64_t at = 0;
     trace(cam, i, j,
-          [&](const double* p) {
+          [&](const double* p, const double*) {
               if (pts && n < cap) std::memcpy(pts + 7 * n, p, 7 * sizeof(double));
               ++n;
           },
@@ -873,7 +897,7 @@ int or_trace_image_range(const hadros_camera* cam, const hadros_scene* s, const 
         int cp = 0, ex = 0;
         int64_t at = 0;
         trace(cam, i, j,
-              [&](const double* p) {
+              [&](const double* p, const double*) {
                   ctx.point(p[0], p[1], p[5], p[6], tau, I);
                   ++np;
               },
@@ -1244,6 +1268,35 @@ int or_rt_from_cache(const hadros_cache_view* c, const hadros_scene* s, const do
         if (out->tau) out->tau[k] = agg[0];
         if (out->p_surv) out->p_surv[k] = agg[1];
         if (out->intensity) out->intensity[k] = agg[2];
+    }
+    return HADROS_OK;
+}
+
+// optical_depth::tau_along_ray (src/optical_depth.cpp:56-91) and
+// survival_probability (:50-53): every point counts (no horizon cut), the
+// local energy is E_inf times the point's redshift factor, points whose
+// energy leaves the table are skipped, and dl is scaled by r_g of 3 M_sun
+// whatever the scene's mass (:83-85).  out: tau, P per ray (either NULL).
+int or_tau_along_ray(const hadros_cache_view* c, const hadros_medium* m, const double* sig_e, const double* sig_s,
+                     int n_sig, double enu_inf, double* tau_out, double* p_out)
+{
+    if (!c || !m || n_sig < 2) return HADROS_ERR_ARG;
+    const Medium med(m);
+    const Sigma sig(sig_e, sig_s, n_sig);
+    const double rg3 = G_CGS * 3.0 * MSUN_G / (C_CGS * C_CGS);   // constants::rg_cm(3.0)
+    for (int64_t k = 0; k < c->n_rays; ++k) {
+        double tau = 0.0;
+        for (int64_t q = c->offset[k]; q < c->offset[k + 1]; ++q) {
+            const double rho = med.rho(c->r[q], c->theta[q]);
+            const double nb = rho / M_U_G;
+            const double E = enu_inf * c->redshift[q];
+            if (E < sig.emin() || E > sig.emax()) continue;
+            const double sc = sig.eval(E);
+            const double dl_cm = c->dl[q] * rg3;
+            tau += nb * sc * dl_cm;
+        }
+        if (tau_out) tau_out[k] = tau;
+        if (p_out) p_out[k] = std::exp(-tau);
     }
     return HADROS_OK;
 }

@@ -55,6 +55,24 @@ double or_hamiltonian(double a, const double y[8]);
    (src/kerr_geodesic.cpp:194-329). */
 void or_step_ref(double a, double h, double tol, double y[8]);
 
+/* KerrGeodesic::step_rk4 (src/kerr_geodesic.cpp:137-192): one classic RK4
+   step of size h on the REF right-hand side (the fallback of step_adaptive
+   after 50 rejected attempts, :328). */
+void or_step_rk4(double a, double h, double y[8]);
+
+/* One Fehlberg 4(5) attempt of size h (tableau of src/kerr_geodesic.cpp:239-308)
+   on the REF or CFG right-hand side: the 4th- and 5th-order results. */
+void or_rkf45_attempt(double a, int mode, double h, const double y[8], double y4[8], double y5[8]);
+
+/* optical_depth::tau_along_ray (src/optical_depth.cpp:56-91) over every ray of
+   a cache, with survival_probability (:50-53): tau_out, p_out per ray. */
+int or_tau_along_ray(const hadros_cache_view* c, const hadros_medium* m, const double* sig_e, const double* sig_s,
+                     int n_sig, double enu_inf, double* tau_out, double* p_out);
+
+/* The state (t, r, theta, phi, p_t, p_r, p_theta, p_phi) at every emitted
+   point of or_trace_pixel's ray; returns the number of points. */
+int64_t or_trace_states(const hadros_camera* cam, int i, int j, double* states, int64_t cap);
+
 /* KerrCamera::initial_state (src/kerr_camera.cpp:110-186): y0 of pixel (i,j). */
 void or_initial_state(const hadros_camera* cam, int i, int j, double y[8]);
 

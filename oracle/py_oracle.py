@@ -54,6 +54,12 @@ def lib():
         L.or_rhs.argtypes = [C.c_double, C.c_int, _dp, _dp]
         L.or_hamiltonian.argtypes = [C.c_double, _dp]
         L.or_step_ref.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        L.or_step_rk4.argtypes = [C.c_double, C.c_double, _dp]
+        L.or_rkf45_attempt.argtypes = [C.c_double, C.c_int, C.c_double, _dp, _dp, _dp]
+        L.or_trace_states.restype = C.c_int64
+        L.or_tau_along_ray.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double,
+                                       _dp, _dp]
+        L.or_trace_states.argtypes = [C.POINTER(abi.Camera), C.c_int, C.c_int, _dp, C.c_int64]
         L.or_initial_state.argtypes = [C.POINTER(abi.Camera), C.c_int, C.c_int, _dp]
         L.or_trace_pixel.restype = C.c_int64
         L.or_trace_pixel.argtypes = [C.POINTER(abi.Camera), C.c_int, C.c_int, _dp, C.c_int64,
@@ -101,6 +107,9 @@ def ref():
         R.ref_hamiltonian.restype = C.c_double
         R.ref_hamiltonian.argtypes = [C.c_double, _dp]
         R.ref_step_adaptive.argtypes = [C.c_double, C.c_double, C.c_double, _dp]
+        R.ref_step_rk4.argtypes = [C.c_double, C.c_double, _dp]
+        R.ref_tau_along_ray.argtypes = [C.c_char_p, C.c_char_p, _dp, C.c_double, _dp, _dp, _dp, _dp,
+                                        C.POINTER(C.c_long), C.c_long, _dp]
         R.ref_trace_pixel.restype = C.c_long
         R.ref_trace_pixel.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_double, C.c_double, C.c_int, C.c_int, _dp, C.c_long,
@@ -178,6 +187,27 @@ def trace_pixel(cam, i, j, cap=400000):
     cp, ex, at = C.c_int(), C.c_int(), C.c_int64()
     n = L.or_trace_pixel(C.byref(cam), i, j, dptr(pts), cap, C.byref(cp), C.byref(ex), C.byref(at))
     return pts[: min(n, cap)].copy(), cp.value, ex.value, at.value
+
+
+def trace_states(cam, i, j, cap=400000):
+    """[n x 8] state (t, r, theta, phi, p_t, p_r, p_theta, p_phi) at every emitted point of a ray."""
+    L = lib()
+    st = np.zeros((cap, 8))
+    n = L.or_trace_states(C.byref(cam), i, j, dptr(st), cap)
+    return st[: min(n, cap)].copy()
+
+
+def tau_along_ray(view, medium, sig_e, sig_s, enu_inf):
+    """Legacy per-ray tau and survival of a cache view (optical_depth::tau_along_ray)."""
+    L = lib()
+    sig_e = np.ascontiguousarray(sig_e, np.float64)
+    sig_s = np.ascontiguousarray(sig_s, np.float64)
+    tau = np.zeros(view.n_rays)
+    p = np.zeros(view.n_rays)
+    rc = L.or_tau_along_ray(C.byref(view), C.byref(medium), dptr(sig_e), dptr(sig_s), sig_e.size, enu_inf,
+                            dptr(tau), dptr(p))
+    assert rc == 0, rc
+    return tau, p
 
 
 def bins(scene, sig_e):

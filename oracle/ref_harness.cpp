@@ -11,6 +11,7 @@
 #include "kerr_geodesic.hpp"
 #include "kerr_metric.hpp"
 #include "mev_neutrino_physics.hpp"
+#include "optical_depth.hpp"
 #include "radiative_transfer.hpp"
 #include "sigma_table.hpp"
 #include "torus_profile.hpp"
@@ -70,6 +71,14 @@ void ref_step_adaptive(double a, double h, double tol, double* y)
     KerrGeodesic g(KerrMetric(a), h, tol);
     GeodesicState s = to_state(y);
     g.step_adaptive(s);
+    from_state(s, y);
+}
+
+void ref_step_rk4(double a, double h, double* y)
+{
+    KerrGeodesic g(KerrMetric(a), h);
+    GeodesicState s = to_state(y);
+    g.step_rk4(s);
     from_state(s, y);
 }
 
@@ -194,6 +203,32 @@ int ref_integrate(const char* sigma_path, const char* profile, const double* tp,
         out8[0] = rt.tau; out8[1] = rt.P_surv; out8[2] = rt.I_obs; out8[3] = rt.tau_mev;
         out8[4] = rt.P_surv_mev; out8[5] = rt.I_obs_mev; out8[6] = rt.r_neutrinosphere_rg;
         out8[7] = rt.leakage_factor;
+    } catch (...) {
+        return 1;
+    }
+    return 0;
+}
+
+// optical_depth::tau_along_ray and survival_probability on rays given as SoA
+// (r, theta, dl, redshift) ranges [off[k], off[k+1]); out2 = tau, P per ray.
+int ref_tau_along_ray(const char* sigma_path, const char* profile, const double* tp, double enu,
+                      const double* r, const double* th, const double* dl, const double* z, const long* off,
+                      long n_rays, double* out2)
+{
+    try {
+        SigmaTable sig(sigma_path);
+        TorusProfile t = make_torus(profile, tp);
+        for (long k = 0; k < n_rays; ++k) {
+            RayPath ray;
+            for (long q = off[k]; q < off[k + 1]; ++q) {
+                PathPoint p{};
+                p.r_rg = r[q]; p.theta = th[q]; p.dl_rg = dl[q]; p.redshift_factor = z[q];
+                ray.points.push_back(p);
+            }
+            const double tau = optical_depth::tau_along_ray(ray, enu, t, sig);
+            out2[2 * k] = tau;
+            out2[2 * k + 1] = optical_depth::survival_probability(tau);
+        }
     } catch (...) {
         return 1;
     }
