@@ -59,6 +59,17 @@ constexpr double EXP_L1 = 0x1.62e42fef00000p-7;      // ln2/64, high 33 bits
 constexpr double EXP_L2 = 0x1.473de6af278edp-40;     // ln2/64 - EXP_L1
 constexpr double EXP_ZERO_BELOW = -745.2;            // e^x < 2^-1075 below this
 
+// exp_tab for arguments known to lie above EXP_ZERO_BELOW (no underflow test)
+HD_FM_QUAL double exp_tab_nz(double x, const double* T)
+{
+    const double k = rint(x * EXP_INV_L);
+    const double r = fma(-k, EXP_L2, fma(-k, EXP_L1, x));
+    const int ki = int(k);
+    const double p =
+        fma(r, fma(r, fma(r, fma(r, fma(r, 1.0 / 120.0, 1.0 / 24.0), 1.0 / 6.0), 0.5), 1.0), 1.0);
+    return ldexp(T[ki & 63] * p, ki >> 6);
+}
+
 HD_FM_QUAL double exp_tab(double x, const double* T)
 {
     const double k = rint(x * EXP_INV_L);

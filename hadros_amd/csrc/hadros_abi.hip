@@ -141,14 +141,22 @@ int check_sigma(const double* e, const double* s, int n)
 // within d/4 of the uniform grid lo0 + k d gets inv_d = 1/d (the guess
 // ceil((ln E - lo0)/d) is then at most one node from the lower bound);
 // other tables get inv_d = 0 (binary search).
-void sigma_guess(const double* se, int n, double& lo0, double& inv_d)
+// exact = 1 when every node lies within 1e-12 d of that grid (log-spaced
+// tables): the lookup then takes the guess as the interval, with no compare.
+void sigma_guess(const double* se, int n, double& lo0, double& inv_d, int* exact = nullptr)
 {
     const double l0 = std::log(se[0]), l1 = std::log(se[n - 1]);
     const double d = (l1 - l0) / double(n - 1);
     bool uniform = d > 0.0;
-    for (int k = 0; uniform && k < n; ++k) uniform = std::fabs(std::log(se[k]) - (l0 + d * double(k))) <= 0.25 * d;
+    double dev = 0.0;
+    for (int k = 0; uniform && k < n; ++k) {
+        const double e = std::fabs(std::log(se[k]) - (l0 + d * double(k)));
+        uniform = e <= 0.25 * d;
+        dev = std::max(dev, e);
+    }
     lo0 = l0;
     inv_d = uniform ? 1.0 / d : 0.0;
+    if (exact) *exact = uniform && dev <= 1.0e-12 * d;
 }
 
 // DevRt of one scene: integrate_kerr_ray_spectral's bins (src/radiative_transfer.cpp:687-761),
@@ -183,7 +191,7 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
     P.n_sigma = n;
     P.e_lo = se[0];
     P.e_hi = se[n - 1];
-    sigma_guess(se, n, P.sig_lo, P.sig_inv_d);
+    sigma_guess(se, n, P.sig_lo, P.sig_inv_d, &P.sig_exact);
     P.rg_cm = G_CGS * s.m_bh_msun * MSUN_G / (C_CGS * C_CGS);
     const double a = dclamp(a_bh, 0.0, 0.999999);
     P.r_h_rt = 1.0 + std::sqrt(1.0 - a * a);

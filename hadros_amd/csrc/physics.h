@@ -73,6 +73,8 @@ struct DevRt {
     double e_lo, e_hi;    // sigma table range
     double sig_lo;        // ln E of the first table node
     double sig_inv_d;     // (n - 1) / (ln E_last - ln E_first) when the nodes are uniform in ln E, else 0
+    int sig_exact;        // the nodes lie within 1e-12 of the spacing of that uniform grid
+    int sig_pad;
     double emid[HADROS_MAX_BINS], lnemid[HADROS_MAX_BINS], de[HADROS_MAX_BINS], w[HADROS_MAX_BINS];
 };
 
@@ -760,6 +762,32 @@ __device__ __forceinline__ int sigma_lower_bound(const T* __restrict__ lnE, int 
         }
     }
     return lo;
+}
+
+// ln sigma at ln E = lx for a table whose ln E nodes are uniform to within
+// 1e-12 of the spacing d (sig_exact): the interval is ceil((lx - lo0) / d),
+// clamped to [1, n-1], with no comparison against the nodes.  Near a node the
+// rounding of that quotient may pick the neighbouring interval; both give the
+// node's value there (the interpolation is continuous), so the result moves by
+// rounding only.  Any lx, NaN included, gives an index inside the table.
+__device__ __forceinline__ double sigma_ln_exact(const double* __restrict__ lnE, const double* __restrict__ lnS,
+                                                 const double* __restrict__ slope, int n, double lx, double lo0,
+                                                 double inv_d)
+{
+    HD_CONTRACT
+    const double t = fmin(fmax((lx - lo0) * inv_d, 1.0), double(n - 1));   // NaN -> 1
+    const int k = int(ceil(t));
+    return lnS[k - 1] + (lx - lnE[k - 1]) * slope[k];
+}
+
+__device__ __forceinline__ float sigma_ln_exact_f(const float* __restrict__ lnE, const float* __restrict__ lnS,
+                                                  const float* __restrict__ slope, int n, float lx, float lo0,
+                                                  float inv_d)
+{
+    HD_CONTRACT
+    const float t = fminf(fmaxf((lx - lo0) * inv_d, 1.0f), float(n - 1));   // NaN -> 1
+    const int k = int(ceilf(t));
+    return lnS[k - 1] + (lx - lnE[k - 1]) * slope[k];
 }
 
 // ln sigma at ln E = lx: log-log interpolation of the table (slope[k] is the

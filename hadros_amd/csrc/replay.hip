@@ -187,19 +187,28 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             pq.rho = 0.0;
         }
         if constexpr (PREC == HADROS_PREC_FP64) {
+            // the scene's bin-loop constants, read once per chunk
+            const double e_lo = P.e_lo, e_hi = P.e_hi, gam = P.gam, invcut = P.invcut;
+            const double sig_lo = P.sig_lo, sig_inv_d = P.sig_inv_d;
+            const bool exact = P.sig_exact != 0;
+            const bool absorbs = ok && pq.rho > 0.0;
+            const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
 #pragma unroll 1
             for (int b = 0; b < nb; ++b) {
                 const double E = bin_e[b] * pq.z;
                 const double lnEb = bin_lne[b] + pq.lnz;
-                double dt = 0.0;
-                if (ok && pq.rho > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                    dt = pq.nbar_dl * exp_tab(sigma_ln(lnE, lnS, slope, n, lnEb, P.sig_lo, P.sig_inv_d), exp_t);
+                // dtau = n_N sigma(E) dl where the point absorbs and E lies in the
+                // table; the lookup runs on every lane (its index stays in the table)
+                // and the mask selects, so the loop body has no branch here
+                const double ls = exact ? sigma_ln_exact(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d)
+                                        : sigma_ln(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d);
+                const double sig_dl = pq.nbar_dl * exp_tab_nz(fmax(ls, EXP_ZERO_BELOW), exp_t);
+                const double dt = (absorbs && E >= e_lo && E <= e_hi) ? sig_dl : 0.0;
                 double inc[1] = {dt};
                 wave_scan_add(inc);
                 const double tau_b = a64.tau[b];
                 const double before = tau_b + (inc[0] - dt);
-                const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
-                const double arg = -P.gam * lnEb - E * P.invcut - before;
+                const double arg = -gam * lnEb - E * invcut - before;
                 // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
                 // every term is zero (behind an opaque column) adds nothing
                 const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
@@ -213,13 +222,18 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
             const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
             const float gam = float(P.gam), invcut = float(P.invcut);
+            const double e_lo = P.e_lo, e_hi = P.e_hi;
+            const float sig_lo = float(P.sig_lo), sig_inv_d = float(P.sig_inv_d);
+            const bool exact = P.sig_exact != 0;
+            const bool absorbs = ok && pq.rho > 0.0 && pq.nbar_dl > 0.0;
 #pragma unroll 1
             for (int b = 0; b < nb; ++b) {
                 const double E = bin_e[b] * pq.z;
                 const float lnEb = float(bin_lne[b]) + lnz;
-                float dt = 0.0f;
-                if (ok && pq.rho > 0.0 && pq.nbar_dl > 0.0 && E >= P.e_lo && E <= P.e_hi)
-                    dt = __expf(lN + sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, float(P.sig_lo), float(P.sig_inv_d)));
+                // as in the FP64 loop: the lookup on every lane, the mask selects
+                const float ls = exact ? sigma_ln_exact_f(lnEf, lnSf, slopef, n, lnEb, sig_lo, sig_inv_d)
+                                       : sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, sig_lo, sig_inv_d);
+                const float dt = (absorbs && E >= e_lo && E <= e_hi) ? __expf(lN + ls) : 0.0f;
                 float inc[1] = {dt};
                 wave_scan_add(inc);
                 const float before = a32.tau[b].s + (inc[0] - dt);
