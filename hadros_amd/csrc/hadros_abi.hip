@@ -514,6 +514,13 @@ struct hadros_batch {
     int *d_next = nullptr, *d_nst = nullptr;
     unsigned long long* d_ctr = nullptr;   // [n_ctr] chunk counter of each sub-batch
     int n_ctr = 0;
+    // with more rays than one sub-batch holds, the pool is two halves: the
+    // replay of a sub-batch (stream s2) overlaps the next sub-batch's
+    // geodesic kernel (the run's stream) in the other half
+    int halves = 1;
+    hipStream_t s2 = nullptr;
+    hipEvent_t evg[2] = {nullptr, nullptr};   // geodesic kernel done, per half
+    hipEvent_t evr[2] = {nullptr, nullptr};   // replay done (half free again)
 };
 
 namespace {
@@ -1608,8 +1615,9 @@ bool batch_want_split(const hadros_batch* b)
 
 double batch_pool_bound(const hadros_batch* b) { return b->pool_max > 0.0 ? b->pool_max : BATCH_POOL_MAX_BYTES; }
 
-// (re)allocate the pool for sub-batches of `sub` rays and `cap` chunks
-int batch_pool(hadros_batch* b, long sub, long cap)
+// (re)allocate the pool for sub-batches of `sub` rays and `cap` chunks, in
+// `halves` halves of that size (and the second stream and events they use)
+int batch_pool(hadros_batch* b, long sub, long cap, int halves)
 {
     dfree(b->d_pool);
     dfree(b->d_next);
@@ -1617,20 +1625,36 @@ int batch_pool(hadros_batch* b, long sub, long cap)
     b->capacity = 0;
     b->sub = 0;
     int rc = 0;
-    if ((rc = dalloc(&b->d_pool, size_t(cap) * CHUNK_DOUBLES)) || (rc = dalloc(&b->d_next, size_t(cap))) ||
-        (rc = dalloc(&b->d_nst, size_t(sub))))
+    const size_t h = size_t(halves);
+    if ((rc = dalloc(&b->d_pool, h * size_t(cap) * CHUNK_DOUBLES)) || (rc = dalloc(&b->d_next, h * size_t(cap))) ||
+        (rc = dalloc(&b->d_nst, h * size_t(sub))))
         return rc;
+    if (halves == 2 && !b->s2) {
+        HIPC(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));   // default priority
+        for (int k = 0; k < 2; ++k) {
+            HIPC(hipEventCreateWithFlags(&b->evg[k], hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&b->evr[k], hipEventDisableTiming));
+        }
+    }
     b->capacity = cap;
     b->sub = sub;
+    b->halves = halves;
     return HADROS_OK;
 }
 
-// first sizing: as many rays per sub-batch as BATCH_CHUNKS_PER_RAY chunks each fit in the bound
+// first sizing: as many rays per sub-batch as BATCH_CHUNKS_PER_RAY chunks each
+// fit in the bound -- or in half of it, when one sub-batch cannot hold them all
 int batch_pool_init(hadros_batch* b)
 {
-    const long fit = long(batch_pool_bound(b) / (CHUNK_BYTES * double(BATCH_CHUNKS_PER_RAY)));
+    const double bound = batch_pool_bound(b);
+    long fit = long(bound / (CHUNK_BYTES * double(BATCH_CHUNKS_PER_RAY)));
+    int halves = 1;
+    if (b->n_rays > fit) {
+        halves = 2;
+        fit = long(0.5 * bound / (CHUNK_BYTES * double(BATCH_CHUNKS_PER_RAY)));
+    }
     const long sub = std::max(1L, std::min(std::min(b->n_rays, fit), BATCH_SUB_MAX));
-    return batch_pool(b, sub, sub * BATCH_CHUNKS_PER_RAY);
+    return batch_pool(b, sub, sub * BATCH_CHUNKS_PER_RAY, halves);
 }
 
 // the first ray_limit rays as sub-batches: k_batch_points into the pool, then
@@ -1652,9 +1676,18 @@ int batch_launch_split(hadros_batch* b, hipStream_t s, long ray_limit)
     HIPC(launch_zero_u64(b->d_cnt, 4, s));
     HIPC(launch_zero_u64(b->d_ctr, int(n_sub), s));
     HIPC(hipEventRecord(b->ev0, s));
+    const bool two = b->halves == 2;
+    hipStream_t sr = two ? b->s2 : s;   // the replays' stream
+    int last_half = 0;
     for (long i = 0; i < n_sub; ++i) {
         const long r0 = i * b->sub, n = std::min(b->sub, ray_limit - r0);
         if (n <= 0) break;
+        const int hf = two ? int(i & 1) : 0;
+        double* pool = b->d_pool + size_t(hf) * size_t(b->capacity) * CHUNK_DOUBLES;
+        int* next = b->d_next + size_t(hf) * size_t(b->capacity);
+        int* nst = b->d_nst + size_t(hf) * size_t(b->sub);
+        // the half is free once the replay of the sub-batch before last is done
+        if (two && i >= 2) HIPC(hipStreamWaitEvent(s, b->evr[hf], 0));
         BatchPointsArgs Q{};
         Q.cams = b->d_cams;
         Q.pix0 = b->d_pix0;
@@ -1663,19 +1696,23 @@ int batch_launch_split(hadros_batch* b, hipStream_t s, long ray_limit)
         Q.ray0 = r0;
         Q.n_rays = n;
         Q.queue = b->d_queue;
-        Q.cp = ChunkPool{b->d_pool, b->d_next, reinterpret_cast<unsigned*>(b->d_ctr + i), n, b->capacity};
-        Q.n_st = b->d_nst;
+        Q.cp = ChunkPool{pool, next, reinterpret_cast<unsigned*>(b->d_ctr + i), n, b->capacity};
+        Q.n_st = nst;
         Q.out = b->args.out;
         HIPC(launch_zero_u64(b->d_queue, 1, s));
         HIPC(launch_batch_points(Q, b->mode, grid, b->block, s));
+        if (two) {
+            HIPC(hipEventRecord(b->evg[hf], s));
+            HIPC(hipStreamWaitEvent(sr, b->evg[hf], 0));
+        }
         ReplayArgs R{};
         R.rt.n_sigma = b->args.n_sigma;
         R.sig = b->d_sig;
         R.sigf = b->d_sigf;
         R.n_rays = int(n);
-        R.pool = b->d_pool;
-        R.next = b->d_next;
-        R.n_st = b->d_nst;
+        R.pool = pool;
+        R.next = next;
+        R.n_st = nst;
         R.rts = b->d_rts;
         R.cams = b->d_cams;
         R.pix0 = b->d_pix0;
@@ -1688,8 +1725,11 @@ int batch_launch_split(hadros_batch* b, hipStream_t s, long ray_limit)
         R.btau = b->d_bt;
         R.bp = b->d_bp;
         R.bi = b->d_bi;
-        HIPC(launch_replay(R, b->precision, s));
+        HIPC(launch_replay(R, b->precision, sr));
+        if (two) HIPC(hipEventRecord(b->evr[hf], sr));
+        last_half = hf;
     }
+    if (two) HIPC(hipStreamWaitEvent(s, b->evr[last_half], 0));   // the run ends when its last replay does
     HIPC(hipEventRecord(b->ev1, s));
     b->last_limit = ray_limit;
     return HADROS_OK;
@@ -1714,13 +1754,13 @@ int batch_settle(hadros_batch* b)
         if (need <= b->capacity) return HADROS_OK;
         if (attempt == 3) break;
         long cap = long(double(need) * 1.25) + 1, sub = b->sub;
-        const double bound = batch_pool_bound(b);
+        const double bound = batch_pool_bound(b) / double(b->halves);   // per half
         if (double(cap) * CHUNK_BYTES > bound) {
             sub = std::max(1L, long(double(sub) * bound / (double(cap) * CHUNK_BYTES)));
             cap = std::max(sub + 1, long(bound / CHUNK_BYTES));
         }
         int rc = 0;
-        if ((rc = batch_pool(b, sub, cap)) || (rc = batch_launch_split(b, nullptr, b->last_limit))) return rc;
+        if ((rc = batch_pool(b, sub, cap, b->halves)) || (rc = batch_launch_split(b, nullptr, b->last_limit))) return rc;
         HIPC(hipEventSynchronize(b->ev1));
     }
     return fail(HADROS_ERR_MAX_POINTS, "split batch: chunk pool too small");
@@ -1953,6 +1993,11 @@ int hadros_batch_destroy(hadros_batch* b)
     dfree(b->d_ctr);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (int k = 0; k < 2; ++k) {
+        if (b->evg[k]) (void)hipEventDestroy(b->evg[k]);
+        if (b->evr[k]) (void)hipEventDestroy(b->evr[k]);
+    }
+    if (b->s2) (void)hipStreamDestroy(b->s2);
     delete b;
     return HADROS_OK;
 }

@@ -8,13 +8,30 @@ of the powerlaw_cutoff spectrum (1e5-1e12 GeV).
 Training rays come from the batched GPU tracer (FP64 geodesics, CFG
 integrator) on cameras of the 512x512 workload's medium/source/spectrum
 conventions with sampled spin, inclination, profile, rho0 and slope; the
-network is trained by the library's own BF16-MFMA forward/backward kernels.
+networks are trained by the library's own BF16-MFMA forward/backward kernels.
+Two parameter families are served:
 
-The scan renders parameter points with the network, traces a fixed 1 %
-pixel sub-lattice of every image exactly, and falls back to the full plain
-tracer for every image whose spot check disagrees by more than the survival
-tolerance.  The exact tracer is thus both the fallback and the judge of the
-network's error.
+- "opaque" (the benchmark's configuration): all 8 profiles, rho0 1e8-1e12
+  g/cm^3, where the medium blocks almost every ray;
+- "translucent": the powerlaw_funnel_envelope profile at rho0 1-1e3 g/cm^3,
+  where survival is spread over (0, 1).
+
+The scan renders parameter points with an ensemble of independently seeded
+networks, traces a fixed 1 % pixel sub-lattice of every image exactly, and
+falls back to the full plain tracer for every image that fails its rule:
+
+- rule "pixel": any spot pixel with |P_served - P_traced| > tol, or any pixel
+  of the check lattice (a quarter of the pixels) where another network of
+  the ensemble differs from the served one by more than tol;
+- rule "metric" (the per-image survival a robustness scan reports, the mean
+  over pixels of the bin-weighted survival of src/radiative_transfer.cpp:
+  763-769): the network mean corrected by the spot pixels' mean residual (a
+  control variate), accepted when three standard errors of that correction
+  and the ensemble's disagreement on the check-lattice mean are both within
+  tol times the estimate.
+
+The exact tracer is thus both the fallback and the judge of the networks'
+error.
 """
 import math
 import time
@@ -29,22 +46,30 @@ N_OUT = 2 * N_BINS
 LOG_TAU_FLOOR = -12.0
 
 SLOPE_RANGE = (1.0, 3.0)     # radial power-law slope of the disk
-RHO_RANGE = (8.0, 12.0)      # log10 rho0 [g/cm^3]
+RHO_RANGE = (8.0, 12.0)      # log10 rho0 [g/cm^3], the opaque family
+TRANSLUCENT_RHO_RANGE = (0.0, 3.0)
+TRANSLUCENT_PROFILE = "powerlaw_funnel_envelope"
 INC_RANGE = (5.0, 85.0)      # inclination [deg]
 SPIN_RANGE = (0.0, 0.998)
 HALF_WIDTH = 20.0            # camera half-width [r_g] at r_obs = 1000
 FOV_JITTER = (0.9, 1.1)      # training cameras: half-width scale, so pixel centres cover the plane densely
 
 
-def sample_params(rng, n):
-    return {
+def sample_params(rng, n, family="opaque"):
+    """Camera and medium parameters of n training cameras of a family."""
+    if family == "translucent":
+        prof, rho = np.full(n, abi.PROFILE_KINDS[TRANSLUCENT_PROFILE]), TRANSLUCENT_RHO_RANGE
+    else:
+        prof, rho = None, RHO_RANGE
+    out = {
         "spin": rng.uniform(*SPIN_RANGE, n),
         "inc": rng.uniform(*INC_RANGE, n),
-        "profile": rng.integers(0, 8, n),
-        "log_rho0": rng.uniform(*RHO_RANGE, n),
+        "profile": rng.integers(0, 8, n) if prof is None else prof,
+        "log_rho0": rng.uniform(*rho, n),
         "slope": rng.uniform(*SLOPE_RANGE, n),
         "fov_scale": rng.uniform(*FOV_JITTER, n),
     }
+    return out
 
 
 def fov_deg(scale=1.0):
@@ -131,10 +156,10 @@ def trace_params(params, res, stride=1, offset=0, precision=abi.PREC_FP64, tunin
         b.close()
 
 
-def generate(n_cams, res, seed, precision=abi.PREC_FP64):
+def generate(n_cams, res, seed, precision=abi.PREC_FP64, family="opaque"):
     """Training / test rays: (X [N x 14], Y [N x 16], trace ms)."""
     rng = np.random.default_rng(seed)
-    params = sample_params(rng, n_cams)
+    params = sample_params(rng, n_cams, family)
     img, ms = trace_params(params, res, precision=precision)
     return features(params, res), targets(img.bin_tau, img.bin_p), ms
 
@@ -226,37 +251,91 @@ def evaluate(sur, X, Y):
     }
 
 
-def sobol_points(n=4096, profile="powerlaw_funnel_envelope"):
+def sobol_points(n=4096, profile="powerlaw_funnel_envelope", family="opaque"):
     """First n points of the unscrambled 4-D Sobol sequence mapped to spin,
     inclination, log10 rho0 and slope, all with one density profile."""
+    import warnings
+
     from scipy.stats import qmc
 
-    u = qmc.Sobol(d=4, scramble=False).random(n)
+    with warnings.catch_warnings():   # prefixes of a power-of-two length are balanced; others are still the sequence
+        warnings.simplefilter("ignore", UserWarning)
+        u = qmc.Sobol(d=4, scramble=False).random(n)
+    rho = TRANSLUCENT_RHO_RANGE if family == "translucent" else RHO_RANGE
     return {
         "spin": SPIN_RANGE[0] + (SPIN_RANGE[1] - SPIN_RANGE[0]) * u[:, 0],
         "inc": INC_RANGE[0] + (INC_RANGE[1] - INC_RANGE[0]) * u[:, 1],
         "profile": np.full(n, abi.PROFILE_KINDS[profile]),
-        "log_rho0": RHO_RANGE[0] + (RHO_RANGE[1] - RHO_RANGE[0]) * u[:, 2],
+        "log_rho0": rho[0] + (rho[1] - rho[0]) * u[:, 2],
         "slope": SLOPE_RANGE[0] + (SLOPE_RANGE[1] - SLOPE_RANGE[0]) * u[:, 3],
     }
+
+
+def bin_weights():
+    """Normalised weights w_b / sum w of the 8 powerlaw_cutoff bins of the
+    scenes (integrate_kerr_ray_spectral's bins, src/radiative_transfer.cpp:
+    698-761, on the synthetic table's range): the aggregate survival of a
+    pixel is sum_b w_b P_b (:763-769)."""
+    e, _, _ = workloads.synthetic_sigma()
+    sc = workloads.c2_scene().spectrum
+    lo, hi = max(sc.e_min_GeV, e[0]), min(sc.e_max_GeV, e[-1])
+    ll, lh = math.log(lo), math.log(hi)
+    d = (lh - ll) / N_BINS
+    w = []
+    for i in range(N_BINS):
+        a, b = ll + d * i, ll + d * i + d
+        em = math.exp(0.5 * (a + b))
+        w.append(em ** (-sc.gamma) * math.exp(-em / sc.ecut_GeV) * (math.exp(b) - math.exp(a)))
+    w = np.array(w)
+    return w / w.sum()
+
+
+def train_ensemble(X, Y, k=2, epochs=8, seed=0, **kw):
+    """k networks on the same rays, each with its own initialisation and
+    minibatch order (seeds seed, seed + 1, ...)."""
+    nets = []
+    for i in range(k):
+        sur = Surrogate(seed=seed + i, **kw)
+        sur.fit(X, Y, epochs=epochs, seed=seed + i)
+        nets.append(sur)
+    return nets
 
 
 def subset(params, idx):
     return {k: v[idx] for k, v in params.items()}
 
 
+def _aggregate(p, wb):
+    """Bin-weighted survival sum_b w_b P_b over the last axis (FP64; an
+    elementwise product and sum, not a BLAS matrix-vector product, whose
+    launch grid is bounded by the row count)."""
+    return (p.double() * wb).sum(dim=-1)
+
+
+def _models(sur):
+    return list(sur) if isinstance(sur, (list, tuple)) else [sur]
+
+
 def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=256, tuning=None,
-         fallback_chunk=512):
-    """Render every parameter point at res x res x 8 bins with the network,
-    spot-check the sub-lattice i, j = offset + stride k of each image (26 x 26
-    = 676 pixels = 1.03 % at 256 x 256) against the exact batched tracer, and
-    re-trace every image whose spot check has any |dP| > tol.  Everything
-    stays on the GPU.  Returns (P images on the GPU [n, res*res, 8] float32,
-    report)."""
+         fallback_chunk=2048, rule="pixel", check_stride=2):
+    """Render every parameter point at res x res x 8 bins with the network
+    ensemble `sur` (one Surrogate or a list; the served image is the first
+    member's, the others are evaluated on the check lattice of every
+    check_stride-th row and column, a quarter of the pixels, and compared
+    with it there), trace the sub-lattice i, j = offset + stride k of each
+    image (26 x 26 = 676 pixels = 1.03 % at 256 x 256) exactly, and re-trace
+    every image that fails `rule` (module docstring) in full.  Everything
+    stays on the GPU.  Returns (P images on the GPU [n, res*res, 8] float32, report); the
+    report's "metric" holds each image's served mean survival (exact for
+    fallback images) and, under rule "metric", its standard error."""
     import torch
 
+    if rule not in ("pixel", "metric"):
+        raise ValueError(f"unknown scan rule {rule}")
+    models = _models(sur)
     n = len(params["spin"])
     npix = res * res
+    wb = torch.from_numpy(bin_weights()).to("cuda", torch.float64)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     # 1. exact spot checks: one batched launch over every image's sub-lattice
@@ -273,34 +352,75 @@ def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=25
     spot = spot.view(n, n_spot, N_BINS)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    # 2. network over every pixel of every image
+    # 2. every network over every pixel of every image; 3. the rule, per image
+    sp_idx = np.arange(spot_offset, res, spot_stride)
+    pix = torch.from_numpy((sp_idx[:, None] * res + sp_idx[None, :]).ravel()).cuda()
     P_img = torch.empty((n, npix, N_BINS), dtype=torch.float32, device="cuda")
     al, be = pixel_coords(res)
     base = torch.from_numpy(np.stack([al, be], 1).astype(np.float32)).cuda()
     prm = torch.from_numpy(features(params, 1)[:, 2:].astype(np.float32)).cuda()   # per-image features
     X = torch.empty((group, npix, N_FEAT), dtype=torch.float32, device="cuda")
-    Y = torch.empty((group * npix, N_OUT), dtype=torch.float32, device="cuda") if sur.head == "p" else None
+    # further members run on the check lattice (every check_stride-th row and
+    # column), where each is compared with the first member
+    ck = np.arange(0, res, check_stride)
+    cpix = torch.from_numpy((ck[:, None] * res + ck[None, :]).ravel()).cuda()
+    ncheck = int(cpix.numel())
+    Xc = torch.empty((group, ncheck, N_FEAT), dtype=torch.float32, device="cuda") if len(models) > 1 else None
+    P_k = torch.empty((group, ncheck, N_BINS), dtype=torch.float32, device="cuda") if len(models) > 1 else None
+    Y = torch.empty((group * npix, N_OUT), dtype=torch.float32, device="cuda") \
+        if any(m.head != "tau" for m in models) else None
+    fail = torch.zeros(n, dtype=torch.bool, device="cuda")
+    spot_err = torch.zeros(n, dtype=torch.float64, device="cuda")
+    disagree = torch.zeros(n, dtype=torch.float64, device="cuda")
+    metric = torch.zeros(n, dtype=torch.float64, device="cuda")
+    metric_se = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    def run_model(m, Xv, dst, rows):
+        if m.head == "tau":   # fused kernel writes P = exp(-10^log10tau) straight into the images
+            m.net.survival(Xv, dst.view(rows, N_BINS))
+        else:
+            Yp = m.net.forward(Xv, Y[:rows])
+            dst.copy_(m.survival_of(Yp).view(dst.shape))
+
     for g0 in range(0, n, group):
         k = min(n, g0 + group) - g0
         X[:k, :, :2] = base[None]
         X[:k, :, 2:] = prm[g0:g0 + k, None, :]
-        Xv = X[:k].view(k * npix, N_FEAT)
-        if sur.head == "tau":   # fused kernel writes P = exp(-10^log10tau) straight into the images
-            sur.net.survival(Xv, P_img[g0:g0 + k].view(k * npix, N_BINS))
+        out = P_img[g0:g0 + k]
+        run_model(models[0], X[:k].view(k * npix, N_FEAT), out, k * npix)
+        if len(models) > 1:
+            Xc[:k] = X[:k, cpix, :]
+            first = out[:, cpix, :]
+            m0 = _aggregate(first, wb).mean(dim=1)
+            for m in models[1:]:
+                dst = P_k[:k]
+                run_model(m, Xc[:k].view(k * ncheck, N_FEAT), dst, k * ncheck)
+                if rule == "pixel":
+                    d = (dst - first).abs().amax(dim=(1, 2)).double()
+                else:
+                    d = (_aggregate(dst, wb).mean(dim=1) - m0).abs()
+                disagree[g0:g0 + k] = torch.maximum(disagree[g0:g0 + k], d)
+        sp_t = spot[g0:g0 + k]
+        sp_n = out[:, pix, :].double()
+        spot_err[g0:g0 + k] = (sp_n - sp_t).abs().amax(dim=(1, 2))
+        # the image's mean served survival from per-bin pixel sums (FP64 accumulation)
+        mean_n = (out.sum(dim=1, dtype=torch.float64) * wb).sum(dim=-1) / npix
+        d = _aggregate(sp_t, wb) - _aggregate(sp_n, wb)    # residuals at the spot pixels
+        if rule == "metric":
+            metric[g0:g0 + k] = mean_n + d.mean(dim=1)
+            metric_se[g0:g0 + k] = d.std(dim=1) / math.sqrt(n_spot)
+            fail[g0:g0 + k] = (3.0 * metric_se[g0:g0 + k] > tol * metric[g0:g0 + k]) | \
+                              (disagree[g0:g0 + k] > tol * metric[g0:g0 + k])
         else:
-            Yp = sur.net.forward(Xv, Y[:k * npix])
-            P_img[g0:g0 + k] = sur.survival_of(Yp).view(k, npix, N_BINS)
+            metric[g0:g0 + k] = mean_n
+            fail[g0:g0 + k] = (spot_err[g0:g0 + k] > tol) | (disagree[g0:g0 + k] > tol)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    # 3. compare at the spot pixels
-    sp_idx = np.arange(spot_offset, res, spot_stride)
-    pix = torch.from_numpy((sp_idx[:, None] * res + sp_idx[None, :]).ravel()).cuda()
-    err = (P_img[:, pix, :].double() - spot).abs().amax(dim=(1, 2))
-    fail = torch.nonzero(err > tol).flatten()
-    fail_h = fail.cpu().numpy()
-    err_h = err.cpu().numpy()
+    fail_idx = torch.nonzero(fail).flatten()
+    fail_h = fail_idx.cpu().numpy()
     t3 = time.perf_counter()
-    # 4. fallback: full plain tracing of the failing images
+    # 4. fallback: full plain tracing of the failing images, in few large
+    # batches (each batch allocates and frees a chunk pool of tens of GB)
     fb_ms = 0.0
     for c0 in range(0, fail_h.size, fallback_chunk):
         idx = fail_h[c0:c0 + fallback_chunk]
@@ -312,16 +432,22 @@ def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=25
             fb_ms += b.last_ms()
         finally:
             b.close()
-        P_img[fail[c0:c0 + fallback_chunk]] = full.view(idx.size, npix, N_BINS).float()
+        full = full.view(idx.size, npix, N_BINS)
+        sel = fail_idx[c0:c0 + fallback_chunk]
+        P_img[sel] = full.float()
+        metric[sel] = _aggregate(full, wb).mean(dim=1)
+        metric_se[sel] = 0.0
     torch.cuda.synchronize()
     t4 = time.perf_counter()
-    ok = err_h <= tol
+    ok = ~fail.cpu().numpy()
+    err_h = spot_err.cpu().numpy()
     return P_img, {
-        "images": n, "res": res, "bins": N_BINS, "spot_rays_per_image": int(n_spot),
-        "spot_fraction": n_spot / npix, "tolerance": tol, "head": sur.head,
-        "fallback_images": int(fail_h.size), "fallback_rate": fail_h.size / n,
+        "images": n, "res": res, "bins": N_BINS, "rule": rule, "ensemble": len(models),
+        "spot_rays_per_image": int(n_spot), "spot_fraction": n_spot / npix, "tolerance": tol,
+        "head": models[0].head, "fallback_images": int(fail_h.size), "fallback_rate": fail_h.size / n,
         "max_spot_err_accepted": float(err_h[ok].max()) if ok.any() else None,
         "median_spot_err": float(np.median(err_h)),
+        "max_disagreement_accepted": float(disagree.cpu().numpy()[ok].max()) if ok.any() and len(models) > 1 else None,
         "seconds": {"spot_trace": t1 - t0, "network": t2 - t1, "check": t3 - t2, "fallback": t4 - t3,
                     "total": t4 - t0},
         "kernel_ms": {"spot_trace": spot_ms, "fallback_trace": fb_ms},
@@ -329,26 +455,36 @@ def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=25
         "images_per_s": n / (t4 - t0),
         "pixels_per_s": n * npix / (t4 - t0),
         "failed": fail_h,
+        "metric": metric.cpu().numpy(),
+        "metric_se": metric_se.cpu().numpy(),
     }
 
 
-def traced_scan(params, res=256, chunk=1024, tuning=None, keep=()):
+def traced_scan(params, res=256, chunk=2048, tuning=None, keep=()):
     """All-traced scan: the batched tracer over every pixel of every image, in
     chunks of `chunk` images.  Returns (wall seconds, kernel ms, {index: P
-    image [res*res, 8]} for the indices in `keep`)."""
+    image [res*res, 8]} for the indices in `keep`, each image's mean
+    aggregate survival)."""
+    import torch
+
     t0 = time.perf_counter()
     n = len(params["spin"])
+    npix = res * res
     total_ms = 0.0
     kept = {}
+    metric = np.zeros(n)
     for c0 in range(0, n, chunk):
         idx = np.arange(c0, min(n, c0 + chunk))
         b = make_batch(subset(params, idx), res, tuning=tuning)
         try:
             b.run()
             total_ms += b.last_ms()
+            agg = torch.empty(idx.size * npix, dtype=torch.float64, device="cuda")
+            b.fetch_all_device(p_surv=agg)
+            metric[idx] = agg.view(idx.size, npix).mean(dim=1).cpu().numpy()
             for q, i in enumerate(idx):
                 if int(i) in keep:
                     kept[int(i)] = b.fetch(q, N_BINS).bin_p.astype(np.float32)
         finally:
             b.close()
-    return time.perf_counter() - t0, total_ms, kept
+    return time.perf_counter() - t0, total_ms, kept, metric
