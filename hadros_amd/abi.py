@@ -30,6 +30,7 @@ PREC_FP32K = 1
 PATH_AUTO, PATH_FUSED, PATH_SPLIT = 0, 1, 2
 
 MAX_BINS = 16
+EVAL_TRANSFER_COLS = 6 + 2 * MAX_BINS   # hadros_eval_transfer_point columns
 MAX_SIGMA = 1024
 NOALLOC_SCRATCH_BYTES = 32768
 FLAG_EXHAUSTED = 1

@@ -124,7 +124,83 @@ __global__ void k_eval_step(double a, int kind, double h, const double* y, doubl
     o[10] = ok ? 1.0 : 0.0;
 }
 
+// The per-point and per-(point, bin) terms of k_replay (the transfer kernel of
+// the split tracer, the cache replay and the batch) at points (r, theta, dl,
+// z), through the device functions it runs: point_q with exp_tab (density,
+// source factor, redshift), the sigma lookup the scene's table selects
+// (exact-uniform or bracketing) with exp_tab, and the emission exponent.  (The
+// fused tracer's library-exp forms are the ones k_eval_rho, k_eval_sigma and
+// k_eval_emis run.)  out[EVAL_TRANSFER_COLS k ..] = ok, z, ln z,
+// g3Adl, nbar_dl, rho, then per bin dtau_b, then per bin the emission term
+// before absorption: FP64 g3Adl e^arg_b, FP32K its log ln(g3Adl) + ln S(E_b)
+// (-inf where the point does not emit).  Bins from nb on are 0.
+template <int PREC>
+__global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* sigf, const double* r,
+                                      const double* th, const double* dl, const double* z, double* out, long n)
+{
+    __shared__ double exp_t[64];
+    if (threadIdx.x < 64) exp_t[threadIdx.x] = EXP_TAB[threadIdx.x];
+    __syncthreads();
+    const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    PointQ pq;
+    const bool ok = point_q(P, r[k], th[k], dl[k], z[k], pq, ExpTab{exp_t});
+    if (!ok) {   // as the kernels: a skipped point adds nothing
+        pq.z = 1.0;
+        pq.lnz = 0.0;
+        pq.g3Adl = 0.0;
+        pq.nbar_dl = 0.0;
+        pq.rho = 0.0;
+    }
+    double* o = out + long(EVAL_TRANSFER_COLS) * k;
+    o[0] = ok ? 1.0 : 0.0;
+    o[1] = pq.z;
+    o[2] = pq.lnz;
+    o[3] = pq.g3Adl;
+    o[4] = pq.nbar_dl;
+    o[5] = pq.rho;
+    const int n_s = P.n_sigma;
+    const bool exact = P.sig_exact != 0;
+    for (int b = 0; b < HADROS_MAX_BINS; ++b) {
+        double dt = 0.0, em = 0.0;
+        if (b < P.nb) {
+            const double E = P.emid[b] * pq.z;
+            if constexpr (PREC == HADROS_PREC_FP64) {
+                const double lnEb = P.lnemid[b] + pq.lnz;
+                dt = bin_dtau64(exact, ok && pq.rho > 0.0, E, lnEb, pq.nbar_dl, sig, sig + n_s, sig + 2 * n_s, n_s,
+                                P.sig_lo, P.sig_inv_d, P.e_lo, P.e_hi, exp_t);
+                if (ok && P.spec_ok && pq.g3Adl != 0.0)
+                    em = pq.g3Adl * exp_tab(bin_emis_arg(P.gam, P.invcut, E, lnEb), exp_t);
+            } else {
+                const float lnEb = float(P.lnemid[b]) + float(pq.lnz);
+                const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
+                const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
+                dt = bin_dtau32(exact, ok && pq.rho > 0.0 && pq.nbar_dl > 0.0, E, lnEb, lN, sigf, sigf + n_s,
+                                sigf + 2 * n_s, n_s, float(P.sig_lo), float(P.sig_inv_d), P.e_lo, P.e_hi);
+                em = (ok && P.spec_ok && pq.g3Adl > 0.0)
+                         ? double(bin_emis_log32(lP, float(P.gam), float(P.invcut), E, lnEb))
+                         : -HUGE_VAL;
+            }
+        }
+        o[6 + b] = dt;
+        o[6 + HADROS_MAX_BINS + b] = em;
+    }
+}
+
 static inline int blocks(long n) { return int((n + 255) / 256); }
+
+hipError_t launch_eval_transfer_point(const DevRt& P, int precision, const double* sig, const float* sigf,
+                                      const double* r, const double* th, const double* dl, const double* z,
+                                      double* out, long n)
+{
+    if (precision == HADROS_PREC_FP64)
+        hipLaunchKernelGGL(k_eval_transfer_point<HADROS_PREC_FP64>, dim3(blocks(n)), dim3(256), 0, 0, P, sig, sigf,
+                           r, th, dl, z, out, n);
+    else
+        hipLaunchKernelGGL(k_eval_transfer_point<HADROS_PREC_FP32K>, dim3(blocks(n)), dim3(256), 0, 0, P, sig, sigf,
+                           r, th, dl, z, out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_eval_step(double a, int kind, double h, const double* y, double* out, long n)
 {

@@ -2242,6 +2242,29 @@ int hadros_eval_step(double a, int kind, double h, const double* y, double* out,
                     [&](std::vector<double*>& d, double* o) { return launch_eval_step(a, kind, h, d[0], o, long(n)); });
 }
 
+int hadros_eval_transfer_point(const hadros_scene* scene, const double* se, const double* ss, int n_sigma,
+                               double a_bh, int precision, const double* r, const double* th, const double* dl,
+                               const double* z, double* out, int64_t n)
+{
+    if (!scene || n < 0) return fail(HADROS_ERR_ARG, "bad argument");
+    if (precision != HADROS_PREC_FP64 && precision != HADROS_PREC_FP32K) return fail(HADROS_ERR_ARG, "unknown precision");
+    DevRt P;
+    int rc = make_rt(*scene, se, ss, n_sigma, a_bh, P);
+    if (rc) return rc;
+    std::vector<double> sd;
+    std::vector<float> sf;
+    sigma_tables(se, ss, n_sigma, sd, sf);
+    // the float table travels in double-sized words
+    std::vector<double> sf_words((sf.size() + 1) / 2, 0.0);
+    std::memcpy(sf_words.data(), sf.data(), sf.size() * sizeof(float));
+    return run_eval({{sd.data(), sd.size()}, {sf_words.data(), sf_words.size()}, {r, size_t(n)}, {th, size_t(n)},
+                     {dl, size_t(n)}, {z, size_t(n)}},
+                    out, size_t(n) * EVAL_TRANSFER_COLS, [&](std::vector<double*>& d, double* o) {
+                        return launch_eval_transfer_point(P, precision, d[0], reinterpret_cast<const float*>(d[1]),
+                                                          d[2], d[3], d[4], d[5], o, long(n));
+                    });
+}
+
 int hadros_eval_rhs(double a, int mode, const double* y, double* dy, int64_t n)
 {
     if (mode != HADROS_MODE_REF && mode != HADROS_MODE_CFG) return fail(HADROS_ERR_ARG, "unknown mode");

@@ -1047,6 +1047,50 @@ constexpr double EXP64_ZERO_BELOW = -750.0;
 // unchanged (zero on their own, below FP32 resolution beside any larger term)
 constexpr float EXP32_LOG_NEGLIGIBLE = -1000.0f;
 
+// The per-(point, bin) terms of the transfer kernels' rolled bin loops, for
+// a point with PointQ q and bin energy E = E_mid z, lnEb = ln E_mid + ln z
+// (FMA contraction allowed, as in the loops).
+//
+// Optical depth increment n_N sigma(E) dl where the point absorbs and E lies
+// inside the table, else 0.  The lookup runs unconditionally (its index stays
+// inside the table) and the mask selects, so the loop body has no branch.
+// FP64: sigma by the table's exact-uniform or bracketing lookup, e^x by
+// exp_tab with T = 2^(j/64).
+__device__ __forceinline__ double bin_dtau64(bool exact, bool absorbs, double E, double lnEb, double nbar_dl,
+                                             const double* lnE, const double* lnS, const double* slope, int n,
+                                             double sig_lo, double sig_inv_d, double e_lo, double e_hi,
+                                             const double* T)
+{
+    HD_CONTRACT
+    const double ls = exact ? sigma_ln_exact(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d)
+                            : sigma_ln(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d);
+    const double sig_dl = nbar_dl * exp_tab_nz(fmax(ls, EXP_ZERO_BELOW), T);
+    return (absorbs && E >= e_lo && E <= e_hi) ? sig_dl : 0.0;
+}
+// FP32 form: e^(ln(n_N dl) + ln sigma), lN = ln(n_N dl)
+__device__ __forceinline__ float bin_dtau32(bool exact, bool absorbs, double E, float lnEb, float lN,
+                                            const float* lnEf, const float* lnSf, const float* slopef, int n,
+                                            float sig_lo, float sig_inv_d, double e_lo, double e_hi)
+{
+    HD_CONTRACT
+    const float ls = exact ? sigma_ln_exact_f(lnEf, lnSf, slopef, n, lnEb, sig_lo, sig_inv_d)
+                           : sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, sig_lo, sig_inv_d);
+    return (absorbs && E >= e_lo && E <= e_hi) ? __expf(lN + ls) : 0.0f;
+}
+// Exponent of the spectral factor S(E) = e^(-gam ln E - E invcut); the FP64
+// emission term is g3Adl e^(arg - tau before the point)
+__device__ __forceinline__ double bin_emis_arg(double gam, double invcut, double E, double lnEb)
+{
+    HD_CONTRACT
+    return -gam * lnEb - E * invcut;
+}
+// FP32 log-form emission term before absorption: ln(g3Adl) + ln S(E)
+__device__ __forceinline__ float bin_emis_log32(float lP, float gam, float invcut, double E, float lnEb)
+{
+    HD_CONTRACT
+    return lP - gam * lnEb - float(E) * invcut;
+}
+
 // FP64 transfer update of every bin for one path point:
 //   I_b   += g^3 j(E_b) e^{-tau_b} dl       (emission uses tau before the point)
 //   tau_b += n_N sigma(E_b) dl              if rho > 0 and E_b inside the table

@@ -197,18 +197,13 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             for (int b = 0; b < nb; ++b) {
                 const double E = bin_e[b] * pq.z;
                 const double lnEb = bin_lne[b] + pq.lnz;
-                // dtau = n_N sigma(E) dl where the point absorbs and E lies in the
-                // table; the lookup runs on every lane (its index stays in the table)
-                // and the mask selects, so the loop body has no branch here
-                const double ls = exact ? sigma_ln_exact(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d)
-                                        : sigma_ln(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d);
-                const double sig_dl = pq.nbar_dl * exp_tab_nz(fmax(ls, EXP_ZERO_BELOW), exp_t);
-                const double dt = (absorbs && E >= e_lo && E <= e_hi) ? sig_dl : 0.0;
+                const double dt = bin_dtau64(exact, absorbs, E, lnEb, pq.nbar_dl, lnE, lnS, slope, n, sig_lo,
+                                             sig_inv_d, e_lo, e_hi, exp_t);
                 double inc[1] = {dt};
                 wave_scan_add(inc);
                 const double tau_b = a64.tau[b];
                 const double before = tau_b + (inc[0] - dt);
-                const double arg = -gam * lnEb - E * invcut - before;
+                const double arg = bin_emis_arg(gam, invcut, E, lnEb) - before;
                 // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
                 // every term is zero (behind an opaque column) adds nothing
                 const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
@@ -230,16 +225,14 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             for (int b = 0; b < nb; ++b) {
                 const double E = bin_e[b] * pq.z;
                 const float lnEb = float(bin_lne[b]) + lnz;
-                // as in the FP64 loop: the lookup on every lane, the mask selects
-                const float ls = exact ? sigma_ln_exact_f(lnEf, lnSf, slopef, n, lnEb, sig_lo, sig_inv_d)
-                                       : sigma_ln_f(lnEf, lnSf, slopef, n, lnEb, sig_lo, sig_inv_d);
-                const float dt = (absorbs && E >= e_lo && E <= e_hi) ? __expf(lN + ls) : 0.0f;
+                const float dt = bin_dtau32(exact, absorbs, E, lnEb, lN, lnEf, lnSf, slopef, n, sig_lo, sig_inv_d,
+                                            e_lo, e_hi);
                 float inc[1] = {dt};
                 wave_scan_add(inc);
                 const float before = a32.tau[b].s + (inc[0] - dt);
                 // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
                 float L = -INFINITY;
-                if (ok && P.spec_ok && pq.g3Adl > 0.0) L = lP - gam * lnEb - float(E) * invcut - before;
+                if (ok && P.spec_ok && pq.g3Adl > 0.0) L = bin_emis_log32(lP, gam, invcut, E, lnEb) - before;
                 // terms below e^EXP32_LOG_NEGLIGIBLE vanish in the FP64 result or
                 // beside any larger term: a chunk of only such terms is skipped
                 const bool live = __ballot(!(L < EXP32_LOG_NEGLIGIBLE)) != 0ull;

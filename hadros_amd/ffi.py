@@ -32,7 +32,7 @@ EXPORTS = [
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric", "hadros_eval_mev",
-    "hadros_eval_step", "hadros_tau_along_ray",
+    "hadros_eval_step", "hadros_tau_along_ray", "hadros_eval_transfer_point",
     "hadros_tracer_set_mev", "hadros_tracer_fetch_mev", "hadros_replay_set_mev", "hadros_replay_fetch_mev",
     "hadros_mev_defaults", "hadros_parse_mev_model", "hadros_parse_mev_flavor", "hadros_parse_mev_thermal",
     "hadros_parse_mev_ye", "hadros_parse_mev_spectral",
@@ -150,6 +150,8 @@ def lib():
     L.hadros_tau_along_ray.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double,
                                        _dp, _dp]
     L.hadros_eval_metric.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int64]
+    L.hadros_eval_transfer_point.argtypes = [C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_double, C.c_int, _dp, _dp,
+                                             _dp, _dp, _dp, C.c_int64]
     _lib = L
     return L
 
@@ -691,6 +693,21 @@ def eval_rhs(a, mode, y):
 
 
 STEP_RK4, STEP_RKF45_REF, STEP_RKF45_CFG, STEP_RKF45_CFG_SHIFT = 0, 1, 2, 3
+
+
+def eval_transfer_point(scene, sig_e, sig_s, a_bh, precision, r, th, dl, z):
+    """The transfer kernel's terms at points (r, theta, dl, redshift)
+    (hadros_eval_transfer_point): a dict of ok, z, lnz, g3Adl, nbar_dl, rho
+    [n] and dtau, emis [n x MAX_BINS] (emis is the log-form term for FP32K)."""
+    r, th, dl, z = (np.ascontiguousarray(x, np.float64) for x in (r, th, dl, z))
+    sig_e = np.ascontiguousarray(sig_e, np.float64)
+    sig_s = np.ascontiguousarray(sig_s, np.float64)
+    out = np.zeros((r.size, abi.EVAL_TRANSFER_COLS))
+    check(lib().hadros_eval_transfer_point(C.byref(scene), dptr(sig_e), dptr(sig_s), sig_e.size, a_bh, precision,
+                                           dptr(r), dptr(th), dptr(dl), dptr(z), dptr(out), r.size))
+    nb = abi.MAX_BINS
+    return {"ok": out[:, 0] != 0.0, "z": out[:, 1], "lnz": out[:, 2], "g3Adl": out[:, 3], "nbar_dl": out[:, 4],
+            "rho": out[:, 5], "dtau": out[:, 6:6 + nb], "emis": out[:, 6 + nb:6 + 2 * nb]}
 
 
 def eval_step(a, kind, h, y):

@@ -583,6 +583,19 @@ int hadros_eval_rhs(double a_spin, int mode, const double* y, double* dy, int64_
    when a stage's theta offset leaves that form's range; the kernels then
    redo the attempt as kind 2). */
 int hadros_eval_step(double a_spin, int kind, double h, const double* y, double* out, int64_t n);
+/* The per-point and per-(point, bin) terms of integrate_kerr_ray_spectral
+   (src/radiative_transfer.cpp:640-685) as the transfer kernel of the split
+   tracer, the cache replay and the batch computes them, at n points
+   (r, theta, dl, redshift) of a ray of spin a_bh: out is n x
+   HADROS_EVAL_TRANSFER_COLS = ok (0 inside r_h + 1e-3), z, ln z, g^3 A dl_cm,
+   n_N dl_cm, rho, then per bin dtau_b = n_N sigma(E_b) dl_cm (0 where rho <= 0
+   or E_b = E_mid,b z is outside the table), then per bin the emission term
+   before absorption g^3 j(E_b) dl_cm -- for HADROS_PREC_FP32K its natural
+   log, -inf where the point does not emit.  Bins past the scene's count are 0. */
+#define HADROS_EVAL_TRANSFER_COLS (6 + 2 * HADROS_MAX_BINS)
+int hadros_eval_transfer_point(const hadros_scene* scene, const double* sigma_e_gev, const double* sigma_cm2,
+                               int n_sigma, double a_bh, int precision, const double* r, const double* th,
+                               const double* dl, const double* redshift, double* out, int64_t n);
 /* KerrMetric::metric / inverse_metric (src/kerr_metric.cpp:34-116):
    5 distinct components each (tt, tphi, rr, thth, phph) per point, n x 10. */
 int hadros_eval_metric(double a_spin, const double* r, const double* th,
