@@ -68,14 +68,15 @@ RIDGE_FLOP_PER_BYTE = PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
 
 
 def kernel_roofline(name, ms, flop, nbytes):
-    """The roofline of one kernel: bound by FP64 issue when its algorithmic
-    intensity (FLOP per HBM byte) is above the MI355X ridge, else by HBM."""
+    """The roofline of one kernel: bound "fp64" (FP64 VALU issue; the MI355X
+    vector FP64 peak equals its matrix FP64 peak, 78.6 TFLOP/s) when its
+    algorithmic intensity (FLOP per HBM byte) is above the ridge, else "hbm"."""
     sec = ms * 1e-3
     intensity = flop / nbytes
     out = {"kernel": name, "ms": float(ms), "flop": flop, "algorithmic_bytes": nbytes,
            "intensity_flop_per_byte": intensity, "tflops": flop / sec / 1e12, "gbs": nbytes / sec / 1e9}
     if intensity >= RIDGE_FLOP_PER_BYTE:
-        out.update(bound="mfma", achieved=out["tflops"], peak=PEAK_FP64_TFLOPS, unit="TFLOP/s")
+        out.update(bound="fp64", achieved=out["tflops"], peak=PEAK_FP64_TFLOPS, unit="TFLOP/s")
     else:
         out.update(bound="hbm", achieved=out["gbs"], peak=PEAK_HBM_GBS, unit="GB/s")
     out["frac"] = out["achieved"] / out["peak"]
@@ -98,6 +99,20 @@ def image_errors(img, ref):
             "max_rel_P": float(rp.max()), "rel_sum_I": abs(si - sr) / abs(sr) if sr != 0 else abs(si),
             "mean_P": float(ref.p.mean()), "sum_I": sr,
             "captured_mismatch": int(np.sum(img.captured != ref.captured))}
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's."""
+    import platform
+
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0, gpu_images=None, opacity=False):
@@ -131,6 +146,7 @@ def cpu_baseline(cam_fn, scene_fn, nx, min_seconds=10.0, gpu_images=None, opacit
             py_oracle.trace_image(cam, sc, e, s, nthreads=threads, k0=k0, k1=k1)
         dt = time.perf_counter() - t0
     out = {"value": reps * (k1 - k0) / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
            "sample": f"{reps} x {k1 - k0} rays = image rows [{i0},{i0 + rows}) of the {nx}x{nx} camera, "
                      f"FP64 oracle (hadros_oracle, CFG mode), {dt:.1f} s"}
     if opacity:
@@ -367,8 +383,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-target", nargs="?", const="tracer", choices=["tracer", "replay", "batch"],
                     help="for rocprofv3 passes: run the plain tracer (or the cache replay, or the 256-camera batch "
-                         "at block 128 and 64 steps per refill, without the autotuner) twice, then copy a 1 GiB "
-                         "device buffer twice (calibration of FETCH_SIZE / WRITE_SIZE), print nothing else")
+                         "at block 128 and 64 steps per refill, without the autotuner) twice, then fill and read a "
+                         "1 GiB device buffer twice each with the library's streaming kernels (calibration of "
+                         "WRITE_SIZE / FETCH_SIZE), print nothing else")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the per-ray work has no "
@@ -400,17 +417,21 @@ def main():
             t.run()
             t.last_ms()
         t.close()
-        # calibration copy of 1 GiB: larger than the 256 MB last-level cache,
-        # so its reads come from HBM
+        # calibration: the library's streaming kernels over 1 GiB (larger than
+        # the 256 MB last-level cache, so every byte crosses HBM): hd::k_stream_fill
+        # writes 1048576 KiB, hd::k_stream_read reads 1048576 KiB
         x = torch.empty(128 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        y = torch.ones_like(x)
+        part = torch.empty(abi.STREAM_GRID, dtype=torch.float64, device="cuda")
         for _ in range(2):
-            x.copy_(y)
+            ffi.stream_probe(1, x, value=1.0)
+        for _ in range(2):
+            ffi.stream_probe(0, x, part)
         torch.cuda.synchronize()
+        assert float(part.sum()) == float(x.numel())
         return
 
-    def timed(precision):
-        t = ffi.Tracer(cam, sc, e, s, precision, args.block)
+    def timed(precision, scene=sc):
+        t = ffi.Tracer(cam, scene, e, s, precision, args.block)
         for _ in range(args.warmup):
             t.run()
             t.last_ms()
@@ -448,6 +469,8 @@ def main():
         dom = kernel_roofline("hd::k_trace_rt<CFG, FP64, 8>", kavg, flops,
                               geodesic_bytes(st.total_points, npix))
     dom_name = dom["kernel"]
+    # the whole plain path (both kernels, or the fused one) under the same FLOP count
+    whole_flop = tracer_flops(st.total_attempts, st.total_points, st.n_bins)
 
     wall32, kern32, _, img32, st32 = timed(abi.PREC_FP32K)
     # the same camera through a medium 1e-8 as dense (rho0 100 g/cm^3): the
@@ -456,8 +479,16 @@ def main():
     thin.medium.rho0 *= 1e-8
     thin.medium.envelope_rho0 *= 1e-8
     thin_imgs = {}
-    for name, prec in (("fp64", abi.PREC_FP64), ("fp32k", abi.PREC_FP32K)):
-        thin_imgs[name], _ = ffi.trace_rt(cam, thin, e, s, prec)
+    wall_t, kern_t, (geo_t, rt_t), thin_imgs["fp64"], st_t = timed(abi.PREC_FP64, thin)
+    thin_imgs["fp32k"], _ = ffi.trace_rt(cam, thin, e, s, abi.PREC_FP32K)
+    thin_leg = {"value": args.steps * npix / wall_t, "unit": "rays/s", "kernel_ms": float(np.mean(kern_t)),
+                "geodesic_ms": float(geo_t), "transfer_ms": float(rt_t),
+                "mean_P": float(thin_imgs["fp64"].p.mean()), "sum_I": float(thin_imgs["fp64"].I.sum()),
+                "points_per_ray": st_t.total_points / npix}
+    if rt_t > 0:
+        thin_leg["transfer"] = kernel_roofline("hd::k_replay<FP64, chunks>", rt_t,
+                                               transfer_flops(st_t.total_points, st_t.n_bins),
+                                               transfer_bytes(st_t.total_points, npix, st_t.n_bins))
     m = (img.tau > 0) & (img.tau < 700)
     err_tau = float(np.max(np.abs(img32.tau[m] - img.tau[m]) / img.tau[m])) if m.any() else 0.0
     err_p = float(np.max(np.abs(img32.p - img.p)))
@@ -480,9 +511,15 @@ def main():
         "config": {"workload": f"plain FP64 tracer, {cam.nx}x{cam.ny} camera, spin 0.99, incl 80 deg, "
                                "powerlaw_funnel_envelope, powerlaw_cutoff 8 bins, funnel_wall, CFG mode rtol 1e-8",
                    "rays_per_step": npix, "device": dev_name, "block": st.block_size},
-        "roofline": {"bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"], "unit": dom["unit"],
+        # the compute roof of the one-line contract is "mfma"; the kernel's is FP64 VALU issue
+        "roofline": {"bound": "mfma" if dom["bound"] == "fp64" else dom["bound"], "achieved": dom["achieved"],
+                     "peak": dom["peak"], "unit": dom["unit"],
                      "frac": dom["frac"], "traffic": None, "kernel": dom_name, "kernel_ms": dom["ms"],
-                     "peak_kind": "fp64 (vector = matrix peak)" if dom["bound"] == "mfma" else "hbm3e",
+                     "kernel_bound": dom["bound"],
+                     "peak_kind": "fp64 VALU issue (no MFMA; MI355X vector FP64 peak = matrix FP64 peak, "
+                                  "78.6 TFLOP/s)" if dom["bound"] == "fp64" else "hbm3e",
+                     "whole_path": {"ms": kavg, "flop": whole_flop, "tflops": whole_flop / (kavg * 1e-3) / 1e12,
+                                    "frac_fp64": whole_flop / (kavg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
                      "ridge_flop_per_byte": RIDGE_FLOP_PER_BYTE, "tracer_ms": kavg,
                      "algorithmic_flop_per_launch": dom["flop"], "algorithmic_bytes_per_launch": dom["algorithmic_bytes"],
                      "points_per_launch": st.total_points, "attempts_per_launch": st.total_attempts,
@@ -492,6 +529,9 @@ def main():
                            "max_abs_p_vs_fp64": err_p, "rel_sum_i_vs_fp64": err_i},
         "image": {"captured": st.n_captured, "exhausted": st.n_exhausted, "mean_P": float(img.p.mean()),
                   "sum_I": si, "points_per_ray": st.total_points / npix},
+        # the same camera and path through the medium 1e-8 as dense: survival
+        # and intensity are non-zero, so no emission term is skipped
+        "thin_medium": thin_leg,
     }
     if not args.no_batch:
         out["batched"] = batched_leg(e, s)
@@ -514,6 +554,16 @@ def main():
                 if k["kernel"] in per:
                     k["traffic"] = per[k["kernel"]]["bytes_per_launch"]
             out["roofline"]["traffic_source"] = t["source"]
+            # hardware counters of the same launches (rocprofv3 --pmc): FP64 FLOPs
+            # executed, VALU issue and active lanes, priced over this run's kernel times
+            for k in out["roofline"]["kernels"].values():
+                c = t.get("valu", {}).get(k["kernel"])
+                if c:
+                    k["hw_fp64_flop"] = c["fp64_flop"]
+                    k["hw_tflops"] = c["fp64_flop"] / (k["ms"] * 1e-3) / 1e12
+                    k["hw_frac_fp64"] = k["hw_tflops"] / PEAK_FP64_TFLOPS
+                    k["valu_issue"] = c["valu_issue"]
+                    k["active_lanes"] = c["active_lanes"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(workloads.c2_camera, workloads.c2_scene, args.size, args.cpu_seconds,
                                            {"workload": (sc, {"fp64": img, "fp32k": img32}),

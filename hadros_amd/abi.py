@@ -31,6 +31,7 @@ PATH_AUTO, PATH_FUSED, PATH_SPLIT = 0, 1, 2
 
 MAX_BINS = 16
 EVAL_TRANSFER_COLS = 6 + 2 * MAX_BINS   # hadros_eval_transfer_point columns
+STREAM_GRID = 2048                       # hadros_stream_probe partial sums
 MAX_SIGMA = 1024
 NOALLOC_SCRATCH_BYTES = 32768
 FLAG_EXHAUSTED = 1

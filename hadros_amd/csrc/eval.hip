@@ -187,6 +187,49 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
     }
 }
 
+// Streaming kernels of known HBM traffic, to scale a profiler's byte counters:
+// k_stream_read reads x[0, n) once (16-byte loads, grid-stride) and writes one
+// partial sum per workgroup; k_stream_fill writes v to x[0, n).
+constexpr int STREAM_BLOCK = 256;
+
+__global__ void __launch_bounds__(STREAM_BLOCK) k_stream_read(const double* __restrict__ x, long n,
+                                                              double* __restrict__ partial)
+{
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    const long stride = long(gridDim.x) * STREAM_BLOCK;
+    double s = 0.0;
+    for (long i = long(blockIdx.x) * STREAM_BLOCK + threadIdx.x; i < n / 2; i += stride) {
+        const double2 v = x2[i];
+        s += v.x + v.y;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) s += x[n - 1];
+    __shared__ double red[STREAM_BLOCK];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = STREAM_BLOCK / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(STREAM_BLOCK) k_stream_fill(double* __restrict__ x, long n, double v)
+{
+    double2* x2 = reinterpret_cast<double2*>(x);
+    const long stride = long(gridDim.x) * STREAM_BLOCK;
+    for (long i = long(blockIdx.x) * STREAM_BLOCK + threadIdx.x; i < n / 2; i += stride) x2[i] = double2{v, v};
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) x[n - 1] = v;
+}
+
+hipError_t launch_stream_probe(int kind, double* x, long n, double* partial, double v, int grid, hipStream_t st)
+{
+    if (kind == 0)
+        hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(STREAM_BLOCK), 0, st, x, n, partial);
+    else
+        hipLaunchKernelGGL(k_stream_fill, dim3(grid), dim3(STREAM_BLOCK), 0, st, x, n, v);
+    return hipGetLastError();
+}
+
 static inline int blocks(long n) { return int((n + 255) / 256); }
 
 hipError_t launch_eval_transfer_point(const DevRt& P, int precision, const double* sig, const float* sigf,

@@ -2265,6 +2265,18 @@ int hadros_eval_transfer_point(const hadros_scene* scene, const double* se, cons
                     });
 }
 
+int hadros_stream_probe(int kind, double* buf, int64_t n, double* partial, double value, void* stream)
+{
+    if (kind != 0 && kind != 1) return fail(HADROS_ERR_ARG, "unknown probe kind");
+    if (n < 0 || (!buf && n > 0) || (kind == 0 && !partial)) return fail(HADROS_ERR_ARG, "bad argument");
+    if (reinterpret_cast<uintptr_t>(buf) % 16 != 0) return fail(HADROS_ERR_ARG, "buffer not 16-byte aligned");
+    int rc = ensure_device();
+    if (rc) return rc;
+    const hipError_t e = launch_stream_probe(kind, buf, long(n), partial, value, HADROS_STREAM_GRID,
+                                             static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HADROS_OK : fail(HADROS_ERR_DEVICE, hipGetErrorString(e));
+}
+
 int hadros_eval_rhs(double a, int mode, const double* y, double* dy, int64_t n)
 {
     if (mode != HADROS_MODE_REF && mode != HADROS_MODE_CFG) return fail(HADROS_ERR_ARG, "unknown mode");

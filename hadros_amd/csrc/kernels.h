@@ -240,6 +240,7 @@ hipError_t launch_eval_mev(const DevMedium& med, const DevMev& M, const double* 
 hipError_t launch_eval_metric(double a, const double* r, const double* th, double* out, long n);
 enum { EVAL_STEP_RK4 = 0, EVAL_STEP_RKF45_REF = 1, EVAL_STEP_RKF45_CFG = 2, EVAL_STEP_RKF45_CFG_SHIFT = 3 };
 hipError_t launch_eval_step(double a, int kind, double h, const double* y, double* out, long n);
+hipError_t launch_stream_probe(int kind, double* x, long n, double* partial, double v, int grid, hipStream_t st);
 constexpr int EVAL_TRANSFER_COLS = 6 + 2 * HADROS_MAX_BINS;
 hipError_t launch_eval_transfer_point(const DevRt& P, int precision, const double* sig, const float* sigf,
                                       const double* r, const double* th, const double* dl, const double* z,

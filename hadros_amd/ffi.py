@@ -32,7 +32,7 @@ EXPORTS = [
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
     "hadros_replay_fetch", "hadros_replay_destroy", "hadros_trace_cache", "hadros_trace_cache_range", "hadros_eval_rho",
     "hadros_eval_sigma", "hadros_eval_emissivity", "hadros_eval_rhs", "hadros_eval_metric", "hadros_eval_mev",
-    "hadros_eval_step", "hadros_tau_along_ray", "hadros_eval_transfer_point",
+    "hadros_eval_step", "hadros_tau_along_ray", "hadros_eval_transfer_point", "hadros_stream_probe",
     "hadros_tracer_set_mev", "hadros_tracer_fetch_mev", "hadros_replay_set_mev", "hadros_replay_fetch_mev",
     "hadros_mev_defaults", "hadros_parse_mev_model", "hadros_parse_mev_flavor", "hadros_parse_mev_thermal",
     "hadros_parse_mev_ye", "hadros_parse_mev_spectral",
@@ -150,6 +150,7 @@ def lib():
     L.hadros_tau_along_ray.argtypes = [C.POINTER(abi.CacheView), C.POINTER(abi.Medium), _dp, _dp, C.c_int, C.c_double,
                                        _dp, _dp]
     L.hadros_eval_metric.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int64]
+    L.hadros_stream_probe.argtypes = [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_void_p]
     L.hadros_eval_transfer_point.argtypes = [C.POINTER(abi.Scene), _dp, _dp, C.c_int, C.c_double, C.c_int, _dp, _dp,
                                              _dp, _dp, _dp, C.c_int64]
     _lib = L
@@ -708,6 +709,20 @@ def eval_transfer_point(scene, sig_e, sig_s, a_bh, precision, r, th, dl, z):
     nb = abi.MAX_BINS
     return {"ok": out[:, 0] != 0.0, "z": out[:, 1], "lnz": out[:, 2], "g3Adl": out[:, 3], "nbar_dl": out[:, 4],
             "rho": out[:, 5], "dtau": out[:, 6:6 + nb], "emis": out[:, 6 + nb:6 + 2 * nb]}
+
+
+def stream_probe(kind, buf, partial=None, value=1.0):
+    """Launch a streaming kernel of known traffic on torch's current stream
+    (hadros_stream_probe): kind 0 reads the float64 device tensor buf once
+    into partial (abi.STREAM_GRID doubles), kind 1 fills buf with value."""
+    import torch
+
+    assert buf.dtype == torch.float64 and buf.is_cuda and buf.is_contiguous()
+    pp = None if partial is None else C.c_void_p(partial.data_ptr())
+    if partial is not None:
+        assert partial.dtype == torch.float64 and partial.numel() >= abi.STREAM_GRID
+    check(lib().hadros_stream_probe(kind, C.c_void_p(buf.data_ptr()), buf.numel(), pp, value,
+                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
 
 def eval_step(a, kind, h, y):

@@ -596,6 +596,15 @@ int hadros_eval_step(double a_spin, int kind, double h, const double* y, double*
 int hadros_eval_transfer_point(const hadros_scene* scene, const double* sigma_e_gev, const double* sigma_cm2,
                                int n_sigma, double a_bh, int precision, const double* r, const double* th,
                                const double* dl, const double* redshift, double* out, int64_t n);
+/* Measurement helper; it has no counterpart in the reference.  Streaming
+   kernels of known HBM traffic, for scaling a profiler's per-kernel byte
+   counters (FETCH_SIZE, WRITE_SIZE): kind 0 (hd::k_stream_read) reads the n
+   doubles of buf once and writes HADROS_STREAM_GRID partial sums (their total
+   is the sum of buf) into partial; kind 1 (hd::k_stream_fill) writes value to
+   the n doubles of buf.  Device pointers, buf 16-byte aligned; asynchronous
+   on `stream` (a hipStream_t or NULL). */
+#define HADROS_STREAM_GRID 2048
+int hadros_stream_probe(int kind, double* buf, int64_t n, double* partial, double value, void* stream);
 /* KerrMetric::metric / inverse_metric (src/kerr_metric.cpp:34-116):
    5 distinct components each (tt, tphi, rr, thth, phph) per point, n x 10. */
 int hadros_eval_metric(double a_spin, const double* r, const double* th,

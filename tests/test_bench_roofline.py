@@ -28,7 +28,7 @@ def test_geodesic_kernel_is_fp64_bound(bench):
     nbytes = bench.geodesic_bytes(points, rays)
     assert nbytes == 32.0 * points + 32.0 * rays
     r = bench.kernel_roofline("geo", 2.0, flop, nbytes)
-    assert r["bound"] == "mfma" and r["unit"] == "TFLOP/s" and r["peak"] == 78.6
+    assert r["bound"] == "fp64" and r["unit"] == "TFLOP/s" and r["peak"] == 78.6
     assert r["achieved"] == pytest.approx(flop / 2.0e-3 / 1e12)
     assert r["frac"] == pytest.approx(r["achieved"] / 78.6)
 
