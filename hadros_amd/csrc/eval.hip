@@ -126,8 +126,8 @@ __global__ void k_eval_step(double a, int kind, double h, const double* y, doubl
 
 // The per-point and per-(point, bin) terms of k_replay (the transfer kernel of
 // the split tracer, the cache replay and the batch) at points (r, theta, dl,
-// z), through the device functions it runs: point_q with exp_tab (density,
-// source factor, redshift), the sigma lookup the scene's table selects
+// z), through the device functions it runs: point_q with exp_tab and log_tab
+// (density, source factor, redshift), the sigma lookup the scene's table selects
 // (exact-uniform or bracketing) with exp_tab, and the emission exponent.  (The
 // fused tracer's library-exp forms are the ones k_eval_rho, k_eval_sigma and
 // k_eval_emis run.)  out[EVAL_TRANSFER_COLS k ..] = ok, z, ln z,
@@ -139,12 +139,14 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
                                       const double* th, const double* dl, const double* z, double* out, long n)
 {
     __shared__ double exp_t[64];
+    __shared__ __align__(16) double log_t[2 * LOG_TAB_N];
     if (threadIdx.x < 64) exp_t[threadIdx.x] = EXP_TAB[threadIdx.x];
+    if (threadIdx.x < 2 * LOG_TAB_N) log_t[threadIdx.x] = LOG_TAB[threadIdx.x];
     __syncthreads();
     const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     PointQ pq;
-    const bool ok = point_q(P, r[k], th[k], dl[k], z[k], pq, ExpTab{exp_t});
+    const bool ok = point_q(P, r[k], th[k], dl[k], z[k], pq, ExpTab{exp_t, log_t});
     if (!ok) {   // as the kernels: a skipped point adds nothing
         pq.z = 1.0;
         pq.lnz = 0.0;
@@ -173,8 +175,8 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
                     em = pq.g3Adl * exp_tab(bin_emis_arg(P.gam, P.invcut, E, lnEb), exp_t);
             } else {
                 const float lnEb = float(P.lnemid[b]) + float(pq.lnz);
-                const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
-                const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
+                const float lN = (pq.nbar_dl > 0.0) ? float(log_tab(pq.nbar_dl, log_t)) : 0.0f;
+                const float lP = (pq.g3Adl > 0.0) ? float(log_tab(pq.g3Adl, log_t)) : 0.0f;
                 dt = bin_dtau32(exact, ok && pq.rho > 0.0 && pq.nbar_dl > 0.0, E, lnEb, lN, sigf, sigf + n_s,
                                 sigf + 2 * n_s, n_s, float(P.sig_lo), float(P.sig_inv_d), P.e_lo, P.e_hi);
                 em = (ok && P.spec_ok && pq.g3Adl > 0.0)

@@ -1,6 +1,7 @@
 // fastmath.h -- FP64 elementary functions of the kernels: the transfer
-// kernels' table-driven exponential, the geodesic kernels' sine/cosine and
-// their angle-addition stage form, and refined hardware reciprocals.
+// kernels' table-driven exponential and logarithm, the geodesic kernels'
+// sine/cosine and their angle-addition stage form, and refined hardware
+// reciprocals.
 //
 // The exponential:
 //
@@ -79,6 +80,86 @@ HD_FM_QUAL double exp_tab(double x, const double* T)
         fma(r, fma(r, fma(r, fma(r, fma(r, 1.0 / 120.0, 1.0 / 24.0), 1.0 / 6.0), 0.5), 1.0), 1.0);
     const double v = ldexp(T[ki & 63] * p, ki >> 6);
     return x < EXP_ZERO_BELOW ? 0.0 : v;
+}
+
+// ------------------------------------------------------------------ log
+// ln x by Tang's table-driven method (P. T. P. Tang, "Table-driven
+// implementation of the logarithm function in IEEE floating-point
+// arithmetic", ACM TOMS 16(4):378-400, 1990), the mantissa normalised to
+// [0.75, 1.5) so that k ln2 and ln F never cancel by more than a factor 2.4:
+//
+//   x = 2^k m,  m in [0.75, 1.5),  F = 1 + j/64 nearest m (j = -16..32),
+//   r = (m - F) / F,  |r| <= 1/96,  ln x = k ln2 + ln F + ln(1 + r).
+//
+// m - F is exact (Sterbenz), r = (m - F) (1/F) with the tabulated reciprocal
+// (exact for F = 1, so ln x keeps its relative accuracy next to x = 1),
+// ln(1 + r) by its degree-9 Taylor polynomial (truncation below 2e-18
+// relative), and k ln2 in two parts whose high part has 42 significant bits
+// (k LOG_LN2_HI exact for |k| < 2^11).  Error under 2 ulp (tested against a
+// long-double logarithm on the host build).  About 25 VALU instructions and
+// one 16-byte LDS read, against about 90 for the general log.
+//
+// Domain: every double; 0 gives -inf, +inf gives +inf, x < 0 and NaN give
+// NaN.  The kernels stage LOG_TAB into LDS and pass it as L.
+
+// (1/F, ln F) for F = 1 + j/64, j = -16..32, each correctly rounded (from
+// 60-digit decimal arithmetic)
+constexpr int LOG_TAB_N = 49;
+constexpr double LOG_TAB[2 * LOG_TAB_N] = {
+    0x1.5555555555555p+0, -0x1.269621134db92p-2, 0x1.4e5e0a72f0539p+0, -0x1.1178e8227e47cp-2,
+    0x1.47ae147ae147bp+0, -0x1.f991c6cb3b379p-3, 0x1.4141414141414p+0, -0x1.d1037f2655e7bp-3,
+    0x1.3b13b13b13b14p+0, -0x1.a93ed3c8ad9e3p-3, 0x1.3521cfb2b78c1p+0, -0x1.823c16551a3c2p-3,
+    0x1.2f684bda12f68p+0, -0x1.5bf406b543db2p-3, 0x1.29e4129e4129ep+0, -0x1.365fcb0159016p-3,
+    0x1.2492492492492p+0, -0x1.1178e8227e47cp-3, 0x1.1f7047dc11f70p+0, -0x1.da727638446a2p-4,
+    0x1.1a7b9611a7b96p+0, -0x1.9335e5d594989p-4, 0x1.15b1e5f75270dp+0, -0x1.4d3115d207eacp-4,
+    0x1.1111111111111p+0, -0x1.08598b59e3a07p-4, 0x1.0c9714fbcda3bp+0, -0x1.894aa149fb343p-5,
+    0x1.0842108421084p+0, -0x1.0415d89e74444p-5, 0x1.0410410410410p+0, -0x1.0205658935847p-6,
+    0x1.0000000000000p+0, 0x0.0p+0, 0x1.f81f81f81f820p-1, 0x1.fc0a8b0fc03e4p-7,
+    0x1.f07c1f07c1f08p-1, 0x1.f829b0e783300p-6, 0x1.e9131abf0b767p-1, 0x1.77458f632dcfcp-5,
+    0x1.e1e1e1e1e1e1ep-1, 0x1.f0a30c01162a6p-5, 0x1.dae6076b981dbp-1, 0x1.341d7961bd1d1p-4,
+    0x1.d41d41d41d41dp-1, 0x1.6f0d28ae56b4cp-4, 0x1.cd85689039b0bp-1, 0x1.a926d3a4ad563p-4,
+    0x1.c71c71c71c71cp-1, 0x1.e27076e2af2e6p-4, 0x1.c0e070381c0e0p-1, 0x1.0d77e7cd08e59p-3,
+    0x1.bacf914c1bad0p-1, 0x1.29552f81ff523p-3, 0x1.b4e81b4e81b4fp-1, 0x1.44d2b6ccb7d1ep-3,
+    0x1.af286bca1af28p-1, 0x1.5ff3070a793d4p-3, 0x1.a98ef606a63bep-1, 0x1.7ab890210d909p-3,
+    0x1.a41a41a41a41ap-1, 0x1.9525a9cf456b4p-3, 0x1.9ec8e951033d9p-1, 0x1.af3c94e80bff3p-3,
+    0x1.999999999999ap-1, 0x1.c8ff7c79a9a22p-3, 0x1.948b0fcd6e9e0p-1, 0x1.e27076e2af2e6p-3,
+    0x1.8f9c18f9c18fap-1, 0x1.fb9186d5e3e2bp-3, 0x1.8acb90f6bf3aap-1, 0x1.0a324e27390e3p-2,
+    0x1.8618618618618p-1, 0x1.1675cababa60ep-2, 0x1.8181818181818p-1, 0x1.22941fbcf7966p-2,
+    0x1.7d05f417d05f4p-1, 0x1.2e8e2bae11d31p-2, 0x1.78a4c8178a4c8p-1, 0x1.3a64c556945eap-2,
+    0x1.745d1745d1746p-1, 0x1.4618bc21c5ec2p-2, 0x1.702e05c0b8170p-1, 0x1.51aad872df82dp-2,
+    0x1.6c16c16c16c17p-1, 0x1.5d1bdbf5809cap-2, 0x1.6816816816817p-1, 0x1.686c81e9b14afp-2,
+    0x1.642c8590b2164p-1, 0x1.739d7f6bbd007p-2, 0x1.6058160581606p-1, 0x1.7eaf83b82afc3p-2,
+    0x1.5c9882b931057p-1, 0x1.89a3386c1425bp-2, 0x1.58ed2308158edp-1, 0x1.947941c2116fbp-2,
+    0x1.5555555555555p-1, 0x1.9f323ecbf984cp-2,
+};
+constexpr double LOG_LN2_HI = 0x1.62e42fefa3800p-1;   // ln 2, high 42 bits
+constexpr double LOG_LN2_LO = 0x1.ef35793c76730p-45;  // ln 2 - LOG_LN2_HI
+
+HD_FM_QUAL double log_tab(double x, const double* L)
+{
+    int e;
+    const double m0 = frexp(x, &e);   // x = m0 2^e, m0 in [0.5, 1) for finite x > 0
+    const bool lo_half = m0 < 0.75;
+    const double m = lo_half ? 2.0 * m0 : m0;
+    const double k = double(lo_half ? e - 1 : e);
+    const double jf = rint((m - 1.0) * 64.0);
+    const int j = int(jf) + 16;
+    const double f = m - (1.0 + jf * (1.0 / 64.0));
+    const double r = f * L[2 * j];
+    double q = 1.0 / 9.0;
+    q = fma(r, q, -1.0 / 8.0);
+    q = fma(r, q, 1.0 / 7.0);
+    q = fma(r, q, -1.0 / 6.0);
+    q = fma(r, q, 1.0 / 5.0);
+    q = fma(r, q, -1.0 / 4.0);
+    q = fma(r, q, 1.0 / 3.0);
+    q = fma(r, q, -0.5);
+    const double p = fma(r * r, q, r);
+    const double hi = fma(k, LOG_LN2_HI, L[2 * j + 1]);
+    const double lo = fma(k, LOG_LN2_LO, p);
+    const double v = hi + lo;
+    // specials: frexp leaves 0, inf and NaN unchanged (and m0 < 0 for x < 0)
+    return (x > 0.0 && x < INFINITY) ? v : (x == 0.0 ? -INFINITY : (x == INFINITY ? x : NAN));
 }
 
 // ------------------------------------------------------------------ sincos

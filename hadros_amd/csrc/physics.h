@@ -590,16 +590,20 @@ __host__ __device__ __forceinline__ double cos_theta(double th)
     return c;
 }
 
-// The exponential of the density and source functions: the math library's
-// (the default), or exp_tab over the 2^(j/64) table a kernel staged in LDS
-// (the transfer kernels, where these exponentials run once per path point).
-// Their arguments are at most moderate positive values, inside exp_tab's domain.
+// The exponential and logarithm of the density and source functions: the
+// math library's (the default), or exp_tab and log_tab over the tables a
+// kernel staged in LDS (the transfer kernels, where these run once per path
+// point).  The exponentials' arguments are at most moderate positive values,
+// inside exp_tab's domain.
 struct ExpLib {
     __host__ __device__ double operator()(double x) const { return exp(x); }
+    __host__ __device__ double ln(double x) const { return log(x); }
 };
 struct ExpTab {
-    const double* T;
+    const double* T;   // EXP_TAB
+    const double* L;   // LOG_TAB
     __device__ double operator()(double x) const { return exp_tab(x, T); }
+    __device__ double ln(double x) const { return log_tab(x, L); }
 };
 
 // x^y for x > 0 as exp(y ln x): relative error about |y ln x| ulp, 1e-15 for
@@ -607,7 +611,7 @@ struct ExpTab {
 template <class Exp = ExpLib>
 __host__ __device__ __forceinline__ double pow_pos(double x, double y, Exp ex = Exp{})
 {
-    return ex(y * log(x));
+    return ex(y * ex.ln(x));
 }
 
 // TorusProfile::rho (src/torus_profile.cpp:150-272); pow(x, 2.0) is x*x.
@@ -620,7 +624,7 @@ __host__ __device__ __forceinline__ double rho(const DevMedium& m, double r, dou
 {
     HD_CONTRACT
     double shape;
-    const double lr = (m.powerlaw_disk || m.envelope) ? log(r * m.inv_r0) : 0.0;   // ln(r / r0)
+    const double lr = (m.powerlaw_disk || m.envelope) ? ex.ln(r * m.inv_r0) : 0.0;   // ln(r / r0)
     if (m.kind == HADROS_PROFILE_COLLAPSAR_NDAF_LIKE) {
         if (!(r >= m.rmin && r <= m.rmax) || r <= 0.0 || m.r0 <= 0.0) {
             shape = 0.0;
@@ -1033,7 +1037,7 @@ __device__ __forceinline__ bool point_q(const DevRt& P, double r, double th, dou
     q.rho = rho(P.med, r, th, ex);
     const double A = source_factor(P.src, P.med, r, th, q.rho, ex);
     q.z = zc;
-    q.lnz = log(zc);
+    q.lnz = ex.ln(zc);
     q.g3Adl = g * g * g * A * dl_cm;
     q.nbar_dl = q.rho * (1.0 / M_U_G) * dl_cm;
     return true;

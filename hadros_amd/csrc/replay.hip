@@ -85,7 +85,8 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                                                          double* bin_e, double* bin_lne, double* bin_w,
                                                          double* bin_de, const double* lnE, const double* lnS,
                                                          const double* slope, const float* lnEf, const float* lnSf,
-                                                         const float* slopef, const double* exp_t, double* em)
+                                                         const float* slopef, const double* exp_t, const double* log_t,
+                                                         double* em)
 {
     HD_CONTRACT
     const DevRt& P = *(const DevRt*)Pc;
@@ -132,7 +133,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 pr = A.r[q];
                 pth = A.th[q];
                 pdl = A.dl[q];
-                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq, ExpTab{exp_t});
+                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq, ExpTab{exp_t, log_t});
             }
         } else {
             if (base + lane < b1) {
@@ -141,7 +142,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 pr = rt.x;
                 pth = rt.y;
                 pdl = dz.x;
-                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq, ExpTab{exp_t});
+                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq, ExpTab{exp_t, log_t});
             }
             if (base + 64 < b1) chunk = A.next[chunk];
         }
@@ -214,8 +215,8 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             }
         } else {
             const float lnz = float(pq.lnz);
-            const float lP = (pq.g3Adl > 0.0) ? float(log(pq.g3Adl)) : 0.0f;
-            const float lN = (pq.nbar_dl > 0.0) ? float(log(pq.nbar_dl)) : 0.0f;
+            const float lP = (pq.g3Adl > 0.0) ? float(log_tab(pq.g3Adl, log_t)) : 0.0f;
+            const float lN = (pq.nbar_dl > 0.0) ? float(log_tab(pq.nbar_dl, log_t)) : 0.0f;
             const float gam = float(P.gam), invcut = float(P.invcut);
             const double e_lo = P.e_lo, e_hi = P.e_hi;
             const float sig_lo = float(P.sig_lo), sig_inv_d = float(P.sig_inv_d);
@@ -340,7 +341,9 @@ k_replay(ReplayArgs A)
     // indexed by the rolled bin loop
     __shared__ double bin_tab[REPLAY_WAVES][4][HADROS_MAX_BINS];
     __shared__ double exp_t[64];   // 2^(j/64) for exp_tab
+    __shared__ __align__(16) double log_t[2 * LOG_TAB_N];   // (1/F, ln F) for log_tab
     if (threadIdx.x < 64) exp_t[threadIdx.x] = EXP_TAB[threadIdx.x];
+    if (threadIdx.x < 2 * LOG_TAB_N) log_t[threadIdx.x] = LOG_TAB[threadIdx.x];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double* bin_e = bin_tab[wv][0];
@@ -403,10 +406,10 @@ k_replay(ReplayArgs A)
             batch_pixel(C, gq - pix0[c], p, q);
             const long g = pix0[c] + long(p) * C.sub_ny + q;
             pts_done += replay_ray<PREC, SRC, MEV>(A, (DevRtC)(A.rts + C.pad_rt), ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, em);
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, em);
         } else {
             pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, em);
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, em);
         }
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);

@@ -2,7 +2,9 @@
 host from the same header and checked against long-double references: the
 transfer kernels' table-driven exponential (within 2.5 ulp of the correctly
 rounded value over the finite range, including subnormal results, exact at the
-special values), and the geodesic kernels' sine/cosine and their stage form."""
+special values) and logarithm (within 2 ulp over the whole range, subnormal
+arguments included, exact at 1, 2, 0.75 and the special values), and the
+geodesic kernels' sine/cosine and their stage form."""
 import math
 import os
 import shutil
@@ -57,3 +59,26 @@ def test_sincos_and_stage_shift(sincos_check):
     assert e_fused < 1.5 and e_plain < 1.5, out
     assert e_shift < 2.5, out
     assert any_out == 0
+
+
+@pytest.fixture(scope="module")
+def log_check(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path_factory.mktemp("fastmath") / "log_tab_check")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "hadros_amd", "csrc"),
+                    "-o", exe, os.path.join(ROOT, "tests", "cpp", "log_tab_check.cpp")], check=True)
+    return exe
+
+
+def test_log_tab_ulp_and_specials(log_check):
+    out = subprocess.run([log_check, "2000000"], check=True, capture_output=True, text=True).stdout.split("\n")
+    max_ulp = float(out[0].split()[1])
+    assert max_ulp < 2.0, out[0]
+    sp = out[1].split()[1:]
+    assert float.fromhex(sp[0]) == 0.0
+    assert float.fromhex(sp[1]) == math.log(2.0)
+    assert sp[2] == "-inf" and sp[3] == "inf"
+    assert sp[4] == "1" and sp[5] == "1"
+    assert float.fromhex(sp[6]) == math.log(0.75)

@@ -1,16 +1,16 @@
 """GPU: the transfer kernel's own per-point terms against the oracle.
 
 k_replay (the transfer kernel of the split tracer, the cache replay and the
-batch) evaluates density and source factor with exp_tab, sigma with the
-exact-uniform lookup where the table allows it (else the bracketing one) and
-exp_tab, and the emission with exp_tab.  hadros_eval_transfer_point runs those
+batch) evaluates density, source factor and ln z with exp_tab and log_tab,
+sigma with the exact-uniform lookup where the table allows it (else the
+bracketing one) and exp_tab, and the emission with exp_tab.  hadros_eval_transfer_point runs those
 same device functions at given points; here every point's terms are compared
 with a one-point ray of the oracle's integrate_kerr_ray_spectral
 (src/radiative_transfer.cpp:640-685), whose per-bin tau is that point's dtau
 and whose per-bin intensity is its emission term before absorption:
 
-- FP64: ok flag equal, rho and every dtau_b and emission term within 1e-12
-  relative (below 1e-290 where the oracle's term is 0 or subnormal);
+- FP64: ok flag equal, ln z within 2 ulp of numpy's, rho and every dtau_b
+  and emission term within 1e-12 relative (below 1e-290 where the oracle's term is 0 or subnormal);
 - FP32K (log-form emission, float sigma table): dtau_b and e^L within 1e-4
   relative;
 - over all 8 profiles, 5 sources and 3 spectra, a log-uniform table (exact
@@ -68,6 +68,8 @@ def test_transfer_terms_fp64(name, scene, tname, se, ss):
     r_h = po.lib().or_horizon(A_BH)
     assert np.array_equal(g["ok"], r >= r_h + 1e-3)
     ok = g["ok"]
+    lz = np.log(z[ok])
+    assert np.all(np.abs(g["lnz"][ok] - lz) <= 2 * np.spacing(np.abs(lz)))
     rho_ref = np.array([po.lib().or_rho(scene.medium, r[k], th[k]) for k in range(r.size)])
     assert rel(g["rho"][ok], rho_ref[ok]).max() <= 1e-12
     dt, em = oracle_terms(scene, se, ss, r, th, dl, z)
