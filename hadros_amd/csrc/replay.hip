@@ -289,9 +289,13 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
         bi = 0.0;
     }
     if (lane < nb_out) {
-        if (A.btau) A.btau[g * nb_out + lane] = bt;
-        if (A.bp) A.bp[g * nb_out + lane] = bp;
-        if (A.bi) A.bi[g * nb_out + lane] = bi;
+        // the output index formed here: hoisted out of the ray loop, the three
+        // per-lane base addresses stayed live across it and spilled
+        long o = g * nb_out + lane;
+        asm volatile("" : "+v"(o));
+        if (A.btau) A.btau[o] = bt;
+        if (A.bp) A.bp[o] = bp;
+        if (A.bi) A.bi[o] = bi;
     }
     row_scan_add(sums);
     const double ws = read_lane(sums[0], 15), ts = read_lane(sums[1], 15);
