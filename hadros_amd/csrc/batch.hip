@@ -190,19 +190,11 @@ k_trace_batch(BatchArgs A)
                     }
                 } else {
                     const double err = err_cfg(L.y, y4, y5, C.tol);
-                    acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
-                    if (acc) {
-                        double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
-                        fac = fmin(5.0, fmax(0.2, fac));
-                        L.h = fmax(L.h * fac, CFG_H_MIN);
-                    } else {
-                        double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
-                        fac = fmin(0.9, fmax(0.1, fac));
-                        L.h = fmax(L.h * fac, CFG_H_MIN);
-                        if (L.attempts >= 8L * C.max_steps + 64) {
-                            L.exhausted = 1;
-                            finished = true;
-                        }
+                    acc = (err <= 1.0) | (L.h <= CFG_H_MIN);
+                    L.h = fmax(L.h * cfg_step_factor(err, acc), CFG_H_MIN);
+                    if (!acc && L.attempts >= 8L * C.max_steps + 64) {
+                        L.exhausted = 1;
+                        finished = true;
                     }
                 }
                 if (acc) {
@@ -398,19 +390,11 @@ k_batch_points(BatchPointsArgs A)
                     }
                 } else {
                     const double err = err_cfg(L.y, y4, y5, C.tol);
-                    acc = (err <= 1.0) || (L.h <= CFG_H_MIN);
-                    if (acc) {
-                        double fac = (err > 0.0) ? 0.9 * inv_fifth_root(err) : 5.0;
-                        fac = fmin(5.0, fmax(0.2, fac));
-                        L.h = fmax(L.h * fac, CFG_H_MIN);
-                    } else {
-                        double fac = (err == err) ? 0.9 * inv_fifth_root(err) : 0.1;
-                        fac = fmin(0.9, fmax(0.1, fac));
-                        L.h = fmax(L.h * fac, CFG_H_MIN);
-                        if (L.attempts >= 8L * C.max_steps + 64) {
-                            L.exhausted = 1;
-                            finished = true;
-                        }
+                    acc = (err <= 1.0) | (L.h <= CFG_H_MIN);
+                    L.h = fmax(L.h * cfg_step_factor(err, acc), CFG_H_MIN);
+                    if (!acc && L.attempts >= 8L * C.max_steps + 64) {
+                        L.exhausted = 1;
+                        finished = true;
                     }
                 }
                 if (acc) {

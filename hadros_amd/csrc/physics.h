@@ -478,15 +478,16 @@ __device__ __forceinline__ double err_cfg(const Y5& y, const Y5& y4, const Y5& y
     HD_CONTRACT
     double nb = 0.0, db = 1.0;
     bool bad = false;
+    // bitwise ors and selects: no control flow (a short-circuit || here made
+    // the compiler branch per component)
 #define HD_E(F)                                                                          \
     {                                                                                    \
         const double sc = tol * (1.0 + fmax(fabs(y.F), fabs(y5.F)));                    \
         const double d = fabs(y5.F - y4.F);                                              \
-        bad = bad || !(d == d) || !(sc == sc);                                           \
-        if (d * db > nb * sc) {                                                          \
-            nb = d;                                                                      \
-            db = sc;                                                                     \
-        }                                                                                \
+        bad = bad | (d != d) | (sc != sc);                                               \
+        const bool gt = d * db > nb * sc;                                                \
+        nb = gt ? d : nb;                                                                \
+        db = gt ? sc : db;                                                               \
     }
     HD_E(r); HD_E(th); HD_E(ph); HD_E(pr); HD_E(pth);
 #undef HD_E
@@ -510,6 +511,16 @@ __device__ __forceinline__ double inv_fifth_root(double x)
         y = y * (6.0 - xc * (y2 * y2 * y)) * 0.2;
     }
     return y;
+}
+
+// The CFG controller's step factor: 0.9 err^-1/5 clamped to [0.2, 5] on
+// accept (5 for err = 0) and to [0.1, 0.9] on reject (0.1 for a NaN error).
+// One err^-1/5 serves both outcomes: a wavefront's lanes accept and reject
+// in the same pass, and a branch per outcome evaluated it twice.
+__device__ __forceinline__ double cfg_step_factor(double err, bool acc)
+{
+    const double f = 0.9 * inv_fifth_root(err);
+    return acc ? fmin(5.0, fmax(0.2, (err > 0.0) ? f : 5.0)) : fmin(0.9, fmax(0.1, (err == err) ? f : 0.1));
 }
 
 // ------------------------------------------------------------------ camera
