@@ -47,6 +47,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
     // split-K: this workgroup covers K range [kb, ke)
     const int kb = blockIdx.z * g.k_chunk;
     const int ke = min(g.K, kb + g.k_chunk);
+    // 16-byte tile loads need 16-byte aligned rows
+    const bool a_vec = (g.lda % 8 == 0) && (reinterpret_cast<uintptr_t>(g.A) % 16 == 0);
+    const bool b_vec = (g.ldb % 8 == 0) && (reinterpret_cast<uintptr_t>(g.B) % 16 == 0);
 
     f32x4 acc[2][2];
 #pragma unroll
@@ -55,43 +58,55 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
         for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
     for (int k0 = kb; k0 < ke; k0 += BK) {
-        // ---- A tile (64 x 32) -> As[m][k]
+        // ---- A tile (64 x 32) -> As[m][k].  Each thread moves 8 elements that
+        // are consecutive in memory: one 16-byte load where all 8 lie inside the
+        // matrix (and rows start 16-byte aligned), element loads at the edges.
         if constexpr (!TA) {
             const int r = t >> 2, c = (t & 3) * 8;
-            const int m = m0 + r;
+            const int m = m0 + r, k = k0 + c;
             bf16x8 v;
+            if (a_vec && m < g.M && k + 8 <= ke) {
+                v = *reinterpret_cast<const bf16x8*>(g.A + long(m) * g.lda + k);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = k0 + c + e;
-                v[e] = (m < g.M && k < ke) ? g.A[long(m) * g.lda + k] : (__bf16)0.0f;
+                for (int e = 0; e < 8; ++e) v[e] = (m < g.M && k + e < ke) ? g.A[long(m) * g.lda + k + e] : (__bf16)0.0f;
             }
             *reinterpret_cast<bf16x8*>(&As[r][c]) = v;
         } else {
             const int kk = t >> 3, mm = (t & 7) * 8;
-            const int k = k0 + kk;
+            const int k = k0 + kk, m = m0 + mm;
+            bf16x8 v;
+            if (a_vec && k < ke && m + 8 <= g.M) {
+                v = *reinterpret_cast<const bf16x8*>(g.A + long(k) * g.lda + m);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int m = m0 + mm + e;
-                As[mm + e][kk] = (m < g.M && k < ke) ? g.A[long(k) * g.lda + m] : (__bf16)0.0f;
+                for (int e = 0; e < 8; ++e) v[e] = (m + e < g.M && k < ke) ? g.A[long(k) * g.lda + m + e] : (__bf16)0.0f;
             }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) As[mm + e][kk] = v[e];
         }
-        // ---- B tile (32 x 64) -> Bt[n][k]
+        // ---- B tile (32 x 64) -> Bt[n][k], the same way
         if constexpr (!TB) {
             const int kk = t >> 3, nn = (t & 7) * 8;
-            const int k = k0 + kk;
+            const int k = k0 + kk, n = n0 + nn;
+            bf16x8 v;
+            if (b_vec && k < ke && n + 8 <= g.N) {
+                v = *reinterpret_cast<const bf16x8*>(g.B + long(k) * g.ldb + n);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int n = n0 + nn + e;
-                Bt[nn + e][kk] = (n < g.N && k < ke) ? g.B[long(k) * g.ldb + n] : (__bf16)0.0f;
+                for (int e = 0; e < 8; ++e) v[e] = (n + e < g.N && k < ke) ? g.B[long(k) * g.ldb + n + e] : (__bf16)0.0f;
             }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) Bt[nn + e][kk] = v[e];
         } else {
             const int r = t >> 2, c = (t & 3) * 8;
-            const int n = n0 + r;
+            const int n = n0 + r, k = k0 + c;
             bf16x8 v;
+            if (b_vec && n < g.N && k + 8 <= ke) {
+                v = *reinterpret_cast<const bf16x8*>(g.B + long(n) * g.ldb + k);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = k0 + c + e;
-                v[e] = (n < g.N && k < ke) ? g.B[long(n) * g.ldb + k] : (__bf16)0.0f;
+                for (int e = 0; e < 8; ++e) v[e] = (n < g.N && k + e < ke) ? g.B[long(n) * g.ldb + k + e] : (__bf16)0.0f;
             }
             *reinterpret_cast<bf16x8*>(&Bt[r][c]) = v;
         }
