@@ -465,6 +465,16 @@ struct hadros_tracer {
     DevRt* d_rt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evm = nullptr;   // split form: between the geodesic and the transfer kernel
+    // split form over segments of tile rows: segment 0 runs on the caller's
+    // stream, segment i > 0 on seg_s[i]; each segment's transfer follows its
+    // own geodesic launch, so finished segments' transfer runs while the
+    // longest rays (a few tiles near the critical curve, which set the
+    // geodesic kernel's length) are still being traced
+    static constexpr int NSEG = 4;
+    int n_seg = 1;                       // segments of the last split run
+    hipStream_t seg_s[NSEG] = {};        // [1, NSEG): default priority, created on first use
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_geo[NSEG] = {}, ev_end[NSEG] = {};
     hipStream_t stream = nullptr;   // stream of the last run (a pool regrowth re-runs on it)
     bool pool_too_big = false;  // AUTO: the pool the rays need exceeds SPLIT_POOL_MAX_BYTES, run fused
     bool ran = false;
@@ -911,9 +921,45 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
     if (t->split) HIPC(launch_zero_u64(t->d_ctr, 1, s));
     HIPC(hipEventRecord(t->ev0, s));
     if (t->split) {
-        HIPC(launch_trace_points(t->pargs, t->block ? t->block : 64, s));
+        const DevCam& C = t->pargs.cam;
+        const int tiles_y = (C.ny + 7) / 8, tile_rows = (C.nx + 7) / 8;
+        // segments of at least 4 tile rows each
+        const int nseg = std::max(1, std::min(hadros_tracer::NSEG, tile_rows / 4));
+        t->n_seg = nseg;
+        if (nseg > 1) {
+            if (!t->ev_fork) HIPC(hipEventCreateWithFlags(&t->ev_fork, hipEventDisableTiming));
+            for (int i = 1; i < nseg; ++i)
+                if (!t->seg_s[i]) HIPC(hipStreamCreateWithFlags(&t->seg_s[i], hipStreamNonBlocking));
+            HIPC(hipEventRecord(t->ev_fork, s));
+        }
+        // all geodesic launches first, so that the segments start together;
+        // then each segment's transfer behind its own geodesic launch
+        for (int i = 0; i < nseg; ++i) {
+            if (!t->ev_geo[i]) {
+                HIPC(hipEventCreate(&t->ev_geo[i]));
+                HIPC(hipEventCreate(&t->ev_end[i]));
+            }
+            hipStream_t si = (i == 0) ? s : t->seg_s[i];
+            if (i > 0) HIPC(hipStreamWaitEvent(si, t->ev_fork, 0));
+            const int r0 = int(long(tile_rows) * i / nseg), r1 = int(long(tile_rows) * (i + 1) / nseg);
+            PointsArgs Q = t->pargs;
+            Q.tile0 = r0 * tiles_y;
+            Q.tile1 = r1 * tiles_y;
+            HIPC(launch_trace_points(Q, t->block ? t->block : 64, si));
+            HIPC(hipEventRecord(t->ev_geo[i], si));
+        }
+        for (int i = 0; i < nseg; ++i) {
+            hipStream_t si = (i == 0) ? s : t->seg_s[i];
+            const int r0 = int(long(tile_rows) * i / nseg), r1 = int(long(tile_rows) * (i + 1) / nseg);
+            // pixels of tile rows [r0, r1): rows [8 r0, 8 r1) of the image
+            ReplayArgs R = t->rargs;
+            R.ray0 = long(std::min(8 * r0, C.nx)) * C.ny;
+            R.n_rays = int(long(std::min(8 * r1, C.nx)) * C.ny - R.ray0);
+            HIPC(launch_replay(R, t->precision, si));
+            HIPC(hipEventRecord(t->ev_end[i], si));
+            if (i > 0) HIPC(hipStreamWaitEvent(s, t->ev_end[i], 0));
+        }
         HIPC(hipEventRecord(t->evm, s));
-        HIPC(launch_replay(t->rargs, t->precision, s));
     } else {
         HIPC(launch_trace_rt(t->args, t->precision, t->block ? t->block : 128, s));
     }
@@ -1019,8 +1065,18 @@ int hadros_tracer_kernel_ms(hadros_tracer* t, double* geodesic_ms, double* trans
     if (rc) return rc;
     float g = 0.0f, r = 0.0f;
     if (t->split) {
-        HIPC(hipEventElapsedTime(&g, t->ev0, t->evm));
-        HIPC(hipEventElapsedTime(&r, t->evm, t->ev1));
+        // the segments overlap: the geodesic span runs from the start to the
+        // last segment's geodesic end, the transfer span from the first
+        // segment's transfer start to the end
+        float tot = 0.0f, first_rt = 0.0f;
+        HIPC(hipEventElapsedTime(&tot, t->ev0, t->ev1));
+        for (int i = 0; i < t->n_seg; ++i) {
+            float gi = 0.0f;
+            HIPC(hipEventElapsedTime(&gi, t->ev0, t->ev_geo[i]));
+            g = std::max(g, gi);
+            first_rt = (i == 0) ? gi : std::min(first_rt, gi);
+        }
+        r = tot - first_rt;
     } else {
         HIPC(hipEventElapsedTime(&g, t->ev0, t->ev1));
     }
@@ -1093,6 +1149,12 @@ int hadros_tracer_destroy(hadros_tracer* t)
     if (t->evm) (void)hipEventDestroy(t->evm);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
+    if (t->ev_fork) (void)hipEventDestroy(t->ev_fork);
+    for (int i = 0; i < hadros_tracer::NSEG; ++i) {
+        if (t->ev_geo[i]) (void)hipEventDestroy(t->ev_geo[i]);
+        if (t->ev_end[i]) (void)hipEventDestroy(t->ev_end[i]);
+        if (t->seg_s[i]) (void)hipStreamDestroy(t->seg_s[i]);
+    }
     delete t;
     return HADROS_OK;
 }

@@ -77,6 +77,7 @@ struct PointsArgs {
     ChunkPool cp;           // base = nx * ny: chunk k < base is pixel k's first chunk
     int* n_st;              // [nx * ny] points stored per pixel
     TraceOut out;           // captured, npoints, flags, alpha, beta, counters
+    int tile0, tile1;       // the launch's 8x8 tiles [tile0, tile1), row-major over tile rows
 };
 
 struct CacheArgs {

@@ -412,7 +412,9 @@ k_replay(ReplayArgs A)
             pts_done += replay_ray<PREC, SRC, MEV>(A, (DevRtC)(A.rts + C.pad_rt), ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w,
                                                    bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, em);
         } else {
-            pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, ray, ray, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
+            // the chunk source may cover a range of the tracer's rays: ray0 + ray
+            const long gr = (SRC == REPLAY_SRC_CHUNKS) ? A.ray0 + ray : ray;
+            pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, gr, gr, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
                                                    bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, em);
         }
     }

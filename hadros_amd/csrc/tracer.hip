@@ -371,11 +371,11 @@ k_trace_points(PointsArgs A)
 {
     const DevCam& C = A.cam;
     const int lane = threadIdx.x & 63;
-    const long gw = (long(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const long gw = A.tile0 + ((long(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
     const int tiles_y = (C.ny + 7) >> 3;
     const int ti = int(gw / tiles_y), tj = int(gw % tiles_y);
     const int i = ti * 8 + (lane >> 3), j = tj * 8 + (lane & 7);
-    const bool valid = (i < C.nx) && (j < C.ny) && (ti * 8 < C.nx);
+    const bool valid = (gw < A.tile1) && (i < C.nx) && (j < C.ny) && (ti * 8 < C.nx);
     const double a = C.a;
     const double u = (2.0 * (i + 0.5) / C.nx - 1.0) * C.tanhalf;
     const double v = (2.0 * (j + 0.5) / C.ny - 1.0) * C.tanhalf;
@@ -494,7 +494,8 @@ k_trace_points(PointsArgs A)
 
 hipError_t launch_trace_points(const PointsArgs& A, int block, hipStream_t s)
 {
-    const int tiles = ((A.cam.nx + 7) / 8) * ((A.cam.ny + 7) / 8);
+    const int tiles = A.tile1 - A.tile0;
+    if (tiles <= 0) return hipSuccess;
     const long threads = long(tiles) * 64;
     const int grid = int((threads + block - 1) / block);
     if (A.cam.mode == HADROS_MODE_CFG) hipLaunchKernelGGL(k_trace_points<HADROS_MODE_CFG>, dim3(grid), dim3(block), 0, s, A);
