@@ -382,7 +382,7 @@ def main():
     ap.add_argument("--no-opacity", action="store_true", help="skip the opacity-surface leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-target", nargs="?", const="tracer", choices=["tracer", "replay", "batch"],
-                    help="for rocprofv3 passes: run the plain tracer (or the cache replay, or the 256-camera batch "
+                    help="for rocprofv3 passes: run the plain tracer as one band (or the cache replay, or the 256-camera batch "
                          "at block 128 and 64 steps per refill, without the autotuner) twice, then fill and read a "
                          "1 GiB device buffer twice each with the library's streaming kernels (calibration of "
                          "WRITE_SIZE / FETCH_SIZE), print nothing else")
@@ -405,7 +405,8 @@ def main():
     npix = cam.nx * cam.ny
     if args.pmc_target:
         if args.pmc_target == "tracer":
-            t = ffi.Tracer(cam, sc, e, s, abi.PREC_FP64, args.block)
+            # one geodesic launch and one transfer launch per run (the kernels' own counters)
+            t = ffi.Tracer(cam, sc, e, s, abi.PREC_FP64, args.block, segments=1)
         elif args.pmc_target == "replay":
             t = ffi.Replay(ffi.trace_cache(cam), sc, e, s, abi.PREC_FP64)
         else:
@@ -430,8 +431,8 @@ def main():
         assert float(part.sum()) == float(x.numel())
         return
 
-    def timed(precision, scene=sc):
-        t = ffi.Tracer(cam, scene, e, s, precision, args.block)
+    def timed(precision, scene=sc, segments=0):
+        t = ffi.Tracer(cam, scene, e, s, precision, args.block, segments=segments)
         for _ in range(args.warmup):
             t.run()
             t.last_ms()
@@ -448,11 +449,17 @@ def main():
         t.close()
         return wall, kern, np.mean(np.array(parts), axis=0), img, st
 
-    wall, kern, (geo_ms, rt_ms), img, st = timed(abi.PREC_FP64)
+    # the headline: the default schedule, whose tile-row bands overlap one
+    # band's transfer with the others' geodesic launches
+    wall, kern, (geo_span, rt_span), img, st = timed(abi.PREC_FP64)
     ms_step = 1e3 * wall / args.steps
     value = args.steps * npix / wall
     kavg = float(np.mean(kern))
-    # per kernel of the split form (HIP events between the two launches)
+    # each kernel priced on its own launch: the same image as one geodesic
+    # launch, then one transfer launch (HIP events between the two on the
+    # launch stream); the bands give the same image bit for bit
+    _, _, (geo_ms, rt_ms), img1, _ = timed(abi.PREC_FP64, segments=1)
+    assert np.array_equal(img1.tau, img.tau) and np.array_equal(img1.I, img.I), "segment counts disagree"
     kernels = {}
     if rt_ms > 0:
         kernels = {
@@ -479,10 +486,12 @@ def main():
     thin.medium.rho0 *= 1e-8
     thin.medium.envelope_rho0 *= 1e-8
     thin_imgs = {}
-    wall_t, kern_t, (geo_t, rt_t), thin_imgs["fp64"], st_t = timed(abi.PREC_FP64, thin)
+    wall_t, kern_t, (geo_ts, rt_ts), thin_imgs["fp64"], st_t = timed(abi.PREC_FP64, thin)
+    _, _, (geo_t, rt_t), _, _ = timed(abi.PREC_FP64, thin, segments=1)
     thin_imgs["fp32k"], _ = ffi.trace_rt(cam, thin, e, s, abi.PREC_FP32K)
     thin_leg = {"value": args.steps * npix / wall_t, "unit": "rays/s", "kernel_ms": float(np.mean(kern_t)),
                 "geodesic_ms": float(geo_t), "transfer_ms": float(rt_t),
+                "banded_spans_ms": {"geodesic": float(geo_ts), "transfer": float(rt_ts)},
                 "mean_P": float(thin_imgs["fp64"].p.mean()), "sum_I": float(thin_imgs["fp64"].I.sum()),
                 "points_per_ray": st_t.total_points / npix}
     if rt_t > 0:
@@ -523,7 +532,11 @@ def main():
                      "ridge_flop_per_byte": RIDGE_FLOP_PER_BYTE, "tracer_ms": kavg,
                      "algorithmic_flop_per_launch": dom["flop"], "algorithmic_bytes_per_launch": dom["algorithmic_bytes"],
                      "points_per_launch": st.total_points, "attempts_per_launch": st.total_attempts,
-                     "kernels": kernels},
+                     "kernels": kernels,
+                     # the timed schedule: the spans of the tile-row bands' overlapping
+                     # launches (start to the last geodesic end; first transfer start to the end)
+                     "banded": {"segments": abi.TRACER_MAX_SEGMENTS, "geodesic_span_ms": float(geo_span),
+                                "transfer_span_ms": float(rt_span), "step_ms": kavg}},
         "precision_form": {"value": args.steps * npix / wall32, "unit": "rays/s", "dtype": "f64 geodesic + f32 Kahan",
                            "kernel_ms": float(np.mean(kern32)), "max_rel_tau_vs_fp64": err_tau,
                            "max_abs_p_vs_fp64": err_p, "rel_sum_i_vs_fp64": err_i},

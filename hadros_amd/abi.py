@@ -28,6 +28,7 @@ MODE_CFG = 1
 PREC_FP64 = 0
 PREC_FP32K = 1
 PATH_AUTO, PATH_FUSED, PATH_SPLIT = 0, 1, 2
+TRACER_MAX_SEGMENTS = 4                  # hadros_tracer_set_segments
 
 MAX_BINS = 16
 EVAL_TRANSFER_COLS = 6 + 2 * MAX_BINS   # hadros_eval_transfer_point columns

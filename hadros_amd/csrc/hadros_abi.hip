@@ -464,13 +464,13 @@ struct hadros_tracer {
     unsigned long long* d_cnt = nullptr;
     DevRt* d_rt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t evm = nullptr;   // split form: between the geodesic and the transfer kernel
     // split form over segments of tile rows: segment 0 runs on the caller's
     // stream, segment i > 0 on seg_s[i]; each segment's transfer follows its
     // own geodesic launch, so finished segments' transfer runs while the
     // longest rays (a few tiles near the critical curve, which set the
     // geodesic kernel's length) are still being traced
-    static constexpr int NSEG = 4;
+    static constexpr int NSEG = HADROS_TRACER_MAX_SEGMENTS;
+    int seg_req = 0;                     // hadros_tracer_set_segments: 0 = automatic
     int n_seg = 1;                       // segments of the last split run
     hipStream_t seg_s[NSEG] = {};        // [1, NSEG): default priority, created on first use
     hipEvent_t ev_fork = nullptr;
@@ -846,7 +846,6 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
     HIPC(hipMemcpy(t->d_sigf, sf.data(), sf.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPC(hipEventCreate(&t->ev0));
     HIPC(hipEventCreate(&t->ev1));
-    HIPC(hipEventCreate(&t->evm));
     TraceArgs& A = t->args;
     A.sig = t->d_sig;
     A.sigf = t->d_sigf;
@@ -923,8 +922,9 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
     if (t->split) {
         const DevCam& C = t->pargs.cam;
         const int tiles_y = (C.ny + 7) / 8, tile_rows = (C.nx + 7) / 8;
-        // segments of at least 4 tile rows each
-        const int nseg = std::max(1, std::min(hadros_tracer::NSEG, tile_rows / 4));
+        // automatic: segments of at least 4 tile rows each
+        const int nseg = t->seg_req ? std::min(t->seg_req, tile_rows)
+                                    : std::max(1, std::min(hadros_tracer::NSEG, tile_rows / 4));
         t->n_seg = nseg;
         if (nseg > 1) {
             if (!t->ev_fork) HIPC(hipEventCreateWithFlags(&t->ev_fork, hipEventDisableTiming));
@@ -959,7 +959,6 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
             HIPC(hipEventRecord(t->ev_end[i], si));
             if (i > 0) HIPC(hipStreamWaitEvent(s, t->ev_end[i], 0));
         }
-        HIPC(hipEventRecord(t->evm, s));
     } else {
         HIPC(launch_trace_rt(t->args, t->precision, t->block ? t->block : 128, s));
     }
@@ -1012,6 +1011,15 @@ int hadros_tracer_set_block(hadros_tracer* t, int block)
     if (block != 0 && (block < 64 || block > 256 || block % 64))
         return fail(HADROS_ERR_ARG, "block size must be 0 (automatic), 64, 128, 192 or 256");
     t->block = block;
+    return HADROS_OK;
+}
+
+int hadros_tracer_set_segments(hadros_tracer* t, int n_segments)
+{
+    if (!t) return fail(HADROS_ERR_ARG, "NULL tracer");
+    if (n_segments < 0 || n_segments > HADROS_TRACER_MAX_SEGMENTS)
+        return fail(HADROS_ERR_ARG, "segments must be 0 (automatic) or 1 to HADROS_TRACER_MAX_SEGMENTS");
+    t->seg_req = n_segments;
     return HADROS_OK;
 }
 
@@ -1146,7 +1154,6 @@ int hadros_tracer_destroy(hadros_tracer* t)
     dfree(t->d_ctr);
     dfree(t->d_rt);
     mev_free(t->mev);
-    if (t->evm) (void)hipEventDestroy(t->evm);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
     if (t->ev1) (void)hipEventDestroy(t->ev1);
     if (t->ev_fork) (void)hipEventDestroy(t->ev_fork);

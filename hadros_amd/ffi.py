@@ -27,6 +27,7 @@ EXPORTS = [
     "hadros_scene_defaults", "hadros_parse_profile", "hadros_parse_source", "hadros_parse_spectrum",
     "hadros_profile_name", "hadros_source_name", "hadros_spectrum_name", "hadros_sigma_load",
     "hadros_sigma_check", "hadros_trace_rt", "hadros_trace_rt_noalloc", "hadros_tracer_create", "hadros_tracer_set_block", "hadros_tracer_set_path",
+    "hadros_tracer_set_segments",
     "hadros_tracer_run", "hadros_tracer_last_ms", "hadros_tracer_kernel_ms", "hadros_tracer_fetch",
     "hadros_tracer_destroy",
     "hadros_rt_from_cache", "hadros_replay_create", "hadros_replay_run", "hadros_replay_last_ms",
@@ -87,6 +88,7 @@ def lib():
                                        C.POINTER(C.c_void_p)]
     L.hadros_tracer_set_block.argtypes = [C.c_void_p, C.c_int]
     L.hadros_tracer_set_path.argtypes = [C.c_void_p, C.c_int]
+    L.hadros_tracer_set_segments.argtypes = [C.c_void_p, C.c_int]
     L.hadros_tracer_run.argtypes = [C.c_void_p, C.c_void_p]
     L.hadros_tracer_last_ms.argtypes = [C.c_void_p, _dp]
     L.hadros_tracer_kernel_ms.argtypes = [C.c_void_p, _dp, _dp]
@@ -259,7 +261,8 @@ def _fetch_mev(fn, h, n):
 class Tracer:
     """Re-usable plan of hadros_trace_rt (device buffers allocated once)."""
 
-    def __init__(self, cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0, path=abi.PATH_AUTO):
+    def __init__(self, cam, scene, sig_e, sig_s, precision=abi.PREC_FP64, block=0, path=abi.PATH_AUTO,
+                 segments=0):
         L = lib()
         self._se, self._ss = _sig(sig_e, sig_s)
         self.cam = cam
@@ -271,6 +274,12 @@ class Tracer:
             check(L.hadros_tracer_set_block(self.h, block))
         if path != abi.PATH_AUTO:
             check(L.hadros_tracer_set_path(self.h, path))
+        if segments:
+            check(L.hadros_tracer_set_segments(self.h, segments))
+
+    def set_segments(self, n):
+        """Tile-row bands of the split form (0 = automatic; hadros_tracer_set_segments)."""
+        check(lib().hadros_tracer_set_segments(self.h, n))
 
     def run(self, stream=None):
         check(lib().hadros_tracer_run(self.h, stream))
@@ -288,8 +297,10 @@ class Tracer:
         return _fetch_mev(lib().hadros_tracer_fetch_mev, self.h, self.cam.nx * self.cam.ny)
 
     def kernel_ms(self):
-        """(geodesic kernel ms, transfer kernel ms) of the last run; the fused
-        form reports its one kernel as the first."""
+        """(geodesic ms, transfer ms) of the last run: with one segment the two
+        launches' durations, with more the spans of the overlapping bands
+        (hadros_tracer_kernel_ms); the fused form reports its one kernel as
+        the first."""
         g, r = C.c_double(), C.c_double()
         check(lib().hadros_tracer_kernel_ms(self.h, C.byref(g), C.byref(r)))
         return g.value, r.value

@@ -374,14 +374,31 @@ int hadros_tracer_set_block(hadros_tracer* t, int block_size);
    8 GB, fused otherwise.  The split form's pool grows on demand. */
 enum { HADROS_PATH_AUTO = 0, HADROS_PATH_FUSED = 1, HADROS_PATH_SPLIT = 2 };
 int hadros_tracer_set_path(hadros_tracer* t, int path);
+/* Segments of the split form (no reference counterpart; scheduling only, the
+   results are bit for bit the same for every count).  The camera's rows of
+   8x8 tiles are cut into n_segments contiguous bands; band 0 runs on the
+   caller's stream, band i > 0 on a stream of the plan's own (default
+   priority, created on first use) that waits for the caller's stream at the
+   start of the run and that the caller's stream waits for at its end.  All
+   bands' geodesic launches are queued first, then each band's transfer
+   launch behind its own geodesic launch: the transfer of the bands whose
+   rays finish early runs while the longest rays are still traced.
+   0 (default) = automatic: HADROS_TRACER_MAX_SEGMENTS bands of at least four
+   tile rows each (fewer on small cameras); 1 = one geodesic launch, then one
+   transfer launch.  Counts above the camera's tile rows are cut to them. */
+#define HADROS_TRACER_MAX_SEGMENTS 4
+int hadros_tracer_set_segments(hadros_tracer* t, int n_segments);
 int hadros_tracer_run(hadros_tracer* t, void* stream);
 /* Device time (ms) of the last run's kernel, measured with HIP events on the
    launch stream; waits for the run to finish. */
 int hadros_tracer_last_ms(hadros_tracer* t, double* ms);
-/* The last run's time split by kernel (HIP events on the launch stream): the
-   split form's geodesic kernel and its transfer kernel; the fused form puts
-   its one kernel in geodesic_ms and 0 in transfer_ms.  No reference
-   counterpart (measurement hook). */
+/* The last run's time split by kernel (HIP events on the launch streams).
+   Split form: geodesic_ms spans the run's start to the end of the last
+   band's geodesic launch, transfer_ms the first band's transfer start to the
+   run's end; with one segment these are the two launches' own durations,
+   with more they overlap.  The fused form puts its one kernel in
+   geodesic_ms and 0 in transfer_ms.  No reference counterpart (measurement
+   hook). */
 int hadros_tracer_kernel_ms(hadros_tracer* t, double* geodesic_ms, double* transfer_ms);
 int hadros_tracer_fetch(hadros_tracer* t, hadros_image_out* out, hadros_stats* stats);
 /* MeV channel of integrate_kerr_ray (src/radiative_transfer.cpp:500-640) for the

@@ -198,6 +198,43 @@ def test_split_pool_grows_and_matches_fused():
     t.close()
 
 
+def test_split_segments_bitwise_invariant():
+    """The split form's tile-row bands are scheduling only: every segment
+    count gives the same image bit for bit, on a ragged camera (70 x 54: a
+    partial last tile row and tile column), in both precision forms, with the
+    thin medium so that every output column is non-trivial.  Counts beyond the
+    camera's tile rows are cut to them; counts outside [0, 4] are rejected."""
+    cam = cfg_camera(nx=70, ny=54)
+    sc = workloads.c2_scene()
+    sc.medium.rho0 *= 1e-8
+    sc.medium.envelope_rho0 *= 1e-8
+    cols = ("tau", "p", "I", "captured", "npoints", "flags", "alpha", "beta", "bin_tau", "bin_p", "bin_i")
+    for prec in (abi.PREC_FP64, abi.PREC_FP32K):
+        base = None
+        for nseg in (1, 0, 2, 3, 4):
+            t = ffi.Tracer(cam, sc, SIG_E, SIG_S, prec, path=abi.PATH_SPLIT, segments=nseg)
+            for _ in range(2):
+                t.run()
+                img, st = t.fetch()
+                g, r = t.kernel_ms()
+                tot = t.last_ms()
+                assert 0.0 < g <= tot and 0.0 < r <= tot, (nseg, g, r, tot)
+                if nseg == 1:
+                    assert g + r <= tot * 1.01 + 0.01, (g, r, tot)
+                if base is None:
+                    base = img
+                    assert 0.0 < img.p.mean() < 1.0
+                    continue
+                for c in cols:
+                    assert np.array_equal(getattr(img, c), getattr(base, c)), (prec, nseg, c)
+            t.close()
+    t = ffi.Tracer(cam, sc, SIG_E, SIG_S, abi.PREC_FP64, path=abi.PATH_SPLIT)
+    for bad in (-1, abi.TRACER_MAX_SEGMENTS + 1):
+        with pytest.raises(ffi.HadrosError):
+            t.set_segments(bad)
+    t.close()
+
+
 def test_trace_cache_matches_oracle_points():
     cam = small_ref_camera(nx=4, ny=4)
     cache = ffi.trace_cache(cam)
