@@ -34,16 +34,11 @@ constexpr int REPLAY_BLOCK = 256;
 typedef const __attribute__((address_space(4))) DevRt* DevRtC;
 constexpr int REPLAY_WAVES = REPLAY_BLOCK / 64;
 
-// Per-wavefront, per-bin running sums of the current ray, kept in LDS so that
-// the bin loop stays rolled (few registers, more resident wavefronts).  Lane
-// 63 holds each chunk's inclusive totals and alone updates them; LDS
-// operations of one wavefront complete in order, so the next bin/chunk reads
-// see the update.  The FP64 intensity is not summed per chunk: each lane adds
-// its own points' emission terms to its slot em[b * 64 + lane] (dynamic LDS),
-// and the 64 partial sums of a bin are added once, at the end of the ray.
-struct ReplayAcc64 {
-    double tau[HADROS_MAX_BINS];
-};
+// FP32 form: per-wavefront, per-bin running sums of the current ray, kept in
+// LDS so that the bin loop stays rolled (few registers, more resident
+// wavefronts).  Lane 63 holds each chunk's inclusive totals and alone updates
+// them; LDS operations of one wavefront complete in order, so the next
+// bin/chunk reads see the update.
 static_assert(HADROS_MAX_BINS <= 16, "the ray-end reduction keeps one bin per lane of a 16-lane row");
 struct ReplayAcc32 {
     Kahan tau[HADROS_MAX_BINS];
@@ -62,11 +57,36 @@ __device__ __forceinline__ const DevRt& scene_per_chunk(DevRtC Pc)
     return *(const DevRt*)Pc;
 }
 
-// Doubles of dynamic LDS per wavefront for the FP64 emission partial sums:
-// one per lane and bin.
+// FP64 form, bin-major.  After a chunk's per-point pass (lane = point) the
+// points' (z, ln z, n_N dl, g^3 A dl) go to LDS, and the bins take the lanes:
+// with NBP = nb rounded up to a power of two, bin b owns the G = 64 / NBP
+// lanes b G .. b G + G - 1, and lane s of them the chunk's points
+// s Lc .. s Lc + Lc - 1 (Lc = ceil(points / G), so a partial last chunk is
+// spread over every lane; with more than 8 bins the chunk is taken in two
+// halves, so that Lc <= RP_SEG).  A lane sums its points' optical depths in path
+// order, a prefix sum over the bin's G lanes gives tau before each segment,
+// and a second pass over the segment weights each point's emission by
+// e^{-tau before it}.  Each lane's running tau and emission partial sum live
+// in LDS across the ray (registers held through the per-point pass would
+// spill); the bin's G emission sums are added at the ray's end.
+// Staged arrays: 64 points with one pad double after every 8 (a lane's
+// segment start is then bank-spread), 72 doubles each.
+constexpr int RP_STAGE = 72;
+constexpr int RP_SEG = 8;   // points per lane per pass, at most
+__host__ __device__ constexpr int replay_stage_index(int q) { return q + (q >> 3); }
+__host__ __device__ constexpr int replay_bins_pow2(int nb)
+{
+    return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16;
+}
+
+// Doubles of dynamic LDS per wavefront: the FP64 form's four staged point
+// arrays, per lane one emission exponent per point of its segment, and per
+// lane the running tau and emission sum.
 __host__ __device__ constexpr int replay_emission_slots(int precision, int nb)
 {
-    return precision == HADROS_PREC_FP64 ? 64 * nb : 0;
+    return precision == HADROS_PREC_FP64
+               ? 4 * RP_STAGE + 64 * (replay_bins_pow2(nb) < RP_SEG ? replay_bins_pow2(nb) : RP_SEG) + 128
+               : 0;
 }
 
 // The batch's camera start table is staged in LDS when it is short and the
@@ -81,12 +101,12 @@ __host__ __device__ constexpr bool replay_pix0_in_lds(int precision, int nb, int
 // scene's parameters P, outputs at global ray index g (bins nb_out apart).
 template <int PREC, int SRC, bool MEV>
 __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, DevRtC Pc, long ray, long g,
-                                                         int nb_out, int lane, ReplayAcc64& a64, ReplayAcc32& a32,
+                                                         int nb_out, int lane, ReplayAcc32& a32,
                                                          double* bin_e, double* bin_lne, double* bin_w,
                                                          double* bin_de, const double* lnE, const double* lnS,
                                                          const double* slope, const float* lnEf, const float* lnSf,
                                                          const float* slopef, const double* exp_t, const double* log_t,
-                                                         double* em)
+                                                         double* wsp)
 {
     HD_CONTRACT
     const DevRt& P = *(const DevRt*)Pc;
@@ -96,17 +116,12 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
     long b0 = 0, b1 = 0;
     int chunk = int(ray);
     if constexpr (SRC == REPLAY_SRC_RANGES) {
-        b0 = A.offset[ray];
-        b1 = A.offset[ray + 1];
+        b0 = uniform_long(A.offset[ray]);
+        b1 = uniform_long(A.offset[ray + 1]);
     } else {
-        b1 = A.n_st[ray];
-    }
-    if constexpr (PREC == HADROS_PREC_FP64) {
-#pragma unroll 1
-        for (int b = 0; b < nb; ++b) em[b * 64 + lane] = 0.0;
+        b1 = uniform_long(A.n_st[ray]);
     }
     if (lane < nb) {
-        a64.tau[lane] = 0.0;
         a32.tau[lane] = {0.0f, 0.0f};
         a32.I[lane] = {{0.0f, 0.0f}, 0.0f, 0};
         if constexpr (SRC == REPLAY_SRC_BATCH) {   // this ray's scene
@@ -117,6 +132,18 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
         }
     }
     __builtin_amdgcn_wave_barrier();
+    // FP64 bin-major layout of this ray's scene (replay_emission_slots): lane
+    // = (bin bl, segment seg); lanes of bins beyond nb compute on and drop
+    const int nbp = replay_bins_pow2(nb);
+    const int lg_g = 6 - __builtin_ctz(nbp);   // log2 G, G = 64 / nbp lanes per bin
+    const int G = 1 << lg_g;
+    const int bl = lane >> lg_g;
+    const bool bin_on = bl < nb;
+    double* run = wsp + 4 * RP_STAGE + 64 * (nbp < RP_SEG ? nbp : RP_SEG);   // [tau, emission][lane]
+    if constexpr (PREC == HADROS_PREC_FP64) {
+        run[lane] = 0.0;
+        run[64 + lane] = 0.0;
+    }
     // MeV channel of integrate_kerr_ray (src/radiative_transfer.cpp:500-640), kept
     // wave-uniform: tau_mev, I_mev, the summed j dl (leakage) and r_ns.  The
     // spectral reference takes it from its first integrated bin, so a scene
@@ -188,31 +215,64 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             pq.rho = 0.0;
         }
         if constexpr (PREC == HADROS_PREC_FP64) {
+            // stage the chunk's points (lane = point); z carries the
+            // absorption flag in its sign (z > 0 always)
+            const bool absorbs = ok && pq.rho > 0.0;
+            const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
+            const int si = replay_stage_index(lane);
+            wsp[si] = absorbs ? pq.z : -pq.z;
+            wsp[RP_STAGE + si] = pq.lnz;
+            wsp[2 * RP_STAGE + si] = pq.nbar_dl;
+            wsp[3 * RP_STAGE + si] = emits ? pq.g3Adl : 0.0;
+            const bool any_emit = __ballot(emits) != 0ull;
+            __builtin_amdgcn_wave_barrier();
             // the scene's bin-loop constants, read once per chunk
             const double e_lo = P.e_lo, e_hi = P.e_hi, gam = P.gam, invcut = P.invcut;
             const double sig_lo = P.sig_lo, sig_inv_d = P.sig_inv_d;
             const bool exact = P.sig_exact != 0;
-            const bool absorbs = ok && pq.rho > 0.0;
-            const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
-#pragma unroll 1
-            for (int b = 0; b < nb; ++b) {
-                const double E = bin_e[b] * pq.z;
-                const double lnEb = bin_lne[b] + pq.lnz;
-                const double dt = bin_dtau64(exact, absorbs, E, lnEb, pq.nbar_dl, lnE, lnS, slope, n, sig_lo,
-                                             sig_inv_d, e_lo, e_hi, exp_t);
-                double inc[1] = {dt};
-                wave_scan_add(inc);
-                const double tau_b = a64.tau[b];
-                const double before = tau_b + (inc[0] - dt);
-                const double arg = bin_emis_arg(gam, invcut, E, lnEb) - before;
-                // exp(arg) is exactly 0 below EXP64_ZERO_BELOW: a chunk whose
-                // every term is zero (behind an opaque column) adds nothing
-                const bool live = __ballot(emits && !(arg < EXP64_ZERO_BELOW)) != 0ull;
-                if (live && emits) em[b * 64 + lane] += pq.g3Adl * exp_tab(arg, exp_t);
-                __builtin_amdgcn_wave_barrier();
-                if (lane == 63) a64.tau[b] = tau_b + inc[0];
+            const double eb = bin_on ? bin_e[bl] : 1.0, lneb = bin_on ? bin_lne[bl] : 0.0;
+            const long left = b1 - base;
+            const int cnt = left < 64 ? int(left) : 64;
+            const int part = RP_SEG << lg_g;   // points per pass: 64, or 32 with 16 bins
+            double* ey = wsp + 4 * RP_STAGE;   // [point of segment][lane]
+            for (int s0 = 0; s0 < cnt; s0 += part) {
+                const int c2 = (cnt - s0 < part) ? cnt - s0 : part;
+                const int lc = (c2 + G - 1) >> lg_g;   // points per lane
+                const int q0 = s0 + (lane & (G - 1)) * lc;
+                // pass 1: optical depth in path order; each point's emission
+                // exponent less tau before it within the segment
+                double loc = 0.0;
+#pragma unroll 2
+                for (int k = 0; k < lc; ++k) {
+                    const int qi = replay_stage_index(q0 + k);
+                    const double zs = wsp[qi], lnz = wsp[RP_STAGE + qi], nbdl = wsp[2 * RP_STAGE + qi];
+                    const double E = eb * fabs(zs);
+                    const double lnEb = lneb + lnz;
+                    const double dt = bin_dtau64(exact, bin_on && zs > 0.0, E, lnEb, nbdl, lnE, lnS, slope, n,
+                                                 sig_lo, sig_inv_d, e_lo, e_hi, exp_t);
+                    ey[k * 64 + lane] = bin_emis_arg(gam, invcut, E, lnEb) - loc;
+                    loc += dt;
+                }
+                const double incl = group_scan_add(loc, lane, G);
+                const double tot = __shfl(incl, lane | (G - 1));
+                const double tau_run = run[lane];
+                const double pre = tau_run + (incl - loc);   // tau before the segment
+                // pass 2: emission; exp(arg) is exactly 0 below EXP64_ZERO_BELOW
+                // (behind an opaque column), so such terms are skipped
+                if (any_emit) {
+                    double em_acc = run[64 + lane];
+#pragma unroll 2
+                    for (int k = 0; k < lc; ++k) {
+                        const double g3 = wsp[3 * RP_STAGE + replay_stage_index(q0 + k)];
+                        const double arg = ey[k * 64 + lane] - pre;
+                        if (bin_on && g3 != 0.0 && !(arg < EXP64_ZERO_BELOW)) em_acc += g3 * exp_tab(arg, exp_t);
+                    }
+                    run[64 + lane] = em_acc;
+                }
+                run[lane] = tau_run + tot;
                 __builtin_amdgcn_wave_barrier();
             }
+            __builtin_amdgcn_wave_barrier();
         } else {
             const float lnz = float(pq.lnz);
             const float lP = (pq.g3Adl > 0.0) ? float(log_tab(pq.g3Adl, log_t)) : 0.0f;
@@ -255,24 +315,19 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
         }
     }
     // Ray-end outputs, one lane per bin (lanes 0..nb_out-1 of the first row).
-    double bi = 0.0;
+    double bi = 0.0, bt64 = 0.0;
     if constexpr (PREC == HADROS_PREC_FP64) {
-        // I_b: lanes b, b+16, b+32, b+48 each add 16 of the bin's 64 partial
-        // sums, then two cross-row steps add the four
-        const int bb = lane & 15;
-        if (bb < nb) {
-            const double* src = em + bb * 64 + (lane >> 4) * 16;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) bi += src[i];
-        }
-        bi += __shfl_xor(bi, 16);
-        bi += __shfl_xor(bi, 32);
+        // bin b's tau from lane b G, its intensity from the sum over its G lanes
+        const double isum = group_scan_add(run[64 + lane], lane, G);
+        const int b0l = (lane < nbp ? lane : nbp - 1) << lg_g;
+        bi = __shfl(isum, b0l + G - 1);
+        bt64 = __shfl(run[lane], b0l);
     }
     double bt = 0.0, bp = 1.0;   // bins beyond the scene's own: tau 0, P 1, I 0
     double sums[4] = {0.0, 0.0, 0.0, 0.0};   // sum over bins of w, w tau, w P, I dE
     if (lane < nb) {
         if constexpr (PREC == HADROS_PREC_FP64) {
-            bt = a64.tau[lane];
+            bt = bt64;
         } else {
             const Kahan kt = a32.tau[lane];
             const ScaledKahan ki = a32.I[lane];
@@ -339,7 +394,6 @@ __attribute__((amdgpu_waves_per_eu(MEV ? 2 : HD_REPLAY_WAVES_PER_EU)))   // the 
 k_replay(ReplayArgs A)
 {
     extern __shared__ double smem[];
-    __shared__ ReplayAcc64 acc64[REPLAY_WAVES];
     __shared__ ReplayAcc32 acc32[REPLAY_WAVES];
     // per-wavefront bin constants (the batch source changes scene per ray),
     // indexed by the rolled bin loop
@@ -363,11 +417,11 @@ k_replay(ReplayArgs A)
             bin_de[lane] = P.de[lane];
         }
     }
-    // dynamic LDS: the FP64 emission partial sums (replay_emission_slots per
-    // wavefront), the sigma table (shared by every scene), then the batch's
+    // dynamic LDS: the FP64 form's staged points and emission exponents
+    // (replay_emission_slots per wavefront), the sigma table (shared by every scene), then the batch's
     // camera start table
     const int n_em = replay_emission_slots(PREC, SRC == REPLAY_SRC_BATCH ? A.nb_out : A.rt.nb);
-    double* em = smem + wv * n_em;
+    double* wsp = smem + wv * n_em;
     double* sig = smem + REPLAY_WAVES * n_em;
     const int n = A.rt.n_sigma;
     const double* lnE = sig;
@@ -389,7 +443,6 @@ k_replay(ReplayArgs A)
         for (int t = threadIdx.x; t <= A.n_cams; t += blockDim.x) pix0_lds[t] = A.pix0[t];
     __syncthreads();
 
-    ReplayAcc64& a64 = acc64[wv];
     ReplayAcc32& a32 = acc32[wv];
     // A.rt where it lies in the kernel argument segment (the kernel's only
     // explicit argument starts at offset 0; offsetof needs a standard layout)
@@ -409,13 +462,13 @@ k_replay(ReplayArgs A)
             int p, q;
             batch_pixel(C, gq - pix0[c], p, q);
             const long g = pix0[c] + long(p) * C.sub_ny + q;
-            pts_done += replay_ray<PREC, SRC, MEV>(A, (DevRtC)(A.rts + C.pad_rt), ray, g, A.nb_out, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, em);
+            pts_done += replay_ray<PREC, SRC, MEV>(A, (DevRtC)(A.rts + C.pad_rt), ray, g, A.nb_out, lane, a32, bin_e, bin_lne, bin_w,
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, wsp);
         } else {
             // the chunk source may cover a range of the tracer's rays: ray0 + ray
             const long gr = (SRC == REPLAY_SRC_CHUNKS) ? A.ray0 + ray : ray;
-            pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, gr, gr, A.rt.nb, lane, a64, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, em);
+            pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, gr, gr, A.rt.nb, lane, a32, bin_e, bin_lne, bin_w,
+                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, wsp);
         }
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);

@@ -87,6 +87,22 @@ __device__ __forceinline__ void row_scan_add(double (&x)[N])
 #undef HD_SHIFT_STEP
 }
 
+// Inclusive prefix sums within groups of G consecutive lanes (G a power of
+// two from 1 to 64, the same on every lane): lane l ends with the sum over
+// lanes (l & ~(G-1)) .. l.  Row shifts by 1, 2, 4, 8 masked to the group,
+// then for groups of 32 and 64 lanes the row-15 and row-31 broadcasts.
+__device__ __forceinline__ double group_scan_add(double x, int lane, int G)
+{
+    const int gl = lane & (G - 1);
+    if (G > 1) { const double d = dpp_shift<0x111>(x); x += (gl >= 1) ? d : 0.0; }
+    if (G > 2) { const double d = dpp_shift<0x112>(x); x += (gl >= 2) ? d : 0.0; }
+    if (G > 4) { const double d = dpp_shift<0x114>(x); x += (gl >= 4) ? d : 0.0; }
+    if (G > 8) { const double d = dpp_shift<0x118>(x); x += (gl >= 8) ? d : 0.0; }
+    if (G > 16) x += dpp_move<0x142, 0xa>(x);   // row_bcast:15 into rows 1, 3
+    if (G > 32) x += dpp_move<0x143, 0xc>(x);   // row_bcast:31 into rows 2, 3
+    return x;
+}
+
 // inclusive prefix maxima (identity -inf)
 template <int N>
 __device__ __forceinline__ void wave_scan_max(float (&x)[N])
@@ -143,6 +159,16 @@ __device__ __forceinline__ double lane63(double v)
 }
 
 __device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+
+// a value the same on every lane (a wave-uniform load), moved to scalar
+// registers so that loops bounded by it are counted in scalar code
+__device__ __forceinline__ long uniform_long(long v)
+{
+    const unsigned long long u = static_cast<unsigned long long>(v);
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(u & 0xffffffffull)));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(u >> 32)));
+    return static_cast<long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
 
 // value of lane `l` (wave-uniform) for every lane
 __device__ __forceinline__ double read_lane(double v, int l)
