@@ -128,7 +128,8 @@ __global__ void k_eval_step(double a, int kind, double h, const double* y, doubl
 // the split tracer, the cache replay and the batch) at points (r, theta, dl,
 // z), through the device functions it runs: point_q with exp_tab and log_tab
 // (density, source factor, redshift), the sigma lookup the scene's table selects
-// (exact-uniform or bracketing) with exp_tab, and the emission exponent.  (The
+// (exact-uniform or bracketing) with exp_tab, or for a power-law table n_N dl
+// z^s sigma(E_b) (FP64), and the emission exponent.  (The
 // fused tracer's library-exp forms are the ones k_eval_rho, k_eval_sigma and
 // k_eval_emis run.)  out[EVAL_TRANSFER_COLS k ..] = ok, z, ln z,
 // g3Adl, nbar_dl, rho, then per bin dtau_b, then per bin the emission term
@@ -169,8 +170,12 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
             const double E = P.emid[b] * pq.z;
             if constexpr (PREC == HADROS_PREC_FP64) {
                 const double lnEb = P.lnemid[b] + pq.lnz;
-                dt = bin_dtau64(exact, ok && pq.rho > 0.0, E, lnEb, pq.nbar_dl, sig, sig + n_s, sig + 2 * n_s, n_s,
-                                P.sig_lo, P.sig_inv_d, P.e_lo, P.e_hi, exp_t);
+                if (P.sig_pow)
+                    dt = bin_dtau64_pow(ok && pq.rho > 0.0, E, pq.nbar_dl * exp_tab(P.sig_s * pq.lnz, exp_t),
+                                        P.sigb[b], P.e_lo, P.e_hi);
+                else
+                    dt = bin_dtau64(exact, ok && pq.rho > 0.0, E, lnEb, pq.nbar_dl, sig, sig + n_s, sig + 2 * n_s,
+                                    n_s, P.sig_lo, P.sig_inv_d, P.e_lo, P.e_hi, exp_t);
                 if (ok && P.spec_ok && pq.g3Adl != 0.0)
                     em = pq.g3Adl * exp_tab(bin_emis_arg(P.gam, P.invcut, E, lnEb), exp_t);
             } else {

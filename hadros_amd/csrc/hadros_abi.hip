@@ -159,6 +159,28 @@ void sigma_guess(const double* se, int n, double& lo0, double& inv_d, int* exact
     if (exact) *exact = uniform && dev <= 1.0e-12 * d;
 }
 
+// A table whose every node lies within 1e-13 (in ln sigma) of the line through
+// its end nodes in ln E - ln sigma is one power law sigma_0 (E / E_0)^s, and its
+// log-log interpolation is that power law: sigma(E z) = sigma(E) z^s.  The
+// transfer kernel then takes z^s once per path point and one product per
+// (point, bin) instead of a table lookup and an exponential.
+void sigma_powerlaw(const double* se, const double* ss, int n, int& pow_ok, double& s)
+{
+    const double l0 = std::log(se[0]), s0 = std::log(ss[0]);
+    s = (std::log(ss[n - 1]) - s0) / (std::log(se[n - 1]) - l0);
+    bool ok = std::isfinite(s);
+    for (int k = 0; ok && k < n; ++k) ok = std::fabs(std::log(ss[k]) - (s0 + s * (std::log(se[k]) - l0))) <= 1.0e-13;
+    pow_ok = ok;
+}
+
+// sigma at E_mid of every bin of a power-law table (sigma_powerlaw)
+void sigma_bins(DevRt& P, const double* se, const double* ss)
+{
+    if (!P.sig_pow) return;
+    const double l0 = std::log(se[0]), s0 = std::log(ss[0]);
+    for (int i = 0; i < P.nb; ++i) P.sigb[i] = std::exp(s0 + P.sig_s * (P.lnemid[i] - l0));
+}
+
 // DevRt of one scene: integrate_kerr_ray_spectral's bins (src/radiative_transfer.cpp:687-761),
 // spectral factor, r_g, horizon of the transfer, sigma range.
 int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, double a_bh, DevRt& P)
@@ -192,6 +214,7 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
     P.e_lo = se[0];
     P.e_hi = se[n - 1];
     sigma_guess(se, n, P.sig_lo, P.sig_inv_d, &P.sig_exact);
+    sigma_powerlaw(se, ss, n, P.sig_pow, P.sig_s);
     P.rg_cm = G_CGS * s.m_bh_msun * MSUN_G / (C_CGS * C_CGS);
     const double a = dclamp(a_bh, 0.0, 0.999999);
     P.r_h_rt = 1.0 + std::sqrt(1.0 - a * a);
@@ -206,6 +229,7 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
         P.lnemid[0] = std::log(s.enu_obs_GeV);
         P.de[0] = 1.0;
         P.w[0] = 1.0;
+        sigma_bins(P, se, ss);
         return HADROS_OK;
     }
     P.gam = sp.gamma;
@@ -238,6 +262,7 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
         P.w[P.nb] = w;
         ++P.nb;
     }
+    sigma_bins(P, se, ss);
     return HADROS_OK;
 }
 

@@ -74,8 +74,10 @@ struct DevRt {
     double sig_lo;        // ln E of the first table node
     double sig_inv_d;     // (n - 1) / (ln E_last - ln E_first) when the nodes are uniform in ln E, else 0
     int sig_exact;        // the nodes lie within 1e-12 of the spacing of that uniform grid
-    int sig_pad;
+    int sig_pow;          // every node lies on one power law sigma = sigma_0 (E / E_0)^sig_s
+    double sig_s;         // its exponent
     double emid[HADROS_MAX_BINS], lnemid[HADROS_MAX_BINS], de[HADROS_MAX_BINS], w[HADROS_MAX_BINS];
+    double sigb[HADROS_MAX_BINS];   // sigma at each bin's E_mid, for sig_pow tables
 };
 
 struct DevCam {
@@ -1081,6 +1083,13 @@ __device__ __forceinline__ double bin_dtau64(bool exact, bool absorbs, double E,
                             : sigma_ln(lnE, lnS, slope, n, lnEb, sig_lo, sig_inv_d);
     const double sig_dl = nbar_dl * exp_tab_nz(fmax(ls, EXP_ZERO_BELOW), T);
     return (absorbs && E >= e_lo && E <= e_hi) ? sig_dl : 0.0;
+}
+// The same for a power-law table (DevRt::sig_pow): wdl = n_N dl z^s staged per
+// point, sig_b = sigma(E_mid) of the bin, E = E_mid z for the table-range test
+__device__ __forceinline__ double bin_dtau64_pow(bool absorbs, double E, double wdl, double sig_b, double e_lo,
+                                                 double e_hi)
+{
+    return (absorbs && E >= e_lo && E <= e_hi) ? wdl * sig_b : 0.0;
 }
 // FP32 form: e^(ln(n_N dl) + ln sigma), lN = ln(n_N dl)
 __device__ __forceinline__ float bin_dtau32(bool exact, bool absorbs, double E, float lnEb, float lN,

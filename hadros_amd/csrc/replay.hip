@@ -103,7 +103,7 @@ template <int PREC, int SRC, bool MEV>
 __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, DevRtC Pc, long ray, long g,
                                                          int nb_out, int lane, ReplayAcc32& a32,
                                                          double* bin_e, double* bin_lne, double* bin_w,
-                                                         double* bin_de, const double* lnE, const double* lnS,
+                                                         double* bin_de, double* bin_sig, const double* lnE, const double* lnS,
                                                          const double* slope, const float* lnEf, const float* lnSf,
                                                          const float* slopef, const double* exp_t, const double* log_t,
                                                          double* wsp)
@@ -129,6 +129,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             bin_lne[lane] = P.lnemid[lane];
             bin_w[lane] = P.w[lane];
             bin_de[lane] = P.de[lane];
+            bin_sig[lane] = P.sigb[lane];
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -216,13 +217,16 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
         }
         if constexpr (PREC == HADROS_PREC_FP64) {
             // stage the chunk's points (lane = point); z carries the
-            // absorption flag in its sign (z > 0 always)
+            // absorption flag in its sign (z > 0 always).  A power-law sigma
+            // table stages n_N dl z^s, so that n_N sigma(E_b z) dl =
+            // n_N dl z^s sigma(E_b)
             const bool absorbs = ok && pq.rho > 0.0;
             const bool emits = ok && P.spec_ok && pq.g3Adl != 0.0;
+            const bool spow = P.sig_pow != 0;
             const int si = replay_stage_index(lane);
             wsp[si] = absorbs ? pq.z : -pq.z;
             wsp[RP_STAGE + si] = pq.lnz;
-            wsp[2 * RP_STAGE + si] = pq.nbar_dl;
+            wsp[2 * RP_STAGE + si] = spow ? pq.nbar_dl * exp_tab(P.sig_s * pq.lnz, exp_t) : pq.nbar_dl;
             wsp[3 * RP_STAGE + si] = emits ? pq.g3Adl : 0.0;
             const bool any_emit = __ballot(emits) != 0ull;
             __builtin_amdgcn_wave_barrier();
@@ -231,6 +235,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             const double sig_lo = P.sig_lo, sig_inv_d = P.sig_inv_d;
             const bool exact = P.sig_exact != 0;
             const double eb = bin_on ? bin_e[bl] : 1.0, lneb = bin_on ? bin_lne[bl] : 0.0;
+            const double sgb = bin_on ? bin_sig[bl] : 0.0;
             const long left = b1 - base;
             const int cnt = left < 64 ? int(left) : 64;
             const int part = RP_SEG << lg_g;   // points per pass: 64, or 32 with 16 bins
@@ -242,16 +247,28 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 // pass 1: optical depth in path order; each point's emission
                 // exponent less tau before it within the segment
                 double loc = 0.0;
+                if (spow) {
 #pragma unroll 2
-                for (int k = 0; k < lc; ++k) {
-                    const int qi = replay_stage_index(q0 + k);
-                    const double zs = wsp[qi], lnz = wsp[RP_STAGE + qi], nbdl = wsp[2 * RP_STAGE + qi];
-                    const double E = eb * fabs(zs);
-                    const double lnEb = lneb + lnz;
-                    const double dt = bin_dtau64(exact, bin_on && zs > 0.0, E, lnEb, nbdl, lnE, lnS, slope, n,
-                                                 sig_lo, sig_inv_d, e_lo, e_hi, exp_t);
-                    ey[k * 64 + lane] = bin_emis_arg(gam, invcut, E, lnEb) - loc;
-                    loc += dt;
+                    for (int k = 0; k < lc; ++k) {
+                        const int qi = replay_stage_index(q0 + k);
+                        const double zs = wsp[qi], lnz = wsp[RP_STAGE + qi], wdl = wsp[2 * RP_STAGE + qi];
+                        const double E = eb * fabs(zs);
+                        const double dt = bin_dtau64_pow(bin_on && zs > 0.0, E, wdl, sgb, e_lo, e_hi);
+                        ey[k * 64 + lane] = bin_emis_arg(gam, invcut, E, lneb + lnz) - loc;
+                        loc += dt;
+                    }
+                } else {
+#pragma unroll 2
+                    for (int k = 0; k < lc; ++k) {
+                        const int qi = replay_stage_index(q0 + k);
+                        const double zs = wsp[qi], lnz = wsp[RP_STAGE + qi], nbdl = wsp[2 * RP_STAGE + qi];
+                        const double E = eb * fabs(zs);
+                        const double lnEb = lneb + lnz;
+                        const double dt = bin_dtau64(exact, bin_on && zs > 0.0, E, lnEb, nbdl, lnE, lnS, slope, n,
+                                                     sig_lo, sig_inv_d, e_lo, e_hi, exp_t);
+                        ey[k * 64 + lane] = bin_emis_arg(gam, invcut, E, lnEb) - loc;
+                        loc += dt;
+                    }
                 }
                 const double incl = group_scan_add(loc, lane, G);
                 const double tot = __shfl(incl, lane | (G - 1));
@@ -397,7 +414,7 @@ k_replay(ReplayArgs A)
     __shared__ ReplayAcc32 acc32[REPLAY_WAVES];
     // per-wavefront bin constants (the batch source changes scene per ray),
     // indexed by the rolled bin loop
-    __shared__ double bin_tab[REPLAY_WAVES][4][HADROS_MAX_BINS];
+    __shared__ double bin_tab[REPLAY_WAVES][5][HADROS_MAX_BINS];
     __shared__ double exp_t[64];   // 2^(j/64) for exp_tab
     __shared__ __align__(16) double log_t[2 * LOG_TAB_N];   // (1/F, ln F) for log_tab
     if (threadIdx.x < 64) exp_t[threadIdx.x] = EXP_TAB[threadIdx.x];
@@ -408,6 +425,7 @@ k_replay(ReplayArgs A)
     double* bin_lne = bin_tab[wv][1];
     double* bin_w = bin_tab[wv][2];
     double* bin_de = bin_tab[wv][3];
+    double* bin_sig = bin_tab[wv][4];
     if constexpr (SRC != REPLAY_SRC_BATCH) {
         const DevRt& P = A.rt;
         if (lane < HADROS_MAX_BINS) {
@@ -415,6 +433,7 @@ k_replay(ReplayArgs A)
             bin_lne[lane] = P.lnemid[lane];
             bin_w[lane] = P.w[lane];
             bin_de[lane] = P.de[lane];
+            bin_sig[lane] = P.sigb[lane];
         }
     }
     // dynamic LDS: the FP64 form's staged points and emission exponents
@@ -463,12 +482,12 @@ k_replay(ReplayArgs A)
             batch_pixel(C, gq - pix0[c], p, q);
             const long g = pix0[c] + long(p) * C.sub_ny + q;
             pts_done += replay_ray<PREC, SRC, MEV>(A, (DevRtC)(A.rts + C.pad_rt), ray, g, A.nb_out, lane, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, wsp);
+                                                   bin_de, bin_sig, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, wsp);
         } else {
             // the chunk source may cover a range of the tracer's rays: ray0 + ray
             const long gr = (SRC == REPLAY_SRC_CHUNKS) ? A.ray0 + ray : ray;
             pts_done += replay_ray<PREC, SRC, MEV>(A, rt_c, gr, gr, A.rt.nb, lane, a32, bin_e, bin_lne, bin_w,
-                                                   bin_de, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, wsp);
+                                                   bin_de, bin_sig, lnE, lnS, slope, lnEf, lnSf, slopef, exp_t, log_t, wsp);
         }
     }
     if (lane == 0 && A.counters) atomicAdd(A.counters, pts_done);
