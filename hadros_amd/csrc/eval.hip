@@ -129,7 +129,8 @@ __global__ void k_eval_step(double a, int kind, double h, const double* y, doubl
 // z), through the device functions it runs: point_q with exp_tab and log_tab
 // (density, source factor, redshift), the sigma lookup the scene's table selects
 // (exact-uniform or bracketing) with exp_tab, or for a power-law table n_N dl
-// z^s sigma(E_b) (FP64), and the emission exponent.  (The
+// z^s sigma(E_b) (FP32K: e^(ln(n_N dl z^s) + ln sigma(E_b))), and the emission
+// exponent.  (The
 // fused tracer's library-exp forms are the ones k_eval_rho, k_eval_sigma and
 // k_eval_emis run.)  out[EVAL_TRANSFER_COLS k ..] = ok, z, ln z,
 // g3Adl, nbar_dl, rho, then per bin dtau_b, then per bin the emission term
@@ -182,8 +183,15 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
                 const float lnEb = float(P.lnemid[b]) + float(pq.lnz);
                 const float lN = (pq.nbar_dl > 0.0) ? float(log_tab(pq.nbar_dl, log_t)) : 0.0f;
                 const float lP = (pq.g3Adl > 0.0) ? float(log_tab(pq.g3Adl, log_t)) : 0.0f;
-                dt = bin_dtau32(exact, ok && pq.rho > 0.0 && pq.nbar_dl > 0.0, E, lnEb, lN, sigf, sigf + n_s,
-                                sigf + 2 * n_s, n_s, float(P.sig_lo), float(P.sig_inv_d), P.e_lo, P.e_hi);
+                const bool ab = ok && pq.rho > 0.0 && pq.nbar_dl > 0.0;
+                if (P.sig_pow) {
+                    const double lnb = (pq.nbar_dl > 0.0) ? log_tab(pq.nbar_dl, log_t) : 0.0;
+                    const float lW = float(lnb + P.sig_s * pq.lnz);
+                    dt = (ab && E >= P.e_lo && E <= P.e_hi) ? __expf(lW + logf(float(P.sigb[b]))) : 0.0f;
+                } else {
+                    dt = bin_dtau32(exact, ab, E, lnEb, lN, sigf, sigf + n_s, sigf + 2 * n_s, n_s, float(P.sig_lo),
+                                    float(P.sig_inv_d), P.e_lo, P.e_hi);
+                }
                 em = (ok && P.spec_ok && pq.g3Adl > 0.0)
                          ? double(bin_emis_log32(lP, float(P.gam), float(P.invcut), E, lnEb))
                          : -HUGE_VAL;

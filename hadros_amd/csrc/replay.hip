@@ -34,11 +34,10 @@ constexpr int REPLAY_BLOCK = 256;
 typedef const __attribute__((address_space(4))) DevRt* DevRtC;
 constexpr int REPLAY_WAVES = REPLAY_BLOCK / 64;
 
-// FP32 form: per-wavefront, per-bin running sums of the current ray, kept in
-// LDS so that the bin loop stays rolled (few registers, more resident
-// wavefronts).  Lane 63 holds each chunk's inclusive totals and alone updates
-// them; LDS operations of one wavefront complete in order, so the next
-// bin/chunk reads see the update.
+// FP32 form: per-wavefront, per-bin running sums of the current ray (Kahan
+// optical depth, intensity in log form), kept in LDS.  The last lane of each
+// bin's lanes (the bin-major layout below) alone updates them; LDS operations
+// of one wavefront complete in order, so the next pass reads the update.
 static_assert(HADROS_MAX_BINS <= 16, "the ray-end reduction keeps one bin per lane of a 16-lane row");
 struct ReplayAcc32 {
     Kahan tau[HADROS_MAX_BINS];
@@ -57,8 +56,9 @@ __device__ __forceinline__ const DevRt& scene_per_chunk(DevRtC Pc)
     return *(const DevRt*)Pc;
 }
 
-// FP64 form, bin-major.  After a chunk's per-point pass (lane = point) the
-// points' (z, ln z, n_N dl, g^3 A dl) go to LDS, and the bins take the lanes:
+// Bin-major transfer.  After a chunk's per-point pass (lane = point) the
+// points' (z, ln z, n_N dl, g^3 A dl; the FP32 form stages their logarithms)
+// go to LDS, and the bins take the lanes:
 // with NBP = nb rounded up to a power of two, bin b owns the G = 64 / NBP
 // lanes b G .. b G + G - 1, and lane s of them the chunk's points
 // s Lc .. s Lc + Lc - 1 (Lc = ceil(points / G), so a partial last chunk is
@@ -66,9 +66,12 @@ __device__ __forceinline__ const DevRt& scene_per_chunk(DevRtC Pc)
 // halves, so that Lc <= RP_SEG).  A lane sums its points' optical depths in path
 // order, a prefix sum over the bin's G lanes gives tau before each segment,
 // and a second pass over the segment weights each point's emission by
-// e^{-tau before it}.  Each lane's running tau and emission partial sum live
-// in LDS across the ray (registers held through the per-point pass would
-// spill); the bin's G emission sums are added at the ray's end.
+// e^{-tau before it}.  In the FP64 form each lane's running tau and emission
+// partial sum live in LDS across the ray (registers held through the
+// per-point pass would spill), and the bin's G emission sums are added at the
+// ray's end; the FP32 form adds each pass's optical depth to its bin's Kahan
+// sum and the pass's log-sum-exp of the emission terms to its bin's
+// intensity.
 // Staged arrays: 64 points with one pad double after every 8 (a lane's
 // segment start is then bank-spread), 72 doubles each.
 constexpr int RP_STAGE = 72;
@@ -79,14 +82,13 @@ __host__ __device__ constexpr int replay_bins_pow2(int nb)
     return nb <= 1 ? 1 : nb <= 2 ? 2 : nb <= 4 ? 4 : nb <= 8 ? 8 : 16;
 }
 
-// Doubles of dynamic LDS per wavefront: the FP64 form's four staged point
-// arrays, per lane one emission exponent per point of its segment, and per
+// Doubles of dynamic LDS per wavefront: the four staged point arrays, per
+// lane one emission exponent per point of its segment, and (FP64 form) per
 // lane the running tau and emission sum.
 __host__ __device__ constexpr int replay_emission_slots(int precision, int nb)
 {
-    return precision == HADROS_PREC_FP64
-               ? 4 * RP_STAGE + 64 * (replay_bins_pow2(nb) < RP_SEG ? replay_bins_pow2(nb) : RP_SEG) + 128
-               : 0;
+    return 4 * RP_STAGE + 64 * (replay_bins_pow2(nb) < RP_SEG ? replay_bins_pow2(nb) : RP_SEG) +
+           (precision == HADROS_PREC_FP64 ? 128 : 0);
 }
 
 // The batch's camera start table is staged in LDS when it is short and the
@@ -133,8 +135,8 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
         }
     }
     __builtin_amdgcn_wave_barrier();
-    // FP64 bin-major layout of this ray's scene (replay_emission_slots): lane
-    // = (bin bl, segment seg); lanes of bins beyond nb compute on and drop
+    // bin-major layout of this ray's scene (replay_emission_slots): lane =
+    // (bin bl, segment seg); lanes of bins beyond nb compute on and drop
     const int nbp = replay_bins_pow2(nb);
     const int lg_g = 6 - __builtin_ctz(nbp);   // log2 G, G = 64 / nbp lanes per bin
     const int G = 1 << lg_g;
@@ -291,42 +293,80 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
             }
             __builtin_amdgcn_wave_barrier();
         } else {
-            const float lnz = float(pq.lnz);
-            const float lP = (pq.g3Adl > 0.0) ? float(log_tab(pq.g3Adl, log_t)) : 0.0f;
-            const float lN = (pq.nbar_dl > 0.0) ? float(log_tab(pq.nbar_dl, log_t)) : 0.0f;
+            // FP32-Kahan form in the same bin-major layout: staged z (sign =
+            // absorbs), ln z, ln(n_N dl) (ln(n_N dl z^s) for a power-law
+            // table) and ln(g^3 A dl) (-inf where the point does not emit)
+            const bool spow = P.sig_pow != 0;
+            const bool absorbs = ok && pq.rho > 0.0 && pq.nbar_dl > 0.0;
+            const bool emits = ok && P.spec_ok && pq.g3Adl > 0.0;
+            const double lnb = (pq.nbar_dl > 0.0) ? log_tab(pq.nbar_dl, log_t) : 0.0;
+            const int si = replay_stage_index(lane);
+            wsp[si] = absorbs ? pq.z : -pq.z;
+            wsp[RP_STAGE + si] = double(float(pq.lnz));
+            wsp[2 * RP_STAGE + si] = double(float(spow ? lnb + P.sig_s * pq.lnz : lnb));
+            wsp[3 * RP_STAGE + si] = emits ? double(float(log_tab(pq.g3Adl, log_t))) : double(-INFINITY);
+            const bool any_emit = __ballot(emits) != 0ull;
+            __builtin_amdgcn_wave_barrier();
             const float gam = float(P.gam), invcut = float(P.invcut);
             const double e_lo = P.e_lo, e_hi = P.e_hi;
             const float sig_lo = float(P.sig_lo), sig_inv_d = float(P.sig_inv_d);
             const bool exact = P.sig_exact != 0;
-            const bool absorbs = ok && pq.rho > 0.0 && pq.nbar_dl > 0.0;
-#pragma unroll 1
-            for (int b = 0; b < nb; ++b) {
-                const double E = bin_e[b] * pq.z;
-                const float lnEb = float(bin_lne[b]) + lnz;
-                const float dt = bin_dtau32(exact, absorbs, E, lnEb, lN, lnEf, lnSf, slopef, n, sig_lo, sig_inv_d,
-                                            e_lo, e_hi);
-                float inc[1] = {dt};
-                wave_scan_add(inc);
-                const float before = a32.tau[b].s + (inc[0] - dt);
-                // emission in log form; the wavefront sum of e^{L - m} uses the chunk maximum m
-                float L = -INFINITY;
-                if (ok && P.spec_ok && pq.g3Adl > 0.0) L = bin_emis_log32(lP, gam, invcut, E, lnEb) - before;
-                // terms below e^EXP32_LOG_NEGLIGIBLE vanish in the FP64 result or
-                // beside any larger term: a chunk of only such terms is skipped
-                const bool live = __ballot(!(L < EXP32_LOG_NEGLIGIBLE)) != 0ull;
-                float mx = -INFINITY, w[1] = {0.0f};
-                if (live) {
-                    float m[1] = {L};
-                    wave_scan_max(m);
-                    mx = lane63(m[0]);
-                    w[0] = (mx > -INFINITY) ? __expf(L - mx) : 0.0f;
-                    wave_scan_add(w);
+            const double eb = bin_on ? bin_e[bl] : 1.0;
+            const float lneb = bin_on ? float(bin_lne[bl]) : 0.0f;
+            const float lsb = (bin_on && spow) ? logf(float(bin_sig[bl])) : 0.0f;
+            const bool last = (lane & (G - 1)) == G - 1;   // the lane that updates its bin's sums
+            const long left = b1 - base;
+            const int cnt = left < 64 ? int(left) : 64;
+            const int part = RP_SEG << lg_g;
+            double* ey = wsp + 4 * RP_STAGE;   // per point of the segment: emission log less tau before it
+            for (int s0 = 0; s0 < cnt; s0 += part) {
+                const int c2 = (cnt - s0 < part) ? cnt - s0 : part;
+                const int lc = (c2 + G - 1) >> lg_g;
+                const int q0 = s0 + (lane & (G - 1)) * lc;
+                float loc = 0.0f;
+#pragma unroll 2
+                for (int k = 0; k < lc; ++k) {
+                    const int qi = replay_stage_index(q0 + k);
+                    const double zs = wsp[qi];
+                    const float lnz = float(wsp[RP_STAGE + qi]), lN = float(wsp[2 * RP_STAGE + qi]);
+                    const double E = eb * fabs(zs);
+                    const float lnEb = lneb + lnz;
+                    const bool ab = bin_on && zs > 0.0;
+                    const float dt = spow ? ((ab && E >= e_lo && E <= e_hi) ? __expf(lN + lsb) : 0.0f)
+                                          : bin_dtau32(exact, ab, E, lnEb, lN, lnEf, lnSf, slopef, n, sig_lo,
+                                                       sig_inv_d, e_lo, e_hi);
+                    ey[k * 64 + lane] = double(bin_emis_log32(0.0f, gam, invcut, E, lnEb) - loc);
+                    loc += dt;
+                }
+                const float incl = group_scan_f<false>(loc, lane, G);
+                const float tot = __shfl(incl, lane | (G - 1));
+                const float pre = a32.tau[bl].s + (incl - loc);   // tau before the segment
+                if (any_emit) {
+                    // emission in log form: L = ln(g^3 A dl) + ln S(E) - tau before the
+                    // point; the bin's sum of e^{L - m} uses the pass's maximum m
+                    float m = -INFINITY;
+#pragma unroll 2
+                    for (int k = 0; k < lc; ++k) {
+                        const float lp = float(wsp[3 * RP_STAGE + replay_stage_index(q0 + k)]);
+                        const float L = (bin_on && lp > -INFINITY) ? lp + float(ey[k * 64 + lane]) - pre : -INFINITY;
+                        ey[k * 64 + lane] = double(L);
+                        m = fmaxf(m, L);
+                    }
+                    // terms below e^EXP32_LOG_NEGLIGIBLE vanish in the FP64 result or
+                    // beside any larger term: a pass of only such terms is skipped
+                    if (__ballot(!(m < EXP32_LOG_NEGLIGIBLE)) != 0ull) {
+                        const float mx = __shfl(group_scan_f<true>(m, lane, G), lane | (G - 1));
+                        float w = 0.0f;
+                        if (mx > -INFINITY) {
+#pragma unroll 2
+                            for (int k = 0; k < lc; ++k) w += __expf(float(ey[k * 64 + lane]) - mx);
+                        }
+                        const float ws = group_scan_f<false>(w, lane, G);
+                        if (last && bin_on && mx > -INFINITY) a32.I[bl].add_log(mx + __logf(ws));
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
-                if (lane == 63) {
-                    if (mx > -INFINITY) a32.I[b].add_log(mx + __logf(w[0]));
-                    a32.tau[b].add(inc[0]);
-                }
+                if (last && bin_on) a32.tau[bl].add(tot);
                 __builtin_amdgcn_wave_barrier();
             }
         }

@@ -103,6 +103,29 @@ __device__ __forceinline__ double group_scan_add(double x, int lane, int G)
     return x;
 }
 
+// The same for floats, with the sum's or the maximum's identity in lanes
+// that have no source
+template <bool MAX>
+__device__ __forceinline__ float group_scan_f(float x, int lane, int G)
+{
+    const float id = MAX ? -INFINITY : 0.0f;
+    const int gl = lane & (G - 1);
+#define HD_GSTEP(CTRL, RM, S)                                         \
+    {                                                                 \
+        const float d = dpp_move<CTRL, RM>(x, id);                    \
+        const float v = (gl >= (S)) ? d : id;                         \
+        x = MAX ? fmaxf(x, v) : x + v;                                \
+    }
+    if (G > 1) HD_GSTEP(0x111, 0xf, 1)
+    if (G > 2) HD_GSTEP(0x112, 0xf, 2)
+    if (G > 4) HD_GSTEP(0x114, 0xf, 4)
+    if (G > 8) HD_GSTEP(0x118, 0xf, 8)
+    if (G > 16) HD_GSTEP(0x142, 0xa, 0)   // row_bcast:15 into rows 1, 3
+    if (G > 32) HD_GSTEP(0x143, 0xc, 0)   // row_bcast:31 into rows 2, 3
+#undef HD_GSTEP
+    return x;
+}
+
 // inclusive prefix maxima (identity -inf)
 template <int N>
 __device__ __forceinline__ void wave_scan_max(float (&x)[N])
