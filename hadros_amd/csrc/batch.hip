@@ -412,8 +412,14 @@ k_batch_points(BatchPointsArgs A)
                     } else {
                         // ---- path point L.step (src/kerr_camera.cpp:236-273)
                         ++L.npts;
-                        w.put(A.cp, L.y.r, L.y.th, zamo_dl(a, L.y, L.yp),
-                              zamo_energy(a, L.y.r, L.y.th, L.pt, L.pph) * L.inv_e_obs);
+                        // the plain split tracer's point forms (sin, cos of theta
+                        // by sincos_m, the midpoint's from them), so that the two
+                        // round their points alike
+                        double sy, cy;
+                        sincos_m(L.y.th, &sy, &cy);
+                        const double dl = (MODE == HADROS_MODE_CFG) ? zamo_dl_sc(a, L.y, L.yp, sy, cy)
+                                                                    : zamo_dl(a, L.y, L.yp);
+                        w.put(A.cp, L.y.r, L.y.th, dl, zamo_energy_sc(a, L.y.r, sy, cy, L.pt, L.pph) * L.inv_e_obs);
                     }
                 }
             }

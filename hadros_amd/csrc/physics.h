@@ -506,7 +506,9 @@ constexpr double CFG_H_MIN = 1.0e-9;
 __device__ __forceinline__ double inv_fifth_root(double x)
 {
     const double xc = dclamp(x, 1.0e-5, 1.0e6);
-    double y = double(exp2f(-0.2f * log2f(float(xc))));
+    // the hardware log2 and exp2 (no denormal scaling: xc and the seed are
+    // normal floats, where the library forms compute the same bits)
+    double y = double(__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(float(xc))));
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
         const double y2 = y * y;
@@ -573,11 +575,12 @@ __device__ __forceinline__ double wrapped_dphi(double pn, double po)
 // src/kerr_camera.cpp:46-76 with the spatial metric of src/kerr_metric.cpp:
 // 34-75 at the midpoint: g_rr = Sigma / Delta, g_thth = Sigma, g_phph =
 // (r^2 + a^2 + 2 a^2 r s^2 / Sigma) s^2, by one reciprocal of Sigma Delta
-__device__ __forceinline__ double zamo_dl(double a, const Y5& cur, const Y5& prev)
+// s, c: sin and cos of the midpoint's theta.  dl = dl^2 x rsqrt(dl^2), the
+// refined hardware estimate (within an ulp); zero, infinite and NaN dl^2 take
+// the library square root.
+__device__ __forceinline__ double zamo_dl_mid(double a, const Y5& cur, const Y5& prev, double s, double c)
 {
     const double r = 0.5 * (cur.r + prev.r);
-    double s, c;
-    sincos_m(0.5 * (cur.th + prev.th), &s, &c);
     const double a2 = a * a;
     double sig, del;
     sigma_delta(a2, r, c, sig, del);
@@ -589,7 +592,26 @@ __device__ __forceinline__ double zamo_dl(double a, const Y5& cur, const Y5& pre
     const double dth = cur.th - prev.th;
     const double dph = wrapped_dphi(cur.ph, prev.ph);
     const double dl2 = fma(g_rr * dr, dr, fma(sig * dth, dth, g_pp * dph * dph));
+    if (dl2 > 0.0 && dl2 < INFINITY) return dl2 * fast_rsqrt(dl2);
     return sqrt(dmax(dl2, 0.0));
+}
+
+__device__ __forceinline__ double zamo_dl(double a, const Y5& cur, const Y5& prev)
+{
+    double s, c;
+    sincos_m(0.5 * (cur.th + prev.th), &s, &c);
+    return zamo_dl_mid(a, cur, prev, s, c);
+}
+
+// The same with sc, cc = sin, cos of cur.th at hand (sincos_m's): the
+// midpoint's by angle addition over half the step back, or by sincos_m where
+// that half step exceeds sincos_shift's range.  The split tracers (plain and
+// batch) use this form, so they round their points alike.
+__device__ __forceinline__ double zamo_dl_sc(double a, const Y5& cur, const Y5& prev, double sc, double cc)
+{
+    double s, c;
+    if (!sincos_shift(sc, cc, 0.5 * (prev.th - cur.th), &s, &c)) sincos_m(0.5 * (cur.th + prev.th), &s, &c);
+    return zamo_dl_mid(a, cur, prev, s, c);
 }
 
 // ------------------------------------------------------------------ medium

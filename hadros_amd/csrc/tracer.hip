@@ -474,7 +474,8 @@ k_trace_points(PointsArgs A)
         } else {
             z = zamo_energy(a, y.r, y.th, pt, pph) * inv_e_obs;
         }
-        w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), z);
+        if constexpr (MODE == HADROS_MODE_CFG) w.put(A.cp, y.r, y.th, zamo_dl_sc(a, y, yp, sy, cy), z);
+        else w.put(A.cp, y.r, y.th, zamo_dl(a, y, yp), z);
     }
 
     const unsigned long long c_pts = valid ? (unsigned long long)npts : 0ull;
