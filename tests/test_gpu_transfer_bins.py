@@ -110,3 +110,21 @@ def test_powerlaw_lookup_matches_table_lookup(n_bins):
         x, y = getattr(a, f), getattr(b, f)
         assert rel(x, y, floor=1e-300).max() <= 1e-9 or np.abs(x - y).max() <= 1e-300, f
     assert math.isfinite(a.I.sum()) and a.I.sum() > 0.0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32k"])
+def test_spectrum_outside_table_integrates_nothing(prec):
+    """A spectrum whose range misses the sigma table integrates no bin
+    (src/radiative_transfer.cpp:698-705): every ray keeps the defaults, tau 0,
+    P 1, I 0, as in the oracle."""
+    cam = cfg_camera(nx=12, ny=12, a=0.9, theta_deg=60.0)
+    sc = _scene(8)
+    sc.spectrum.e_min_GeV, sc.spectrum.e_max_GeV = 1.0e13, 1.0e14
+    p = abi.PREC_FP64 if prec == "fp64" else abi.PREC_FP32K
+    img, st = ffi.trace_rt(cam, sc, SIG_E, SIG_S, p, path=abi.PATH_SPLIT)
+    ref, rst = po.trace_image(cam, sc, SIG_E, SIG_S)
+    assert st.n_bins in (0, -1) and rst.n_bins in (0, -1)
+    assert np.array_equal(img.captured, ref.captured)
+    for f in ("tau", "p", "I"):
+        assert np.array_equal(getattr(img, f), getattr(ref, f)), f
+    assert np.all(img.tau == 0.0) and np.all(img.p == 1.0) and np.all(img.I == 0.0)
