@@ -617,13 +617,8 @@ __device__ __forceinline__ double zamo_dl_sc(double a, const Y5& cur, const Y5& 
 // ------------------------------------------------------------------ medium
 __host__ __device__ __forceinline__ double sq(double x) { return x * x; }
 
-// cos of a polar angle through sincos_m (cheaper than the general cos)
-__host__ __device__ __forceinline__ double cos_theta(double th)
-{
-    double s, c;
-    sincos_m<false>(th, &s, &c);
-    return c;
-}
+// cos of a polar angle (fastmath.h:cos_polar; cheaper than the general cos)
+__host__ __device__ __forceinline__ double cos_theta(double th) { return cos_polar(th); }
 
 // The exponential and logarithm of the density and source functions: the
 // math library's (the default), or exp_tab and log_tab over the tables a

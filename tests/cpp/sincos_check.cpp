@@ -1,7 +1,7 @@
-// Host build of hd::sincos_m and hd::sincos_shift (hadros_amd/csrc/fastmath.h)
-// against the long-double sine and cosine: prints the largest absolute error
-// (in units of 2^-53) of each over seeded samples, then the range flags of
-// sincos_shift.
+// Host build of hd::sincos_m, hd::sincos_shift and hd::cos_polar
+// (hadros_amd/csrc/fastmath.h) against the long-double sine and cosine: prints
+// the largest absolute error (in units of 2^-53) of each over seeded samples,
+// the range flags of sincos_shift, then cos_polar's error.
 #include "fastmath.h"
 #include <cmath>
 #include <cstdio>
@@ -34,5 +34,15 @@ int main()
     const bool out = hd::sincos_shift(0.5, 0.8, 0.2500001, &s, &c) || hd::sincos_shift(0.5, 0.8, -0.26, &s, &c) ||
                      hd::sincos_shift(0.5, 0.8, NAN, &s, &c);
     std::printf("err53 %.4f %.4f %.4f %d\n", e_fused, e_plain, e_shift, int(out));
+    // cos_polar over [0, pi], its ends, near pi/2, and angles just outside
+    double e_polar = 0.0;
+    for (int i = 0; i < 1000000; ++i) {
+        const double th = 3.141592653589793 * u(g);
+        e_polar = std::fmax(e_polar, err53(hd::cos_polar(th), cosl((long double)th)));
+    }
+    const double pts[] = {0.0, 3.141592653589793, 1.5707963267948966, 1.5707963267948966 + 1e-12, 1.5707963267948966 - 3e-9,
+                          -1e-3, -2.0, 3.1426, 5.0, 6.28};
+    for (double th : pts) e_polar = std::fmax(e_polar, err53(hd::cos_polar(th), cosl((long double)th)));
+    std::printf("polar %.4f\n", e_polar);
     return 0;
 }

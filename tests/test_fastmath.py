@@ -4,7 +4,8 @@ transfer kernels' table-driven exponential (within 2.5 ulp of the correctly
 rounded value over the finite range, including subnormal results, exact at the
 special values) and logarithm (within 2 ulp over the whole range, subnormal
 arguments included, exact at 1, 2, 0.75 and the special values), and the
-geodesic kernels' sine/cosine and their stage form."""
+geodesic kernels' sine/cosine, their stage form and the densities' cosine of
+a polar angle."""
 import math
 import os
 import shutil
@@ -59,6 +60,8 @@ def test_sincos_and_stage_shift(sincos_check):
     assert e_fused < 1.5 and e_plain < 1.5, out
     assert e_shift < 2.5, out
     assert any_out == 0
+    # cos_polar (the densities' cos theta) over [0, pi], near pi/2 and outside
+    assert out[5] == "polar" and float(out[6]) < 2.5, out
 
 
 @pytest.fixture(scope="module")
