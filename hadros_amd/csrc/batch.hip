@@ -322,6 +322,10 @@ k_batch_points(BatchPointsArgs A)
     w.begin(-1);
     bool drained = false;   // wave-uniform: the queue is empty
     bool direct = false;    // redo the lane's attempt in rkf45_cfg's form (rkf45_try)
+    // sin and cos of the lane's L.y.th (CFG mode), kept from the ray's start or
+    // its last accepted point for the next attempt's first stage, as the plain
+    // split tracer keeps them: the same bits as computing them per attempt
+    double sy = 0.0, cy = 1.0;
     // per-thread points / attempts / captured / exhausted totals, in LDS: they
     // change once per ray, and in registers they would cost the 8 VGPRs that
     // keep the kernel at 4 wavefronts per SIMD
@@ -347,6 +351,7 @@ k_batch_points(BatchPointsArgs A)
                     const long my = long(base) + __popcll(below);
                     if (my < A.n_rays) {
                         batch_init_ray<MODE>(A.cams, A.n_cams, pix0, L, A.ray0 + my);
+                        if constexpr (MODE == HADROS_MODE_CFG) sincos_m(L.y.th, &sy, &cy);
                         w.begin(my);
                         const bool cap0 = batch_inside_stop<MODE>(A.cams[L.cam], L.y.r);
                         if (cap0) L.captured = 1;
@@ -368,7 +373,7 @@ k_batch_points(BatchPointsArgs A)
             const double a = C.a;
             Y5 y4, y5;
             // a discarded angle-addition attempt is redone on the next pass (rkf45_try)
-            if (rkf45_try<MODE>(a, L.y, L.pt, L.pph, L.h, direct, y4, y5)) {
+            if (rkf45_try_sc<MODE>(a, L.y, sy, cy, L.pt, L.pph, L.h, direct, y4, y5)) {
                 ++L.attempts;
                 bool acc;
                 if constexpr (MODE == HADROS_MODE_REF) {
@@ -415,7 +420,6 @@ k_batch_points(BatchPointsArgs A)
                         // the plain split tracer's point forms (sin, cos of theta
                         // by sincos_m, the midpoint's from them), so that the two
                         // round their points alike
-                        double sy, cy;
                         sincos_m(L.y.th, &sy, &cy);
                         const double dl = (MODE == HADROS_MODE_CFG) ? zamo_dl_sc(a, L.y, L.yp, sy, cy)
                                                                     : zamo_dl(a, L.y, L.yp);

@@ -28,9 +28,12 @@ constexpr double PI = 3.141592653589793238462643383279502884;
 constexpr double M_U_G = 1.66053906660e-24;   // atomic mass unit [g]
 
 // Products feeding a sum may fuse into FMAs (one rounding instead of two) in
-// the functions that open with HD_CONTRACT: the CFG integrator, the per-point
-// geometry and the transfer, all compared with the oracle to a tolerance.  The
-// REF-mode integrator keeps the written operation order.
+// the functions that open with HD_CONTRACT: the transfer's terms, compared with
+// the oracle to a tolerance.  The CFG integrator and the per-point geometry
+// write their FMAs out instead (the compiler's choice of which products to
+// fuse depends on the surrounding code, and every kernel that traces a ray
+// must round it alike); the REF-mode integrator keeps the written operation
+// order.
 #define HD_CONTRACT _Pragma("clang fp contract(fast)")
 
 __host__ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
@@ -193,35 +196,43 @@ __device__ __forceinline__ Y5 rhs_ref(double a, const Y5& y, double pt, double p
 // and V = p_phi/sin + a sin p_t, 2 Sigma H = N = Delta p_r^2 + p_th^2 - W^2/Delta + V^2
 // (the contraction g^{mu nu} p_mu p_nu of src/kerr_metric.cpp:77-116 times Sigma).
 // Takes sin and cos of y.th (y.th itself is not read); rhs_cfg computes them.
+// The fused multiply-adds are written out (no contraction pragma), so every
+// kernel that inlines this rounds alike whatever code surrounds it.
 __device__ __forceinline__ Y5 rhs_cfg_sc(double a, const Y5& y, double s, double c, double pt, double pph)
 {
-    HD_CONTRACT
-    const double r = y.r;
-    const double r2 = r * r, a2 = a * a;
-    const double sig = r2 + a2 * c * c;
-    const double del = r2 - 2.0 * r + a2;
+    const double r = y.r, a2 = a * a, apt = a * pt;
+    const double r2 = r * r;
+    const double sig = fma(a2 * c, c, r2);
     const double ra = r2 + a2;
-    const double W = ra * pt + a * pph;
+    const double del = fma(-2.0, r, ra);
+    const double W = fma(ra, pt, a * pph);
     // one reciprocal for 1/s, 1/Sigma, 1/Delta
     const double sd = sig * del;
     const double inv_all = fast_rcp(s * sd);
     const double inv_s = sd * inv_all;
-    const double inv_sig = s * del * inv_all;
-    const double V = pph * inv_s + a * s * pt;
-    const double Wd = W * (s * sig * inv_all);
-    const double N = del * y.pr * y.pr + y.pth * y.pth - W * Wd + V * V;
-    const double ddel = 2.0 * r - 2.0;
-    const double dNr = ddel * y.pr * y.pr - 4.0 * r * pt * Wd + Wd * Wd * ddel;
-    const double Vth = c * (a * pt - pph * inv_s * inv_s);
-    const double dNth = 2.0 * V * Vth;
+    const double s_ia = s * inv_all;          // 1 / (Sigma Delta)
+    const double inv_sig = s_ia * del;
+    const double pinv = pph * inv_s;          // p_phi / sin
+    const double V = fma(apt, s, pinv);
+    const double Wd = W * (sig * s_ia);       // W / Delta
+    const double pr2 = y.pr * y.pr;
+    const double dpr = del * y.pr;
+    const double N = fma(V, V, fma(-W, Wd, fma(dpr, y.pr, y.pth * y.pth)));
+    // dN/dr = Delta' (p_r^2 + (W/Delta)^2) - 4 r p_t W/Delta, Delta' = 2 r - 2
+    const double ddel = fma(2.0, r, -2.0);
+    const double dNr = fma(-4.0 * pt * r, Wd, ddel * fma(Wd, Wd, pr2));
+    // dN/dtheta / 2 = V dV/dtheta, dV/dtheta = cos (a p_t - p_phi / sin^2)
+    const double VVth = V * (c * fma(-pinv, inv_s, apt));
     const double h_is = 0.5 * inv_sig;
-    const double n_over = N * h_is * inv_sig;
+    const double n_over = (N * h_is) * inv_sig;   // N / (2 Sigma^2)
     Y5 d;
-    d.r = del * y.pr * inv_sig;
+    d.r = dpr * inv_sig;
     d.th = y.pth * inv_sig;
-    d.ph = (V * inv_s - a * Wd) * inv_sig;
-    d.pr = -(dNr * h_is - n_over * (2.0 * r));
-    d.pth = -(dNth * h_is - n_over * (-2.0 * a2 * c * s));
+    d.ph = fma(V, inv_s, -a * Wd) * inv_sig;
+    // -dH/dr = -(dN/dr) / (2 Sigma) + N Sigma_r / (2 Sigma^2), Sigma_r = 2 r
+    d.pr = fma(n_over, 2.0 * r, -dNr * h_is);
+    // -dH/dtheta, Sigma_theta = -2 a^2 cos sin
+    d.pth = -fma(n_over, (2.0 * a2) * (c * s), VVth * inv_sig);
     return d;
 }
 
@@ -244,45 +255,52 @@ __device__ __forceinline__ Y5 axpy1(const Y5& y, double a1, const Y5& k1)
     return {y.r + a1 * k1.r, y.th + a1 * k1.th, y.ph + a1 * k1.ph, y.pr + a1 * k1.pr, y.pth + a1 * k1.pth};
 }
 
-// Fehlberg 4(5) attempt (tableau of src/kerr_geodesic.cpp:239-308), stage
-// sums left to right as the reference evaluates them.
-// the stage sums of rkf45 below, with FMA contraction allowed (CFG mode) and
-// the tableau's fractions as rounded constants: h times a constant instead
-// of the reference's (h x numerator) / denominator, a run-time division each
+// y + a1 k1 by fused multiply-adds (the CFG attempt forms)
+__device__ __forceinline__ Y5 axpy1_f(const Y5& y, double a1, const Y5& k1)
+{
+    return {fma(a1, k1.r, y.r), fma(a1, k1.th, y.th), fma(a1, k1.ph, y.ph), fma(a1, k1.pr, y.pr),
+            fma(a1, k1.pth, y.pth)};
+}
+
+// Fehlberg 4(5) attempt (tableau of src/kerr_geodesic.cpp:239-308) in CFG
+// mode: the stage sums of rkf45 below as explicit fused multiply-adds in the
+// reference's left-to-right order, and the tableau's fractions as rounded
+// constants: h times a constant instead of the reference's
+// (h x numerator) / denominator, a run-time division each.  Written out, not
+// contracted by the compiler, so that every kernel rounds an attempt alike.
+#define HD_ST3(F) t.F = fma(a32, k2.F, fma(a31, k1.F, y.F))
+#define HD_ST4(F) t.F = fma(a43, k3.F, fma(a42, k2.F, fma(a41, k1.F, y.F)))
+#define HD_ST5(F) t.F = fma(a54, k4.F, fma(a53, k3.F, fma(a52, k2.F, fma(a51, k1.F, y.F))))
+#define HD_ST6(F) t.F = fma(a65, k5.F, fma(a64, k4.F, fma(a63, k3.F, fma(a62, k2.F, fma(a61, k1.F, y.F)))))
+#define HD_Y4(F) \
+    y4.F = fma(h, fma(-1.0 / 5.0, k5.F, fma(2197.0 / 4104.0, k4.F, fma(1408.0 / 2565.0, k3.F, (25.0 / 216.0) * k1.F))), y.F)
+#define HD_Y5(F)                                                                                                 \
+    y5.F = fma(h,                                                                                                \
+               fma(2.0 / 55.0, k6.F,                                                                             \
+                   fma(-9.0 / 50.0, k5.F,                                                                        \
+                       fma(28561.0 / 56430.0, k4.F, fma(6656.0 / 12825.0, k3.F, (16.0 / 135.0) * k1.F)))), \
+               y.F)
 __device__ __forceinline__ void rkf45_cfg(double a, const Y5& y, double pt, double pph, double h, Y5& y4, Y5& y5)
 {
-    HD_CONTRACT
     const Y5 k1 = rhs_cfg(a, y, pt, pph);
     const double a21 = h * 0.25;
-    const Y5 k2 = rhs_cfg(a, axpy1(y, a21, k1), pt, pph);
+    const Y5 k2 = rhs_cfg(a, axpy1_f(y, a21, k1), pt, pph);
     const double a31 = h * (3.0 / 32.0), a32 = h * (9.0 / 32.0);
     Y5 t;
-#define HD_ST3(F) t.F = y.F + a31 * k1.F + a32 * k2.F
     HD_ST3(r); HD_ST3(th); HD_ST3(ph); HD_ST3(pr); HD_ST3(pth);
     const Y5 k3 = rhs_cfg(a, t, pt, pph);
     const double a41 = h * (1932.0 / 2197.0), a42 = h * (-7200.0 / 2197.0), a43 = h * (7296.0 / 2197.0);
-#define HD_ST4(F) t.F = y.F + a41 * k1.F + a42 * k2.F + a43 * k3.F
     HD_ST4(r); HD_ST4(th); HD_ST4(ph); HD_ST4(pr); HD_ST4(pth);
     const Y5 k4 = rhs_cfg(a, t, pt, pph);
     const double a51 = h * (439.0 / 216.0), a52 = h * -8.0, a53 = h * (3680.0 / 513.0), a54 = h * (-845.0 / 4104.0);
-#define HD_ST5(F) t.F = y.F + a51 * k1.F + a52 * k2.F + a53 * k3.F + a54 * k4.F
     HD_ST5(r); HD_ST5(th); HD_ST5(ph); HD_ST5(pr); HD_ST5(pth);
     const Y5 k5 = rhs_cfg(a, t, pt, pph);
     const double a61 = h * (-8.0 / 27.0), a62 = h * 2.0, a63 = h * (-3544.0 / 2565.0), a64 = h * (1859.0 / 4104.0),
                  a65 = h * (-11.0 / 40.0);
-#define HD_ST6(F) t.F = y.F + a61 * k1.F + a62 * k2.F + a63 * k3.F + a64 * k4.F + a65 * k5.F
     HD_ST6(r); HD_ST6(th); HD_ST6(ph); HD_ST6(pr); HD_ST6(pth);
     const Y5 k6 = rhs_cfg(a, t, pt, pph);
-#define HD_Y4(F) y4.F = y.F + h * (25.0 / 216.0 * k1.F + 1408.0 / 2565.0 * k3.F + 2197.0 / 4104.0 * k4.F - 1.0 / 5.0 * k5.F)
-#define HD_Y5(F) y5.F = y.F + h * (16.0 / 135.0 * k1.F + 6656.0 / 12825.0 * k3.F + 28561.0 / 56430.0 * k4.F - 9.0 / 50.0 * k5.F + 2.0 / 55.0 * k6.F)
     HD_Y4(r); HD_Y4(th); HD_Y4(ph); HD_Y4(pr); HD_Y4(pth);
     HD_Y5(r); HD_Y5(th); HD_Y5(ph); HD_Y5(pr); HD_Y5(pth);
-#undef HD_ST3
-#undef HD_ST4
-#undef HD_ST5
-#undef HD_ST6
-#undef HD_Y4
-#undef HD_Y5
 }
 
 // The same attempt with the five later stages' sin/cos by angle addition
@@ -296,31 +314,27 @@ __device__ __forceinline__ void rkf45_cfg(double a, const Y5& y, double pt, doub
 __device__ __forceinline__ bool rkf45_cfg_shift_sc(double a, const Y5& y, double s0, double c0, double pt, double pph,
                                                    double h, Y5& y4, Y5& y5)
 {
-    HD_CONTRACT
     double sj, cj;
     bool ok = true;
     const Y5 k1 = rhs_cfg_sc(a, y, s0, c0, pt, pph);
     double ph4 = (25.0 / 216.0) * k1.ph, ph5 = (16.0 / 135.0) * k1.ph;
     const double a21 = h * 0.25;
     ok &= sincos_shift(s0, c0, a21 * k1.th, &sj, &cj);
-    const Y5 k2 = rhs_cfg_sc(a, axpy1(y, a21, k1), sj, cj, pt, pph);
+    const Y5 k2 = rhs_cfg_sc(a, axpy1_f(y, a21, k1), sj, cj, pt, pph);
     const double a31 = h * (3.0 / 32.0), a32 = h * (9.0 / 32.0);
     Y5 t;
-#define HD_ST3(F) t.F = y.F + a31 * k1.F + a32 * k2.F
     HD_ST3(r); HD_ST3(pr); HD_ST3(pth);
     ok &= sincos_shift(s0, c0, fma(a32, k2.th, a31 * k1.th), &sj, &cj);
     const Y5 k3 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
     ph4 = fma(1408.0 / 2565.0, k3.ph, ph4);
     ph5 = fma(6656.0 / 12825.0, k3.ph, ph5);
     const double a41 = h * (1932.0 / 2197.0), a42 = h * (-7200.0 / 2197.0), a43 = h * (7296.0 / 2197.0);
-#define HD_ST4(F) t.F = y.F + a41 * k1.F + a42 * k2.F + a43 * k3.F
     HD_ST4(r); HD_ST4(pr); HD_ST4(pth);
     ok &= sincos_shift(s0, c0, fma(a43, k3.th, fma(a42, k2.th, a41 * k1.th)), &sj, &cj);
     const Y5 k4 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
     ph4 = fma(2197.0 / 4104.0, k4.ph, ph4);
     ph5 = fma(28561.0 / 56430.0, k4.ph, ph5);
     const double a51 = h * (439.0 / 216.0), a52 = h * -8.0, a53 = h * (3680.0 / 513.0), a54 = h * (-845.0 / 4104.0);
-#define HD_ST5(F) t.F = y.F + a51 * k1.F + a52 * k2.F + a53 * k3.F + a54 * k4.F
     HD_ST5(r); HD_ST5(pr); HD_ST5(pth);
     ok &= sincos_shift(s0, c0, fma(a54, k4.th, fma(a53, k3.th, fma(a52, k2.th, a51 * k1.th))), &sj, &cj);
     const Y5 k5 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
@@ -328,26 +342,23 @@ __device__ __forceinline__ bool rkf45_cfg_shift_sc(double a, const Y5& y, double
     ph5 = fma(-9.0 / 50.0, k5.ph, ph5);
     const double a61 = h * (-8.0 / 27.0), a62 = h * 2.0, a63 = h * (-3544.0 / 2565.0), a64 = h * (1859.0 / 4104.0),
                  a65 = h * (-11.0 / 40.0);
-#define HD_ST6(F) t.F = y.F + a61 * k1.F + a62 * k2.F + a63 * k3.F + a64 * k4.F + a65 * k5.F
     HD_ST6(r); HD_ST6(pr); HD_ST6(pth);
     ok &= sincos_shift(s0, c0, fma(a65, k5.th, fma(a64, k4.th, fma(a63, k3.th, fma(a62, k2.th, a61 * k1.th)))),
                        &sj, &cj);
     const Y5 k6 = rhs_cfg_sc(a, t, sj, cj, pt, pph);
     ph5 = fma(2.0 / 55.0, k6.ph, ph5);
-#define HD_Y4(F) y4.F = y.F + h * (25.0 / 216.0 * k1.F + 1408.0 / 2565.0 * k3.F + 2197.0 / 4104.0 * k4.F - 1.0 / 5.0 * k5.F)
-#define HD_Y5(F) y5.F = y.F + h * (16.0 / 135.0 * k1.F + 6656.0 / 12825.0 * k3.F + 28561.0 / 56430.0 * k4.F - 9.0 / 50.0 * k5.F + 2.0 / 55.0 * k6.F)
     HD_Y4(r); HD_Y4(th); HD_Y4(pr); HD_Y4(pth);
     HD_Y5(r); HD_Y5(th); HD_Y5(pr); HD_Y5(pth);
     y4.ph = fma(h, ph4, y.ph);
     y5.ph = fma(h, ph5, y.ph);
+    return ok;
+}
 #undef HD_ST3
 #undef HD_ST4
 #undef HD_ST5
 #undef HD_ST6
 #undef HD_Y4
 #undef HD_Y5
-    return ok;
-}
 
 __device__ __forceinline__ bool rkf45_cfg_shift(double a, const Y5& y, double pt, double pph, double h, Y5& y4,
                                                 Y5& y5)
@@ -477,7 +488,6 @@ __device__ __forceinline__ double err_ref(const Y5& a, const Y5& b)
 // so only the winning ratio is divided out.
 __device__ __forceinline__ double err_cfg(const Y5& y, const Y5& y4, const Y5& y5, double tol)
 {
-    HD_CONTRACT
     double nb = 0.0, db = 1.0;
     bool bad = false;
     // bitwise ors and selects: no control flow (a short-circuit || here made

@@ -1,6 +1,9 @@
 """Per-ray comparison of a traced image with the oracle's, under the error
 metrics of the parity tests: rel(tau) where tau_ref > 0; rel(P) where
-P_ref > 1e-300, else |dP|; rel(sum I) per image.  Every ray is bounded:
+P_ref > 1e-300, else |dP|; rel(sum I) per image, relative to the image's
+I-weighted mean of max(1, tau_ref) (a point's emission is attenuated by
+e^{-tau before it}, so a relative tau error e moves I by up to e tau_ref, as
+it moves P).  Every ray is bounded:
 
 - rays whose point count equals the oracle's integrate the same points up to
   rounding and meet the tight bars: rel(tau) <= tol_tau, and for survival
@@ -59,12 +62,15 @@ def compare_images(img, ref, tol_tau=1e-6, tol_p=1e-6, tol_i=1e-6, max_chaotic=0
     dp = np.abs(img.p - ref.p)
     si, sr = img.I.sum(), ref.I.sum()
     ri = abs(si - sr) / abs(sr) if sr != 0 else abs(si)
+    # the sum-I bar's scale: I-weighted mean of max(1, tau_ref)
+    wi = np.abs(ref.I)
+    i_scale = float((wi * np.maximum(1.0, ref.tau)).sum() / wi.sum()) if wi.sum() > 0 else 1.0
     gate = matched & ((rt > tol_tau) | (ep > tol_p))
     rep = dict(rays=int(rt.size), chaotic=float(chaotic.mean()), gate=float(gate.mean()),
                max_rel_tau=float(rt.max()), p99_rel_tau=float(np.percentile(rt, 99)),
                frac_tau_over=float(np.mean(rt > tol_tau)), max_p_err=float(ep.max()),
                p99_p_err=float(np.percentile(ep, 99)), frac_p_over=float(np.mean(ep > tol_p)),
-               max_rel_P=float(ep_plain.max()), rel_sum_I=float(ri))
+               max_rel_P=float(ep_plain.max()), rel_sum_I=float(ri), sum_I_scale=i_scale)
     if rep["chaotic"] > max_chaotic:
         bad.append(f"chaotic fraction {rep['chaotic']} > {max_chaotic}")
     if rep["gate"] > max_gate:
@@ -73,8 +79,8 @@ def compare_images(img, ref, tol_tau=1e-6, tol_p=1e-6, tol_i=1e-6, max_chaotic=0
         bad.append(f"gate rays beyond {GATE_TOL}: rel tau {rt[gate].max()}, P {ep[gate].max()}")
     if chaotic.any() and (rt[chaotic].max() > CHAOS_TAU or dp[chaotic].max() > CHAOS_P):
         bad.append(f"chaotic rays beyond the bars: rel tau {rt[chaotic].max()}, |dP| {dp[chaotic].max()}")
-    if ri > tol_i:
-        bad.append(f"rel sum I {ri} > {tol_i}")
+    if ri > tol_i * i_scale:
+        bad.append(f"rel sum I {ri} > {tol_i} x {i_scale}")
     nb = ref.bin_tau.shape[1] if ref.bin_tau.ndim == 2 else 0
     tight = matched & ~gate
     if nb and tight.any():
