@@ -947,9 +947,27 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
     if (t->split) {
         const DevCam& C = t->pargs.cam;
         const int tiles_y = (C.ny + 7) / 8, tile_rows = (C.nx + 7) / 8;
-        // automatic: segments of at least 4 tile rows each
-        const int nseg = t->seg_req ? std::min(t->seg_req, tile_rows)
-                                    : std::max(1, std::min(hadros_tracer::NSEG, tile_rows / 4));
+        // band i covers tile rows [rows[i], rows[i + 1])
+        static_assert(hadros_tracer::NSEG >= 4, "the automatic bands need four streams");
+        int rows[hadros_tracer::NSEG + 1];
+        int nseg;
+        if (!t->seg_req && tile_rows >= 16) {
+            // automatic: the image's middle rows hold the rays that pass near
+            // the photon orbit (the longest, for a camera aimed at the hole),
+            // so two narrow middle bands finish last, and the outer bands'
+            // transfer (most of the image's points) runs under their tail
+            nseg = 4;
+            rows[0] = 0;
+            rows[1] = (3 * tile_rows) / 8;
+            rows[2] = tile_rows / 2;
+            rows[3] = (5 * tile_rows) / 8;
+            rows[4] = tile_rows;
+        } else {
+            // n equal bands (automatic on small cameras: of at least 4 tile rows)
+            nseg = t->seg_req ? std::min(t->seg_req, tile_rows)
+                              : std::max(1, std::min(hadros_tracer::NSEG, tile_rows / 4));
+            for (int i = 0; i <= nseg; ++i) rows[i] = int(long(tile_rows) * i / nseg);
+        }
         t->n_seg = nseg;
         if (nseg > 1) {
             if (!t->ev_fork) HIPC(hipEventCreateWithFlags(&t->ev_fork, hipEventDisableTiming));
@@ -966,7 +984,7 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
             }
             hipStream_t si = (i == 0) ? s : t->seg_s[i];
             if (i > 0) HIPC(hipStreamWaitEvent(si, t->ev_fork, 0));
-            const int r0 = int(long(tile_rows) * i / nseg), r1 = int(long(tile_rows) * (i + 1) / nseg);
+            const int r0 = rows[i], r1 = rows[i + 1];
             PointsArgs Q = t->pargs;
             Q.tile0 = r0 * tiles_y;
             Q.tile1 = r1 * tiles_y;
@@ -975,7 +993,7 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
         }
         for (int i = 0; i < nseg; ++i) {
             hipStream_t si = (i == 0) ? s : t->seg_s[i];
-            const int r0 = int(long(tile_rows) * i / nseg), r1 = int(long(tile_rows) * (i + 1) / nseg);
+            const int r0 = rows[i], r1 = rows[i + 1];
             // pixels of tile rows [r0, r1): rows [8 r0, 8 r1) of the image
             ReplayArgs R = t->rargs;
             R.ray0 = long(std::min(8 * r0, C.nx)) * C.ny;

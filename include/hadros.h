@@ -383,9 +383,14 @@ int hadros_tracer_set_path(hadros_tracer* t, int path);
    bands' geodesic launches are queued first, then each band's transfer
    launch behind its own geodesic launch: the transfer of the bands whose
    rays finish early runs while the longest rays are still traced.
-   0 (default) = automatic: HADROS_TRACER_MAX_SEGMENTS bands of at least four
-   tile rows each (fewer on small cameras); 1 = one geodesic launch, then one
-   transfer launch.  Counts above the camera's tile rows are cut to them. */
+   0 (default) = automatic: on cameras of at least 16 tile rows, four bands
+   at 0, 3/8, 1/2 and 5/8 of the tile rows, so that the two middle bands,
+   where a camera aimed at the hole has its longest rays (near the photon
+   orbit), hold a quarter of the rows and the outer bands' transfer runs
+   under their tail; on smaller cameras up to HADROS_TRACER_MAX_SEGMENTS
+   equal bands of at least four tile rows each.  n >= 1 = n equal bands
+   (1 = one geodesic launch, then one transfer launch); counts above the
+   camera's tile rows are cut to them. */
 #define HADROS_TRACER_MAX_SEGMENTS 4
 int hadros_tracer_set_segments(hadros_tracer* t, int n_segments);
 int hadros_tracer_run(hadros_tracer* t, void* stream);

@@ -200,16 +200,18 @@ def test_split_pool_grows_and_matches_fused():
 
 def test_split_segments_bitwise_invariant():
     """The split form's tile-row bands are scheduling only: every segment
-    count gives the same image bit for bit, on a ragged camera (70 x 54: a
-    partial last tile row and tile column), in both precision forms, with the
-    thin medium so that every output column is non-trivial.  Counts beyond the
-    camera's tile rows are cut to them; counts outside [0, 4] are rejected."""
-    cam = cfg_camera(nx=70, ny=54)
+    count gives the same image bit for bit, on ragged cameras (a partial last
+    tile row and tile column): 70 x 54 (9 tile rows: the automatic bands are
+    equal ones) and 130 x 44 (17 tile rows: the automatic bands split at 3/8,
+    1/2 and 5/8 of them), in both precision forms, with the thin medium so that
+    every output column is non-trivial.  Counts beyond the camera's tile rows
+    are cut to them; counts outside [0, 4] are rejected."""
     sc = workloads.c2_scene()
     sc.medium.rho0 *= 1e-8
     sc.medium.envelope_rho0 *= 1e-8
     cols = ("tau", "p", "I", "captured", "npoints", "flags", "alpha", "beta", "bin_tau", "bin_p", "bin_i")
-    for prec in (abi.PREC_FP64, abi.PREC_FP32K):
+    for cam, prec in ((cfg_camera(nx=70, ny=54), abi.PREC_FP64), (cfg_camera(nx=70, ny=54), abi.PREC_FP32K),
+                      (cfg_camera(nx=130, ny=44), abi.PREC_FP64)):
         base = None
         for nseg in (1, 0, 2, 3, 4):
             t = ffi.Tracer(cam, sc, SIG_E, SIG_S, prec, path=abi.PATH_SPLIT, segments=nseg)
@@ -226,9 +228,9 @@ def test_split_segments_bitwise_invariant():
                     assert 0.0 < img.p.mean() < 1.0
                     continue
                 for c in cols:
-                    assert np.array_equal(getattr(img, c), getattr(base, c)), (prec, nseg, c)
+                    assert np.array_equal(getattr(img, c), getattr(base, c)), (cam.nx, prec, nseg, c)
             t.close()
-    t = ffi.Tracer(cam, sc, SIG_E, SIG_S, abi.PREC_FP64, path=abi.PATH_SPLIT)
+    t = ffi.Tracer(cfg_camera(nx=70, ny=54), sc, SIG_E, SIG_S, abi.PREC_FP64, path=abi.PATH_SPLIT)
     for bad in (-1, abi.TRACER_MAX_SEGMENTS + 1):
         with pytest.raises(ffi.HadrosError):
             t.set_segments(bad)
