@@ -480,13 +480,13 @@ struct hadros_tracer {
     long capacity = 0;         // chunks
     double* d_pool = nullptr;
     int *d_next = nullptr, *d_nst = nullptr;
-    unsigned long long* d_ctr = nullptr;
+    unsigned long long* d_ctr = nullptr;   // the pool's chunk counter: d_cnt + 4
     double *d_sig = nullptr;
     float* d_sigf = nullptr;
     double *d_tau = nullptr, *d_p = nullptr, *d_I = nullptr, *d_alpha = nullptr, *d_beta = nullptr;
     double *d_bt = nullptr, *d_bp = nullptr, *d_bi = nullptr;
     int *d_cap = nullptr, *d_np = nullptr, *d_flags = nullptr;
-    unsigned long long* d_cnt = nullptr;
+    unsigned long long* d_cnt = nullptr;   // [5]: 4 run counters, then the chunk counter
     DevRt* d_rt = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // split form over segments of tile rows: segment 0 runs on the caller's
@@ -495,6 +495,7 @@ struct hadros_tracer {
     // longest rays (a few tiles near the critical curve, which set the
     // geodesic kernel's length) are still being traced
     static constexpr int NSEG = HADROS_TRACER_MAX_SEGMENTS;
+    bool settled = false;                // tracer_settle has checked the last run's pool
     int seg_req = 0;                     // hadros_tracer_set_segments: 0 = automatic
     int n_seg = 1;                       // segments of the last split run
     hipStream_t seg_s[NSEG] = {};        // [1, NSEG): default priority, created on first use
@@ -862,7 +863,7 @@ int hadros_tracer_create(const hadros_camera* cam, const hadros_scene* scene, co
         (rc = dalloc(&t->d_p, np)) || (rc = dalloc(&t->d_I, np)) || (rc = dalloc(&t->d_alpha, np)) ||
         (rc = dalloc(&t->d_beta, np)) || (rc = dalloc(&t->d_bt, np * nb)) || (rc = dalloc(&t->d_bp, np * nb)) ||
         (rc = dalloc(&t->d_bi, np * nb)) || (rc = dalloc(&t->d_cap, np)) || (rc = dalloc(&t->d_np, np)) ||
-        (rc = dalloc(&t->d_flags, np)) || (rc = dalloc(&t->d_cnt, 4)) || (rc = dalloc(&t->d_rt, 1))) {
+        (rc = dalloc(&t->d_flags, np)) || (rc = dalloc(&t->d_cnt, 5)) || (rc = dalloc(&t->d_rt, 1))) {
         hadros_tracer_destroy(t);
         return rc;
     }
@@ -907,7 +908,7 @@ int tracer_pool(hadros_tracer* t, long extra)
     int rc = 0;
     if ((rc = dalloc(&t->d_pool, size_t(cap) * CHUNK_DOUBLES)) || (rc = dalloc(&t->d_next, size_t(cap)))) return rc;
     if (!t->d_nst && (rc = dalloc(&t->d_nst, size_t(t->npix)))) return rc;
-    if (!t->d_ctr && (rc = dalloc(&t->d_ctr, 1))) return rc;
+    t->d_ctr = t->d_cnt + 4;
     t->capacity = cap;
     t->pargs.cp = ChunkPool{t->d_pool, t->d_next, reinterpret_cast<unsigned*>(t->d_ctr), t->npix, cap};
     t->pargs.n_st = t->d_nst;
@@ -933,8 +934,11 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
 {
     if (t->mev.on && t->path == HADROS_PATH_FUSED)
         return fail(HADROS_ERR_ARG, "the MeV channel runs in the split form: HADROS_PATH_FUSED cannot compute it");
-    HIPC(launch_zero_u64(t->d_cnt, 4, s));
+    // the run counters and the chunk counter, cleared by one fill kernel (no
+    // memset nodes if the caller captures the stream)
+    HIPC(launch_zero_u64(t->d_cnt, 5, s));
     t->stream = s;
+    t->settled = false;
     t->split = want_split(t);
     mev_apply(t->mev, t->rargs);
     t->ran_mev = t->split && t->mev.on;
@@ -942,7 +946,6 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
         const int rc = tracer_pool(t, 2 * t->npix);
         if (rc) return rc;
     }
-    if (t->split) HIPC(launch_zero_u64(t->d_ctr, 1, s));
     HIPC(hipEventRecord(t->ev0, s));
     if (t->split) {
         const DevCam& C = t->pargs.cam;
@@ -1014,6 +1017,8 @@ int tracer_launch(hadros_tracer* t, hipStream_t s)
 // asked for (or give up on the split form) and run again
 int tracer_settle(hadros_tracer* t)
 {
+    // the fetch and timing calls after one run settle it once (one counter copy)
+    if (t->settled) return HADROS_OK;
     HIPC(hipEventSynchronize(t->ev1));
     // the counter holds every extra chunk the rays needed, so one retry suffices;
     // the loop bound only guards against a pool that cannot be allocated
@@ -1021,7 +1026,10 @@ int tracer_settle(hadros_tracer* t)
         unsigned long long req = 0;
         HIPC(hipMemcpy(&req, t->d_ctr, sizeof(req), hipMemcpyDeviceToHost));
         const long need = t->npix + long(unsigned(req));
-        if (need <= t->capacity) return HADROS_OK;
+        if (need <= t->capacity) {
+            t->settled = true;
+            return HADROS_OK;
+        }
         const long grown = long(double(need) * 1.25);
         const double bytes = double(grown) * CHUNK_DOUBLES * sizeof(double);
         int rc = 0;
@@ -1043,6 +1051,7 @@ int tracer_settle(hadros_tracer* t)
         HIPC(hipMemcpy(&req, t->d_ctr, sizeof(req), hipMemcpyDeviceToHost));
         if (t->npix + long(unsigned(req)) > t->capacity) return fail(HADROS_ERR_MAX_POINTS, "split tracer: chunk pool too small");
     }
+    t->settled = true;
     return HADROS_OK;
 }
 
@@ -1194,7 +1203,6 @@ int hadros_tracer_destroy(hadros_tracer* t)
     dfree(t->d_pool);
     dfree(t->d_next);
     dfree(t->d_nst);
-    dfree(t->d_ctr);
     dfree(t->d_rt);
     mev_free(t->mev);
     if (t->ev0) (void)hipEventDestroy(t->ev0);
