@@ -40,6 +40,13 @@ __device__ __forceinline__ bool batch_inside_stop(const DevCam& C, double r)
     else return r <= C.r_h + 1.0e-3;
 }
 
+template <int MODE>
+__device__ __forceinline__ bool batch_inside_stop_rh(double r_h, double r)
+{
+    if constexpr (MODE == HADROS_MODE_CFG) return r < 1.01 * r_h;
+    else return r <= r_h + 1.0e-3;
+}
+
 // global output index of queue ray `ray` of camera `cam` (recomputed when the
 // ray finishes rather than held in registers while it is traced)
 __device__ __forceinline__ long batch_out_index(const DevCam* __restrict__ cams, const long* __restrict__ pix0, int cam,
@@ -374,14 +381,18 @@ k_batch_points(BatchPointsArgs A)
             Y5 y4, y5;
             // a discarded angle-addition attempt is redone on the next pass (rkf45_try)
             if (rkf45_try_sc<MODE>(a, L.y, sy, cy, L.pt, L.pph, L.h, direct, y4, y5)) {
+                // the camera's step-control constants, read together after the
+                // attempt (their loads share one wait instead of one each)
+                const double tol = C.tol, r_max = C.r_max, r_h = C.r_h;
+                const int max_steps = C.max_steps;
                 ++L.attempts;
                 bool acc;
                 if constexpr (MODE == HADROS_MODE_REF) {
                     const double h_min = C.h * 1.0e-5;
                     const double err = err_ref(y4, y5);
-                    acc = (err < C.tol) || (L.h <= h_min);
+                    acc = (err < tol) || (L.h <= h_min);
                     if (!acc) {
-                        const double factor = 0.8 * sqrt(sqrt(C.tol / dmax(err, 1.0e-30)));
+                        const double factor = 0.8 * sqrt(sqrt(tol / dmax(err, 1.0e-30)));
                         L.h *= dclamp(factor, 0.1, 5.0);
                         if (L.h < h_min) L.h = h_min;
                         if (++L.att_in_step == 50) {
@@ -394,10 +405,10 @@ k_batch_points(BatchPointsArgs A)
                         L.att_in_step = 0;
                     }
                 } else {
-                    const double err = err_cfg(L.y, y4, y5, C.tol);
+                    const double err = err_cfg(L.y, y4, y5, tol);
                     acc = (err <= 1.0) | (L.h <= CFG_H_MIN);
                     L.h = fmax(L.h * cfg_step_factor(err, acc), CFG_H_MIN);
-                    if (!acc && L.attempts >= 8L * C.max_steps + 64) {
+                    if (!acc && L.attempts >= 8L * max_steps + 64) {
                         L.exhausted = 1;
                         finished = true;
                     }
@@ -406,13 +417,13 @@ k_batch_points(BatchPointsArgs A)
                     L.yp = L.y;
                     L.y = y5;
                     ++L.step;
-                    if (L.step >= C.max_steps) {
+                    if (L.step >= max_steps) {
                         L.exhausted = 1;
                         finished = true;
-                    } else if (batch_inside_stop<MODE>(A.cams[L.cam], L.y.r)) {
+                    } else if (batch_inside_stop_rh<MODE>(r_h, L.y.r)) {
                         L.captured = 1;
                         finished = true;
-                    } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > C.r_max) : (L.y.r >= C.r_max)) && L.step > 10) {
+                    } else if (((MODE == HADROS_MODE_CFG) ? (L.y.r > r_max) : (L.y.r >= r_max)) && L.step > 10) {
                         finished = true;
                     } else {
                         // ---- path point L.step (src/kerr_camera.cpp:236-273)
