@@ -337,6 +337,9 @@ k_batch_points(BatchPointsArgs A)
     // change once per ray, and in registers they would cost the 8 VGPRs that
     // keep the kernel at 4 wavefronts per SIMD
     __shared__ unsigned long long cnt[256][4];
+    // each lane's redshift factor of its ray (set at the ray's start, read at
+    // every path point): in LDS, the registers it held spilled to scratch
+    __shared__ double inv_e_lds[256];
     unsigned long long* my_cnt = cnt[threadIdx.x];
     my_cnt[0] = my_cnt[1] = my_cnt[2] = my_cnt[3] = 0ull;
     int since_refill = A.steps_per_chunk;
@@ -359,6 +362,7 @@ k_batch_points(BatchPointsArgs A)
                     if (my < A.n_rays) {
                         batch_init_ray<MODE>(A.cams, A.n_cams, pix0, L, A.ray0 + my);
                         if constexpr (MODE == HADROS_MODE_CFG) sincos_m(L.y.th, &sy, &cy);
+                        inv_e_lds[threadIdx.x] = L.inv_e_obs;
                         w.begin(my);
                         const bool cap0 = batch_inside_stop<MODE>(A.cams[L.cam], L.y.r);
                         if (cap0) L.captured = 1;
@@ -434,7 +438,8 @@ k_batch_points(BatchPointsArgs A)
                         sincos_m(L.y.th, &sy, &cy);
                         const double dl = (MODE == HADROS_MODE_CFG) ? zamo_dl_sc(a, L.y, L.yp, sy, cy)
                                                                     : zamo_dl(a, L.y, L.yp);
-                        w.put(A.cp, L.y.r, L.y.th, dl, zamo_energy_sc(a, L.y.r, sy, cy, L.pt, L.pph) * L.inv_e_obs);
+                        w.put(A.cp, L.y.r, L.y.th, dl,
+                              zamo_energy_sc(a, L.y.r, sy, cy, L.pt, L.pph) * inv_e_lds[threadIdx.x]);
                     }
                 }
             }

@@ -204,8 +204,11 @@ int make_rt(const hadros_scene& s, const double* se, const double* ss, int n, do
     P.src.rho_ref = so.rho_ref;
     P.src.cmin = so.cutoff_min;
     P.src.cmax = so.cutoff_max;
-    P.src.gdr = so.gradient_dr;
-    P.src.gdth = so.gradient_dtheta_rad;
+    // the shock layer's finite-difference steps, clamped here as the
+    // source term clamps them (dmax(dr, 1e-4), dmax(dtheta, 1e-5)): the
+    // kernels read them as they are
+    P.src.gdr = dmax(so.gradient_dr, 1.0e-4);
+    P.src.gdth = dmax(so.gradient_dtheta_rad, 1.0e-5);
     P.src.inv_rc = 1.0 / so.r_center;
     P.src.inv_sr = 1.0 / so.sigma_r;
     P.src.inv_tw = 1.0 / so.theta_width_rad;

@@ -55,7 +55,8 @@ struct DevMedium {
 struct DevSource {
     int kind;
     int ring_ok;          // emissivity_collapsar_ring guard (src/radiative_transfer.cpp:343-349)
-    double rc, sr, tw, norm, ft, q, s, rho_ref, cmin, cmax, gdr, gdth;
+    double rc, sr, tw, norm, ft, q, s, rho_ref, cmin, cmax;
+    double gdr, gdth;   // shock-layer steps, already clamped to >= 1e-4 and >= 1e-5
     double rho_norm_c;    // torus.rho(r_center, pi/2), a per-scene constant
     double inv_rc, inv_sr, inv_tw;
 };
@@ -561,7 +562,9 @@ __device__ __forceinline__ double zamo_energy_sc(double a, double r, double s, d
     sigma_delta(a2, r, c, sig, del);
     const double ra = fma(r, r, a2);
     const double inv_A = fast_rcp(fma(ra, ra, -(a2 * del) * (s * s)));
-    const double omega = 2.0 * a * r * inv_A;
+    // 2 (a r), not (2 a) r: the same bits (doubling is exact), and no loop
+    // invariant 2a for a caller's loop to hold in registers
+    const double omega = 2.0 * (a * r) * inv_A;
     return -fma(omega, pph, pt) * fast_rsqrt(sig * del * inv_A);
 }
 
@@ -746,8 +749,8 @@ __device__ __forceinline__ double source_factor(const DevSource& so, const DevMe
         const double rw = pow(dmax(r / dmax(so.rc, 1.0e-300), 1.0e-300), -so.s);
         W = dclamp(dw * rw, dmax(so.cmin, 0.0), dmax(so.cmax, dmax(so.cmin, 0.0)));
     } else if (so.kind == HADROS_SOURCE_SHOCK_LAYER) {
-        const double dr = dmax(so.gdr, 1.0e-4);
-        const double dth = dmax(so.gdth, 1.0e-5);
+        const double dr = so.gdr;     // dmax(dr, 1e-4), clamped on the host
+        const double dth = so.gdth;   // dmax(dtheta, 1e-5), clamped on the host
         // rho at (r +- dr, th) and (r, th +- dth): one density evaluation in a
         // rolled loop (four inlined copies would set the kernels' register use)
         double sr = 0.0, st = 0.0;
