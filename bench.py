@@ -435,17 +435,27 @@ def main():
         t = ffi.Tracer(cam, scene, e, s, precision, args.block, segments=segments)
         for _ in range(args.warmup):
             t.run()
-            t.last_ms()
+            t.last_ms()   # each warmup run settled: the chunk pool has grown to the image's points
         torch.cuda.synchronize()
-        kern, parts = [], []
+        # timed: the K images queued back to back on the launch stream, one
+        # synchronization at each end.  Every run traces the whole image, and
+        # the pool holds every run's points as it held the warmup's (a camera's
+        # rays store the same points on every run); the last run is settled.
         t0 = time.perf_counter()
+        for _ in range(args.steps):
+            t.run()
+        t.last_ms()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        n_pts = t.fetch()[1].total_points
+        # each run's device time and the bands' spans (HIP events), untimed
+        kern, parts = [], []
         for _ in range(args.steps):
             t.run()
             kern.append(t.last_ms())
             parts.append(t.kernel_ms())
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
         img, st = t.fetch()
+        assert st.total_points == n_pts, "runs of one camera stored different point counts"
         t.close()
         return wall, kern, np.mean(np.array(parts), axis=0), img, st
 
