@@ -181,11 +181,12 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
                     em = pq.g3Adl * exp_tab(bin_emis_arg(P.gam, P.invcut, E, lnEb), exp_t);
             } else {
                 const float lnEb = float(P.lnemid[b]) + float(pq.lnz);
-                const float lN = (pq.nbar_dl > 0.0) ? float(log_tab(pq.nbar_dl, log_t)) : 0.0f;
-                const float lP = (pq.g3Adl > 0.0) ? float(log_tab(pq.g3Adl, log_t)) : 0.0f;
+                // the transfer kernel's staged logarithms (ln_f32, rounded to float)
+                const float lN = (pq.nbar_dl > 0.0) ? float(ln_f32(pq.nbar_dl)) : 0.0f;
+                const float lP = (pq.g3Adl > 0.0) ? float(ln_f32(pq.g3Adl)) : 0.0f;
                 const bool ab = ok && pq.rho > 0.0 && pq.nbar_dl > 0.0;
                 if (P.sig_pow) {
-                    const double lnb = (pq.nbar_dl > 0.0) ? log_tab(pq.nbar_dl, log_t) : 0.0;
+                    const double lnb = (pq.nbar_dl > 0.0) ? ln_f32(pq.nbar_dl) : 0.0;
                     const float lW = float(lnb + P.sig_s * pq.lnz);
                     dt = (ab && E >= P.e_lo && E <= P.e_hi) ? __expf(lW + logf(float(P.sigb[b]))) : 0.0f;
                 } else {

@@ -306,6 +306,18 @@ __device__ __forceinline__ double fast_rsqrt(double x)
     }
     return y;
 }
+
+// ln x for finite x > 0 to the accuracy of a float, for logarithms that are
+// then rounded to float: x = m 2^e exactly (frexp, m in [0.5, 1)), and
+// ln x = (e + log2 m) ln 2 with the hardware FP32 log2 of m (about 1e-7
+// absolute, below the float rounding of the result).  About 8 instructions
+// against log_tab's 25 and its table read.
+__device__ __forceinline__ double ln_f32(double x)
+{
+    int e;
+    const double m = frexp(x, &e);
+    return (double(e) + double(__builtin_amdgcn_logf(float(m)))) * 0.69314718055994530942;
+}
 #endif
 
 }  // namespace hd

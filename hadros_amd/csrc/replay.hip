@@ -189,11 +189,12 @@ __device__ __forceinline__ bool stage_point(const DevRt& P, const ReplayLds& L, 
     } else {
         const bool absorbs = ok && pq.rho > 0.0 && pq.nbar_dl > 0.0;
         const bool emits = ok && P.spec_ok && pq.g3Adl > 0.0;
-        const double lnb = (pq.nbar_dl > 0.0) ? log_tab(pq.nbar_dl, L.log_t) : 0.0;
+        // the staged logarithms are rounded to float: ln_f32 is accurate enough
+        const double lnb = (pq.nbar_dl > 0.0) ? ln_f32(pq.nbar_dl) : 0.0;
         wsp[si] = absorbs ? pq.z : -pq.z;
         wsp[RP_STAGE + si] = double(float(pq.lnz));
         wsp[2 * RP_STAGE + si] = double(float(spow ? lnb + P.sig_s * pq.lnz : lnb));
-        wsp[3 * RP_STAGE + si] = emits ? double(float(log_tab(pq.g3Adl, L.log_t))) : double(-INFINITY);
+        wsp[3 * RP_STAGE + si] = emits ? double(float(ln_f32(pq.g3Adl))) : double(-INFINITY);
         return emits;
     }
 }
