@@ -148,7 +148,10 @@ __global__ void k_eval_transfer_point(DevRt P, const double* sig, const float* s
     const long k = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     PointQ pq;
-    const bool ok = point_q(P, r[k], th[k], dl[k], z[k], pq, ExpTab{exp_t, log_t});
+    // the transfer kernel's forms: FP64 tables, float-accurate in the FP32 form
+    bool ok;
+    if constexpr (PREC == HADROS_PREC_FP32K) ok = point_q(P, r[k], th[k], dl[k], z[k], pq, ExpF32{});
+    else ok = point_q(P, r[k], th[k], dl[k], z[k], pq, ExpTab{exp_t, log_t});
     if (!ok) {   // as the kernels: a skipped point adds nothing
         pq.z = 1.0;
         pq.lnz = 0.0;

@@ -648,6 +648,12 @@ struct ExpTab {
     __device__ double operator()(double x) const { return exp_tab(x, T); }
     __device__ double ln(double x) const { return log_tab(x, L); }
 };
+// Float-accurate forms (fastmath.h:exp_f32, ln_f32) for the FP32 transfer
+// form, whose per-point terms are rounded to float before they are used.
+struct ExpF32 {
+    __device__ double operator()(double x) const { return exp_f32(x); }
+    __device__ double ln(double x) const { return ln_f32(x); }
+};
 
 // x^y for x > 0 as exp(y ln x): relative error about |y ln x| ulp, 1e-15 for
 // the density radii (r / r0 < 1e3, |y| < 10), and a fraction of pow's cost

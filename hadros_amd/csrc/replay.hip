@@ -173,6 +173,17 @@ __device__ __forceinline__ void ray_begin(const DevRt& P, const ReplayLds& L, Re
 // of a point the reference skips, or of a lane without a point, holds the
 // neutral values (z 1, ln z 0, nothing absorbed or emitted).  Returns whether
 // the point emits.
+// The exponential and logarithm of a point's density, source and ln z: the
+// FP64 tables, or in the FP32 form (without the MeV channel, which keeps its
+// FP64 terms) the float-accurate forms, as every such term of the FP32 form is
+// rounded to float (its staged logarithms) before it is used.
+template <int PREC, bool MEV>
+__device__ __forceinline__ auto point_exp(const ReplayLds& L)
+{
+    if constexpr (PREC == HADROS_PREC_FP32K && !MEV) return ExpF32{};
+    else return ExpTab{L.exp_t, L.log_t};
+}
+
 template <int PREC>
 __device__ __forceinline__ bool stage_point(const DevRt& P, const ReplayLds& L, const PointQ& pq, bool ok, int si)
 {
@@ -472,7 +483,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 pr = A.r[q];
                 pth = A.th[q];
                 pdl = A.dl[q];
-                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq, ExpTab{L.exp_t, L.log_t});
+                ok = point_q(scene_per_chunk(Pc), pr, pth, pdl, A.z[q], pq, point_exp<PREC, MEV>(L));
             }
         } else {
             if (base + lane < b1) {
@@ -481,7 +492,7 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
                 pr = rt.x;
                 pth = rt.y;
                 pdl = dz.x;
-                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq, ExpTab{L.exp_t, L.log_t});
+                ok = point_q(scene_per_chunk(Pc), rt.x, rt.y, dz.x, dz.y, pq, point_exp<PREC, MEV>(L));
             }
             if (base + 64 < b1) chunk = A.next[chunk];
         }
@@ -532,7 +543,9 @@ __device__ __forceinline__ unsigned long long replay_ray(const ReplayArgs& A, De
 
 template <int PREC, int SRC, bool MEV>
 __global__ void __launch_bounds__(REPLAY_BLOCK)
-__attribute__((amdgpu_waves_per_eu(MEV ? 2 : HD_REPLAY_WAVES_PER_EU)))   // the MeV channel needs the registers
+// the MeV channel needs the registers; the FP32 form's float-accurate
+// per-point terms leave room for 4 waves per SIMD
+__attribute__((amdgpu_waves_per_eu(MEV ? 2 : (PREC == HADROS_PREC_FP32K ? 4 : HD_REPLAY_WAVES_PER_EU))))
 k_replay(ReplayArgs A)
 {
     extern __shared__ double smem[];
