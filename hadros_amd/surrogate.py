@@ -434,7 +434,8 @@ def scan(sur, params, res=256, spot_stride=10, spot_offset=5, tol=1e-3, group=25
             b.close()
         full = full.view(idx.size, npix, N_BINS)
         sel = fail_idx[c0:c0 + fallback_chunk]
-        P_img[sel] = full.float()
+        # whole images along dim 0 (advanced indexing would address every element)
+        P_img.index_copy_(0, sel, full.float())
         metric[sel] = _aggregate(full, wb).mean(dim=1)
         metric_se[sel] = 0.0
     torch.cuda.synchronize()
